@@ -1,0 +1,1251 @@
+// rsg_member.h -- the SIMPLE + SST k-omega outer iteration of ONE ensemble
+// member, executed by ONE workgroup.
+//
+// Design (MI355X): a member's mesh is small enough (10^2..10^4 cells) that a
+// grid-wide launch per phase would be launch- and sync-bound, so a whole
+// workgroup owns a member and runs its outer iterations back to back inside a
+// persistent launch; an ensemble is a grid of such workgroups.  Every phase is
+// a strided loop over cells or faces separated by workgroup barriers, every
+// reduction is a fixed-order block reduction (so a member's result never
+// depends on which other members share the launch), and every per-cell
+// accumulation is a gather over the cell's faces in ascending face order --
+// the order in which the reference's face-loop scatter adds contributions --
+// so assembly and Green-Gauss gradients round exactly like the reference.
+//
+// The same source also compiles as plain C++ with RSG_EMULATE defined: one
+// "thread" per block, barriers become no-ops and reductions are sequential.
+// That build exists only for the CPU test suite (tests/emu); the product
+// always runs the HIP build.
+//
+// Reference being restated (file:line in the reference repository):
+//   SIMPLESolver::solve            incompressible/src/SIMPLESolver.cpp:696-1043
+//   assembleMomentum               SIMPLESolver.cpp:56-203
+//   assemblePressureCorrection     SIMPLESolver.cpp:322-450
+//   correctVelocity/Pressure       SIMPLESolver.cpp:456-475
+//   assembleKEquation              SIMPLESolver.cpp:483-560
+//   assembleOmegaEquation          SIMPLESolver.cpp:567-669
+//   initUniform                    SIMPLESolver.cpp:23-51
+//   SSTModel::computeFields        core/src/SSTModel.cpp:122-172 (+ :9-96)
+//   greenGaussGrad, |S|            core/include/Field.hpp:122-152, :254-267
+//   transposeStressSource          core/include/StressOperators.hpp:31-84
+//   apply*BC                       core/include/BoundaryCondition.hpp:196-300
+//   BiCGSTAB / PCG / AMG-PCG       core/include/LinearSolver.hpp:147-247, :497-656
+#pragma once
+#include "rsg_layout.h"
+
+#ifdef RSG_EMULATE
+#include <math.h>
+#define RSG_D static inline
+#define RSG_SYNC() ((void)0)
+#else
+#include <hip/hip_runtime.h>
+#define RSG_D __device__ static __attribute__((always_inline)) inline
+#define RSG_SYNC() __syncthreads()
+#endif
+
+namespace rsg {
+
+// ------------------------------------------------------------ primitives
+RSG_D double dmax(double a, double b) { return (a < b) ? b : a; }   // std::max
+RSG_D double dmin(double a, double b) { return (b < a) ? b : a; }   // std::min
+
+struct Thr {
+    int tid;        // thread index in the workgroup
+    int bs;         // workgroup size
+    double* red;    // LDS reduction scratch: 2 phases x 16 waves x 4 values
+    double* lds;    // LDS tile for the coarsest-level dense inverse [64 x 64]
+    int phase;
+};
+
+#ifdef RSG_EMULATE
+template <int NV> RSG_D void bsum(Thr&, double (&)[NV]) {}
+template <int NV> RSG_D void bmax(Thr&, double (&)[NV]) {}
+RSG_D double wave_bcast0(double v) { return v; }
+#else
+RSG_D double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+RSG_D double wmax(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+// Block sum of NV values, result identical in every thread: lane 0 of each
+// wave deposits its wave total, then every thread adds the wave totals in
+// wave order.  Double-buffered LDS so consecutive calls need one barrier.
+template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
+    const int lane = t.tid & 63, wid = t.tid >> 6, nw = (t.bs + 63) >> 6;
+    double* buf = t.red + t.phase * 64;
+    t.phase ^= 1;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wsum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) buf[wid * NV + i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += buf[w * NV + i];
+        v[i] = s;
+    }
+}
+template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
+    const int lane = t.tid & 63, wid = t.tid >> 6, nw = (t.bs + 63) >> 6;
+    double* buf = t.red + t.phase * 64;
+    t.phase ^= 1;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wmax(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) buf[wid * NV + i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = buf[i];
+        for (int w = 1; w < nw; ++w) s = dmax(s, buf[w * NV + i]);
+        v[i] = s;
+    }
+}
+#endif
+
+#define FOR_CELLS(c) for (int c = t.tid; c < nC; c += t.bs)
+#define FOR_IFACES(f) for (int f = t.tid; f < nIF; f += t.bs)
+#define FOR_BFACES(b) for (int b = t.tid; b < nB; b += t.bs)
+
+// Everything a phase needs, hoisted once per launch.
+struct Ctx {
+    Thr t;
+    const RsgMeshDev* m;
+    const RsgBcDev* bc;
+    const RsgSettingsDev* s;
+    RsgMemberDev* M;
+    int nC, nIF, nB;
+    double nu;
+    // SST coefficients, InferenceParameters.hpp:12-14 order
+    double sk1, sw1, b1, al1, sk2, sw2, b2, al2, bs, a1, kap;
+};
+
+RSG_D double blend(double p1, double p2, double F1) { return F1 * p1 + (1.0 - F1) * p2; }
+
+// ------------------------------------------------------- Green-Gauss (gather)
+// greenGaussGrad Field.hpp:122-152 for NF fields at once, cell c.
+template <int NF>
+RSG_D void gg_cell(const RsgMeshDev* m, int c, const double* const (&phi)[NF],
+                   const double* const (&phib)[NF], double (&g)[NF][3]) {
+    const int nIF = m->nIF;
+#pragma unroll
+    for (int q = 0; q < NF; ++q) g[q][0] = g[q][1] = g[q][2] = 0.0;
+    for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+        const int code = m->cfIdx[e];
+        const int f = code >> 1;
+        const double sx = m->sfx[f], sy = m->sfy[f], sz = m->sfz[f];
+        if (f < nIF) {
+            const int o = m->own[f], n = m->nbr[f];
+            const double w = m->w[f], wm = 1.0 - w;
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                const double pf = phi[q][o] * w + phi[q][n] * wm;
+                const double ax = sx * pf, ay = sy * pf, az = sz * pf;
+                if (code & 1) { g[q][0] = g[q][0] - ax; g[q][1] = g[q][1] - ay; g[q][2] = g[q][2] - az; }
+                else          { g[q][0] = g[q][0] + ax; g[q][1] = g[q][1] + ay; g[q][2] = g[q][2] + az; }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                const double pb = phib[q][f - nIF];
+                g[q][0] = g[q][0] + sx * pb; g[q][1] = g[q][1] + sy * pb; g[q][2] = g[q][2] + sz * pb;
+            }
+        }
+    }
+    const double V = m->vol[c];
+    if (V > 1e-30) {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) { g[q][0] = g[q][0] / V; g[q][1] = g[q][1] / V; g[q][2] = g[q][2] / V; }
+    }
+}
+
+// ------------------------------------------------------------------ SST
+// SSTModel.cpp:9-26 computeF1
+RSG_D double sst_F1(const Ctx& C, double k, double om, double y, double nu, double CDin) {
+    double sqrtK = sqrt(dmax(k, 0.0));
+    double omSafe = dmax(om, 1e-20);
+    double ySafe = dmax(y, 1e-20);
+    double y2s = ySafe * ySafe;
+    double term1 = sqrtK / (C.bs * omSafe * ySafe);
+    double term2 = 500.0 * nu / (y2s * omSafe);
+    double CDpos = dmax(CDin, 1e-20);
+    double term3 = 4.0 * C.sw2 * dmax(k, 0.0) / (CDpos * y2s);
+    double arg1 = dmin(dmax(term1, term2), term3);
+    double a4 = arg1 * arg1 * arg1 * arg1;
+    return tanh(a4);
+}
+// SSTModel.cpp:30-44 computeF2
+RSG_D double sst_F2(const Ctx& C, double k, double om, double y, double nu) {
+    double sqrtK = sqrt(dmax(k, 0.0));
+    double omSafe = dmax(om, 1e-20);
+    double ySafe = dmax(y, 1e-20);
+    double y2 = ySafe * ySafe;
+    double term1 = 2.0 * sqrtK / (C.bs * omSafe * ySafe);
+    double term2 = 500.0 * nu / (y2 * omSafe);
+    double arg2 = dmax(term1, term2);
+    return tanh(arg2 * arg2);
+}
+// SSTModel.cpp:57-65 eddyViscosity
+RSG_D double sst_nut(const Ctx& C, double k, double om, double S, double F2) {
+    double kSafe = dmax(k, 0.0);
+    if (C.s->variant == 1) return kSafe / dmax(om, 1e-20);
+    double denom = dmax(C.a1 * om, S * F2);
+    denom = dmax(denom, 1e-20);
+    return C.a1 * kSafe / denom;
+}
+// SSTModel.cpp:70-74 production
+RSG_D double sst_prod(const Ctx& C, double nuT, double S, double k, double om) {
+    double raw = nuT * S * S;
+    double lim = 10.0 * C.bs * dmax(k, 0.0) * dmax(om, 1e-20);
+    return dmin(raw, lim);
+}
+// SSTModel.cpp:91-96 productionOmega
+RSG_D double sst_prod_om(const Ctx& C, double nuT, double S, double k, double om) {
+    double S2 = S * S;
+    double lim = 10.0 * C.bs * dmax(k, 0.0) * dmax(om, 1e-20) / dmax(nuT, 1e-30);
+    return dmin(S2, lim);
+}
+
+// --------------------------------------------------------------- BCs
+// applyVelocityBC BoundaryCondition.hpp:196-225 (per-face types; profiles
+// resolved by the host into velV)
+RSG_D void vel_bc(Ctx& C) {
+    const RsgMeshDev* m = C.m; const RsgBcDev* bc = C.bc; RsgMemberDev* M = C.M;
+    Thr& t = C.t; const int nB = C.nB, nIF = C.nIF;
+    FOR_BFACES(b) {
+        const int f = nIF + b, o = m->own[f];
+        switch (bc->velT[b]) {
+            case RSGBC_WALL_NOSLIP: M->Ubx[b] = 0.0; M->Uby[b] = 0.0; M->Ubz[b] = 0.0; break;
+            case RSGBC_WALL_MOVING:
+            case RSGBC_INLET_VELOCITY:
+            case RSGBC_DIRICHLET:
+                M->Ubx[b] = bc->velV[3 * b]; M->Uby[b] = bc->velV[3 * b + 1]; M->Ubz[b] = bc->velV[3 * b + 2];
+                break;
+            case RSGBC_SYMMETRY: {
+                const double ux = M->Ux[o], uy = M->Uy[o], uz = M->Uz[o];
+                const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f];
+                const double d = ux * nx + uy * ny + uz * nz;
+                M->Ubx[b] = ux - nx * d; M->Uby[b] = uy - ny * d; M->Ubz[b] = uz - nz * d;
+                break;
+            }
+            case RSGBC_KEEP: break;   // face in no patch: keeps its value
+            default: M->Ubx[b] = M->Ux[o]; M->Uby[b] = M->Uy[o]; M->Ubz[b] = M->Uz[o]; break;
+        }
+    }
+}
+// applyPressureBC :229-245
+RSG_D void p_bc(Ctx& C) {
+    const RsgMeshDev* m = C.m; const RsgBcDev* bc = C.bc; RsgMemberDev* M = C.M;
+    Thr& t = C.t; const int nB = C.nB, nIF = C.nIF;
+    FOR_BFACES(b) {
+        const int ty = bc->pT[b];
+        if (ty == RSGBC_KEEP) continue;
+        M->pb[b] = (ty == RSGBC_OUTLET_PRESSURE || ty == RSGBC_DIRICHLET) ? bc->pV[b] : M->p[m->own[nIF + b]];
+    }
+}
+// applyKBC :249-269
+RSG_D void k_bc(Ctx& C) {
+    const RsgMeshDev* m = C.m; const RsgBcDev* bc = C.bc; RsgMemberDev* M = C.M;
+    Thr& t = C.t; const int nB = C.nB, nIF = C.nIF;
+    FOR_BFACES(b) {
+        const int ty = bc->kT[b];
+        if (ty == RSGBC_KEEP) continue;
+        M->kb[b] = ty == RSGBC_WALL_KOMEGA ? 0.0 : ty == RSGBC_DIRICHLET ? bc->kV[b] : M->k[m->own[nIF + b]];
+    }
+}
+// applyOmegaBC :275-300
+RSG_D void om_bc(Ctx& C, double beta1) {
+    const RsgMeshDev* m = C.m; const RsgBcDev* bc = C.bc; RsgMemberDev* M = C.M;
+    Thr& t = C.t; const int nB = C.nB, nIF = C.nIF;
+    FOR_BFACES(b) {
+        const int ty = bc->omT[b], f = nIF + b;
+        if (ty == RSGBC_KEEP) continue;
+        if (ty == RSGBC_WALL_KOMEGA) {
+            const double y1 = m->dlt[f];
+            M->omb[b] = 60.0 * C.nu / (beta1 * y1 * y1);
+        } else if (ty == RSGBC_DIRICHLET) {
+            M->omb[b] = bc->omV[b];
+        } else {
+            M->omb[b] = M->om[m->own[f]];
+        }
+    }
+}
+
+// ------------------------------------------------------- LDU operations
+// y = A x (LinearSystem::matvec LinearSolver.hpp:68-77) for cell c, gathering
+// the off-diagonal terms in ascending face order.
+RSG_D double ldu_row(const RsgMeshDev* m, const double* diag, const double* upper,
+                     const double* lower, const double* x, int c) {
+    double y = diag[c] * x[c];
+    const int nIF = m->nIF;
+    for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+        const int code = m->cfIdx[e];
+        const int f = code >> 1;
+        if (f >= nIF) break;   // boundary faces sort after every internal face
+        if (code & 1) y += lower[f] * x[m->own[f]];
+        else          y += upper[f] * x[m->nbr[f]];
+    }
+    return y;
+}
+
+struct SolveRes { int iterations; double finalRes, initialRes; int converged; };
+
+RSG_D double jac(double r, double d) { return (fabs(d) > 1e-30) ? r / d : r; }
+
+// BiCGSTABSolver::solve LinearSolver.hpp:193-245 (Jacobi right preconditioning)
+RSG_D SolveRes bicgstab(Ctx& C, double* x, int maxIter, double tol) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const int nC = C.nC;
+    const double *dg = M->diag, *up = M->upper, *lo = M->lower, *b = M->src;
+    double *r = M->kv[0], *rh = M->kv[1], *p = M->kv[2], *v = M->kv[3];
+    double *s = M->kv[4], *tt_ = M->kv[5], *ph = M->kv[6], *sh = M->kv[7];
+    SolveRes res = {0, 0.0, 0.0, 0};
+    double acc[1] = {0.0};
+    FOR_CELLS(c) {
+        const double rc = b[c] - ldu_row(m, dg, up, lo, x, c);
+        r[c] = rc; rh[c] = rc; p[c] = 0.0; v[c] = 0.0;
+        acc[0] += rc * rc;
+    }
+    bsum(t, acc);
+    const double r0 = sqrt(acc[0]);
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return res; }
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    double rhoN = acc[0];   // dot(rh, r) with rh == r: the same sum
+    for (int it = 0; it < maxIter; ++it) {
+        if (fabs(rhoN) < 1e-30) break;
+        const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
+        FOR_CELLS(c) {
+            const double pc = r[c] + beta * (p[c] - omega * v[c]);
+            p[c] = pc;
+            ph[c] = jac(pc, dg[c]);
+        }
+        RSG_SYNC();
+        double a1[1] = {0.0};
+        FOR_CELLS(c) {
+            const double vc = ldu_row(m, dg, up, lo, ph, c);
+            v[c] = vc;
+            a1[0] += rh[c] * vc;
+        }
+        bsum(t, a1);
+        const double rv = a1[0];
+        if (fabs(rv) < 1e-30) break;
+        alpha = rhoN / rv;
+        double a2[1] = {0.0};
+        FOR_CELLS(c) {
+            const double sc = r[c] - alpha * v[c];
+            s[c] = sc;
+            sh[c] = jac(sc, dg[c]);
+            a2[0] += sc * sc;
+        }
+        bsum(t, a2);
+        const double sn = sqrt(a2[0]);
+        if (sn / r0 < tol) {
+            FOR_CELLS(c) x[c] += alpha * ph[c];
+            res.iterations = it + 1; res.finalRes = sn / r0; res.initialRes = r0; res.converged = 1;
+            return res;
+        }
+        double a3[2] = {0.0, 0.0};
+        FOR_CELLS(c) {
+            const double tc = ldu_row(m, dg, up, lo, sh, c);
+            tt_[c] = tc;
+            a3[0] += tc * tc;
+            a3[1] += tc * s[c];
+        }
+        bsum(t, a3);
+        omega = (a3[0] > 1e-30) ? a3[1] / a3[0] : 0.0;
+        double a4[2] = {0.0, 0.0};
+        FOR_CELLS(c) {
+            x[c] += alpha * ph[c] + omega * sh[c];
+            const double rc = s[c] - omega * tt_[c];
+            r[c] = rc;
+            a4[0] += rc * rc;
+            a4[1] += rh[c] * rc;
+        }
+        bsum(t, a4);
+        const double rn = sqrt(a4[0]);
+        res.iterations = it + 1;
+        res.finalRes = rn / r0;
+        if (res.finalRes < tol) { res.converged = 1; return res; }
+        if (fabs(omega) < 1e-30) break;
+        rho = rhoN;
+        rhoN = a4[1];
+    }
+    return res;
+}
+
+// ------------------------------------------------------------------ AMG
+// Serial hierarchy construction, run by thread 0 while the workgroup waits.
+// Aggregation is the reference's greedy pass bit for bit (buildAggregates
+// LinearSolver.hpp:407-443); the Galerkin pattern and the gather lists
+// reproduce galerkinProduct's summation order (:446-485: coarse entry (I,J)
+// sums fine entries in ascending fine row, then ascending entry).
+RSG_D int amg_aggregate(int n, const int* rp, const int* ci, const double* v, const double* dg, int* map) {
+    int nc = 0;
+    for (int i = 0; i < n; ++i) map[i] = -1;
+    for (int i = 0; i < n; ++i) {
+        if (map[i] >= 0) continue;
+        const int agg = nc++;
+        map[i] = agg;
+        const double aii = fabs(dg[i]);
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int c = ci[j];
+            if (c == i || map[c] >= 0) continue;
+            const double ajj = fabs(dg[c]);
+            const double strength = fabs(v[j]) / sqrt(aii * ajj + 1e-30);
+            if (strength > 0.25) map[c] = agg;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (map[i] >= 0) continue;
+        double best = -1.0;
+        int bestAgg = 0;
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int c = ci[j];
+            if (c == i && map[c] < 0) continue;
+            if (map[c] >= 0 && fabs(v[j]) > best) { best = fabs(v[j]); bestAgg = map[c]; }
+        }
+        map[i] = bestAgg;
+    }
+    return nc;
+}
+
+// Builds levels 1.. from level 0 (whose values must be current).
+RSG_D void amg_setup_serial(const RsgMeshDev* m, RsgAmgDev& A) {
+    int L = 1;
+    for (int l = 0; l < RSG_MAX_LEVELS; ++l) {
+        const int n = A.lvN[l];
+        if (n <= RSG_MIN_COARSE) break;
+        const int* rp = A.rp + A.lvRp[l];
+        const int* ci = A.ci + A.lvNz[l];
+        const double* v = A.v + A.lvNz[l];
+        const double* dg = A.dg + A.lvRow[l];
+        int* map = A.map + A.lvRow[l];
+        const int nnz = rp[n];
+        const int nc = amg_aggregate(n, rp, ci, v, dg, map);
+        if (nc == 0 || nc >= n) break;
+        // capacity of the next level (coarse entries never exceed fine ones)
+        const int rowNext = A.lvRow[l] + n;
+        const int nzNext = A.lvNz[l] + nnz;
+        if (rowNext + nc > A.rowCap || nzNext + nnz > A.nnzCap) break;
+        A.lvRow[l + 1] = rowNext;
+        A.lvRp[l + 1] = rowNext + (l + 1);
+        A.lvNz[l + 1] = nzNext;
+        A.lvN[l + 1] = nc;
+        // aggregate -> fine rows (ascending): counting sort of map
+        int* aP = A.aggPtr + A.lvRp[l + 1];
+        int* aI = A.aggIdx + A.lvRow[l];
+        for (int I = 0; I <= nc; ++I) aP[I] = 0;
+        for (int i = 0; i < n; ++i) aP[map[i] + 1]++;
+        for (int I = 0; I < nc; ++I) aP[I + 1] += aP[I];
+        int* fill = A.scr;                  // [nc]
+        for (int I = 0; I < nc; ++I) fill[I] = aP[I];
+        for (int i = 0; i < n; ++i) aI[fill[map[i]]++] = i;
+        // coarse pattern, row by row
+        int* mark = A.scr;                  // [nc] (reuses fill)
+        int* pos = A.scr + A.rowCap;        // [nc]
+        int* cols = A.scr + 2 * A.rowCap;   // [row length <= nc]
+        int* f2c = A.scr + 3 * A.rowCap;    // [nnz] fine entry -> coarse entry
+        for (int J = 0; J < nc; ++J) mark[J] = -1;
+        int* crp = A.rp + A.lvRp[l + 1];
+        int* cci = A.ci + A.lvNz[l + 1];
+        int cnz = 0;
+        for (int I = 0; I < nc; ++I) {
+            int ncol = 0;
+            for (int q = aP[I]; q < aP[I + 1]; ++q) {
+                const int i = aI[q];
+                for (int j = rp[i]; j < rp[i + 1]; ++j) {
+                    const int J = map[ci[j]];
+                    if (mark[J] != I) { mark[J] = I; cols[ncol++] = J; }
+                }
+            }
+            for (int a = 1; a < ncol; ++a) {
+                const int key = cols[a];
+                int b = a - 1;
+                while (b >= 0 && cols[b] > key) { cols[b + 1] = cols[b]; --b; }
+                cols[b + 1] = key;
+            }
+            crp[I] = cnz;
+            for (int a = 0; a < ncol; ++a) { pos[cols[a]] = cnz + a; cci[cnz + a] = cols[a]; }
+            for (int q = aP[I]; q < aP[I + 1]; ++q) {
+                const int i = aI[q];
+                for (int j = rp[i]; j < rp[i + 1]; ++j) f2c[j] = pos[map[ci[j]]];
+            }
+            cnz += ncol;
+        }
+        crp[nc] = cnz;
+        // coarse entry -> fine entries, stable in fine entry order
+        int* gP = A.gPtr + A.lvNz[l + 1] + (l + 1);
+        int* gI = A.gIdx + A.lvNz[l];
+        for (int e = 0; e <= cnz; ++e) gP[e] = 0;
+        for (int j = 0; j < nnz; ++j) gP[f2c[j] + 1]++;
+        for (int e = 0; e < cnz; ++e) gP[e + 1] += gP[e];
+        int* gfill = A.scr + 3 * A.rowCap + A.nnzCap;   // [cnz]
+        for (int e = 0; e < cnz; ++e) gfill[e] = gP[e];
+        for (int j = 0; j < nnz; ++j) gI[gfill[f2c[j]]++] = j;
+        // coarse values (serially here; refreshed in parallel every iteration)
+        double* cv = A.v + A.lvNz[l + 1];
+        double* cdg = A.dg + A.lvRow[l + 1];
+        for (int I = 0; I < nc; ++I) cdg[I] = 0.0;
+        for (int I = 0; I < nc; ++I)
+            for (int e = crp[I]; e < crp[I + 1]; ++e) {
+                double s = 0.0;
+                for (int q = gP[e]; q < gP[e + 1]; ++q) s += v[gI[q]];
+                cv[e] = s;
+                if (cci[e] == I) cdg[I] = s;
+            }
+        L = l + 2;
+    }
+    A.lvN[RSG_MAX_LEVELS] = L;   // level count
+}
+
+// Parallel refresh of every level's values from the current LDU matrix.
+RSG_D void amg_refresh_values(Ctx& C) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
+    RsgAmgDev& A = M->amg;
+    const int nC = C.nC, nIF = C.nIF, nnz0 = m->nnz0;
+    for (int e = t.tid; e < nnz0; e += t.bs) {
+        const int s = m->src0[e];
+        A.v[e] = s < 0 ? M->diag[-1 - s] : (s < nIF ? M->upper[s] : M->lower[s - nIF]);
+    }
+    FOR_CELLS(c) A.dg[c] = M->diag[c];
+    RSG_SYNC();
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    for (int l = 0; l + 1 < L; ++l) {
+        const int nc = A.lvN[l + 1];
+        const int* crp = A.rp + A.lvRp[l + 1];
+        const int* cci = A.ci + A.lvNz[l + 1];
+        const int* gP = A.gPtr + A.lvNz[l + 1] + (l + 1);
+        const int* gI = A.gIdx + A.lvNz[l];
+        const double* fv = A.v + A.lvNz[l];
+        double* cv = A.v + A.lvNz[l + 1];
+        double* cdg = A.dg + A.lvRow[l + 1];
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double d = 0.0;
+            for (int e = crp[I]; e < crp[I + 1]; ++e) {
+                double s = 0.0;
+                for (int q = gP[e]; q < gP[e + 1]; ++q) s += fv[gI[q]];
+                cv[e] = s;
+                if (cci[e] == I) d = s;
+            }
+            cdg[I] = d;
+        }
+        RSG_SYNC();
+    }
+}
+
+// Dense inverse of the coarsest level (Gauss-Jordan, no pivoting: the level
+// is a Galerkin image of an SPD M-matrix).  Built in LDS by wave 0, lane j
+// owning column j; LDS accesses of one wave execute in program order, so the
+// in-place elimination needs no workgroup barrier.  The inverse is written
+// to global memory for the V-cycle's coarse solve.
+RSG_D void amg_coarse_inverse(Ctx& C) {
+    Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg;
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    const int n = A.lvN[L - 1];
+    const int DN = RSG_COARSE_DENSE;
+    if (n > DN) { if (t.tid == 0) M->sc->coarseDense = 0; RSG_SYNC(); return; }
+    const int* rp = A.rp + A.lvRp[L - 1];
+    const int* ci = A.ci + A.lvNz[L - 1];
+    const double* v = A.v + A.lvNz[L - 1];
+    double* G = t.lds;
+    for (int idx = t.tid; idx < n * DN; idx += t.bs) G[idx] = 0.0;
+    RSG_SYNC();
+    for (int i = t.tid; i < n; i += t.bs)
+        for (int e = rp[i]; e < rp[i + 1]; ++e) G[i * DN + ci[e]] += v[e];
+    RSG_SYNC();
+    int ok = 1;
+#ifdef RSG_EMULATE
+    for (int k = 0; k < n; ++k) {
+        const double piv = G[k * DN + k];
+        if (!(fabs(piv) > 1e-300)) { ok = 0; break; }
+        for (int j = 0; j < n; ++j) G[k * DN + j] = (j == k ? 1.0 : G[k * DN + j]) / piv;
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const double fct = G[i * DN + k];
+            for (int j = 0; j < n; ++j) G[i * DN + j] = (j == k ? 0.0 : G[i * DN + j]) - fct * G[k * DN + j];
+        }
+    }
+#else
+    if (t.tid < 64) {
+        const int j = t.tid;
+        for (int k = 0; k < n; ++k) {
+            const double piv = G[k * DN + k];
+            if (!(fabs(piv) > 1e-300)) { ok = 0; break; }
+            const double rk = (j < n) ? (j == k ? 1.0 : G[k * DN + j]) / piv : 0.0;
+            for (int i = 0; i < n; ++i) {
+                if (i == k) continue;
+                const double fct = G[i * DN + k];
+                __builtin_amdgcn_wave_barrier();
+                if (j < n) G[i * DN + j] = (j == k ? 0.0 : G[i * DN + j]) - fct * rk;
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (j < n) G[k * DN + j] = rk;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+#endif
+    RSG_SYNC();
+    for (int idx = t.tid; idx < n * DN; idx += t.bs) A.cinv[idx] = G[idx];
+    if (t.tid == 0) M->sc->coarseDense = ok;
+    RSG_SYNC();
+}
+
+// V-cycle preconditioner z = M^{-1} r (AMGSolver::vcycle :553-589 with
+// damped-Jacobi smoothing in place of the sequential Gauss-Seidel sweep, and
+// an exact dense coarsest solve in place of 50 Gauss-Seidel sweeps).
+RSG_D void csr_jacobi(Thr& t, int n, const int* rp, const int* ci, const double* v, const double* dg,
+                      const double* b, const double* xin, double* xout, bool xzero) {
+    const double wJ = 2.0 / 3.0;
+    for (int i = t.tid; i < n; i += t.bs) {
+        double s = b[i];
+        if (!xzero)
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xin[ci[e]];
+        const double d = dg[i];
+        const double x0 = xzero ? 0.0 : xin[i];
+        xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+    }
+}
+
+RSG_D void amg_vcycle(Ctx& C, const double* rin, double* z) {
+    Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg;
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    // level 0 right-hand side
+    {
+        double* b0 = A.b;
+        const int n0 = A.lvN[0];
+        for (int i = t.tid; i < n0; i += t.bs) b0[i] = rin[i];
+    }
+    RSG_SYNC();
+    for (int l = 0; l + 1 < L; ++l) {
+        const int n = A.lvN[l];
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
+        const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
+        double* x = A.x + A.lvRow[l]; double* b = A.b + A.lvRow[l]; double* r = A.r + A.lvRow[l];
+        csr_jacobi(t, n, rp, ci, v, dg, b, x, r, true);
+        RSG_SYNC();
+        csr_jacobi(t, n, rp, ci, v, dg, b, r, x, false);
+        RSG_SYNC();
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = 0.0;
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+            r[i] = b[i] - s;
+        }
+        RSG_SYNC();
+        const int nc = A.lvN[l + 1];
+        const int* aP = A.aggPtr + A.lvRp[l + 1];
+        const int* aI = A.aggIdx + A.lvRow[l];
+        double* cb = A.b + A.lvRow[l + 1];
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
+            cb[I] = s;
+        }
+        RSG_SYNC();
+    }
+    {   // coarsest level
+        const int l = L - 1, n = A.lvN[l];
+        double* x = A.x + A.lvRow[l]; const double* b = A.b + A.lvRow[l];
+        if (M->sc->coarseDense) {
+            const int DN = RSG_COARSE_DENSE;
+            for (int i = t.tid; i < n; i += t.bs) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j) s += A.cinv[i * DN + j] * b[j];
+                x[i] = s;
+            }
+            RSG_SYNC();
+        } else {
+            const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
+            const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
+            double* r = A.r + A.lvRow[l];
+            csr_jacobi(t, n, rp, ci, v, dg, b, x, x, true);
+            RSG_SYNC();
+            for (int s = 0; s < 25; ++s) {
+                csr_jacobi(t, n, rp, ci, v, dg, b, x, r, false);
+                RSG_SYNC();
+                csr_jacobi(t, n, rp, ci, v, dg, b, r, x, false);
+                RSG_SYNC();
+            }
+        }
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        const int n = A.lvN[l];
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
+        const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
+        double* x = A.x + A.lvRow[l]; double* b = A.b + A.lvRow[l]; double* r = A.r + A.lvRow[l];
+        const int* map = A.map + A.lvRow[l];
+        const double* cx = A.x + A.lvRow[l + 1];
+        for (int i = t.tid; i < n; i += t.bs) x[i] += cx[map[i]];
+        RSG_SYNC();
+        csr_jacobi(t, n, rp, ci, v, dg, b, x, r, false);
+        RSG_SYNC();
+        double* out = (l == 0) ? z : x;
+        csr_jacobi(t, n, rp, ci, v, dg, b, r, out, false);
+        RSG_SYNC();
+    }
+    if (L == 1) {
+        const int n = A.lvN[0];
+        for (int i = t.tid; i < n; i += t.bs) z[i] = A.x[i];
+        RSG_SYNC();
+    }
+}
+
+// PCG (:147-186) with Jacobi or AMG V-cycle preconditioning (:602-653).
+RSG_D SolveRes pcg(Ctx& C, double* x, int maxIter, double tol, bool useAmg) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const int nC = C.nC;
+    const double *dg = M->diag, *up = M->upper, *lo = M->lower, *b = M->src;
+    double *r = M->kv[0], *z = M->kv[1], *p = M->kv[2], *Ap = M->kv[3];
+    SolveRes res = {0, 0.0, 0.0, 0};
+    double acc[1] = {0.0};
+    FOR_CELLS(c) {
+        const double rc = b[c] - ldu_row(m, dg, up, lo, x, c);
+        r[c] = rc;
+        acc[0] += rc * rc;
+    }
+    bsum(t, acc);
+    const double r0 = sqrt(acc[0]);
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return res; }
+    double rzv[1] = {0.0};
+    if (useAmg) {
+        amg_vcycle(C, r, z);
+        FOR_CELLS(c) { p[c] = z[c]; rzv[0] += r[c] * z[c]; }
+    } else {
+        FOR_CELLS(c) { const double zc = jac(r[c], dg[c]); z[c] = zc; p[c] = zc; rzv[0] += r[c] * zc; }
+    }
+    bsum(t, rzv);
+    double rz = rzv[0];
+    for (int it = 0; it < maxIter; ++it) {
+        double a1[1] = {0.0};
+        FOR_CELLS(c) { const double q = ldu_row(m, dg, up, lo, p, c); Ap[c] = q; a1[0] += p[c] * q; }
+        bsum(t, a1);
+        const double pAp = a1[0];
+        if (fabs(pAp) < 1e-30) break;
+        const double alpha = rz / pAp;
+        double a2[1] = {0.0};
+        FOR_CELLS(c) {
+            x[c] += alpha * p[c];
+            const double rc = r[c] - alpha * Ap[c];
+            r[c] = rc;
+            a2[0] += rc * rc;
+        }
+        bsum(t, a2);
+        const double rn = sqrt(a2[0]);
+        res.iterations = it + 1;
+        res.finalRes = rn / r0;
+        if (res.finalRes < tol) { res.converged = 1; return res; }
+        double a3[1] = {0.0};
+        if (useAmg) {
+            amg_vcycle(C, r, z);
+            FOR_CELLS(c) a3[0] += r[c] * z[c];
+        } else {
+            FOR_CELLS(c) { const double zc = jac(r[c], dg[c]); z[c] = zc; a3[0] += r[c] * zc; }
+        }
+        bsum(t, a3);
+        const double rzNew = a3[0];
+        const double beta = rzNew / (rz + 1e-30);
+        FOR_CELLS(c) p[c] = z[c] + beta * p[c];
+        rz = rzNew;
+        RSG_SYNC();
+    }
+    return res;
+}
+
+RSG_D SolveRes lsolve(Ctx& C, int code, double* x, bool amgReady) {
+    const RsgSettingsDev* s = C.s;
+    if (code == 0) return pcg(C, x, s->innerIter, s->innerTol, amgReady);
+    if (code == 1) return pcg(C, x, s->innerIter, s->innerTol, false);
+    return bicgstab(C, x, s->innerIter, s->innerTol);
+}
+
+// ------------------------------------------------------------ assembly
+RSG_D double* ucomp(RsgMemberDev* M, int comp) { return comp == 0 ? M->Ux : comp == 1 ? M->Uy : M->Uz; }
+RSG_D double* ubcomp(RsgMemberDev* M, int comp) { return comp == 0 ? M->Ubx : comp == 1 ? M->Uby : M->Ubz; }
+
+// Convection-diffusion face coefficients shared by the momentum, k and omega
+// assemblies: upper = -(Df + cNeg), lower = -(Df + cPos).
+RSG_D void conv_diff_face(const RsgMeshDev* m, const RsgMemberDev* M, int f, double nuEff_o,
+                          double nuEff_n, double& up, double& lo) {
+    const int o = m->own[f], n = m->nbr[f];
+    const double Sf = m->area[f], delta = m->dlt[f], w = m->w[f], wm = 1.0 - w;
+    const double nuEff_f = w * nuEff_o + (1.0 - w) * nuEff_n;
+    const double Df = nuEff_f * Sf / delta;
+    const double ufx = M->Ux[o] * w + M->Ux[n] * wm;
+    const double ufy = M->Uy[o] * w + M->Uy[n] * wm;
+    const double ufz = M->Uz[o] * w + M->Uz[n] * wm;
+    const double mFlux = (ufx * m->nx[f] + ufy * m->ny[f] + ufz * m->nz[f]) * Sf;
+    const double cPos = dmax(mFlux, 0.0), cNeg = dmax(-mFlux, 0.0);
+    up = -(Df + cNeg);
+    lo = -(Df + cPos);
+}
+
+// boundary-face convection-diffusion of a scalar with face value phib
+// (outflow / inflow branches, SIMPLESolver.cpp:132-140)
+RSG_D void conv_diff_bface(const RsgMeshDev* m, const RsgMemberDev* M, int f, int b, double nuEff,
+                           double phib, double& dg, double& src) {
+    const double Sf = m->area[f], delta = m->dlt[f];
+    const double Db = nuEff * Sf / delta;
+    const double mFlux = (M->Ubx[b] * m->nx[f] + M->Uby[b] * m->ny[f] + M->Ubz[b] * m->nz[f]) * Sf;
+    if (mFlux >= 0) { dg += mFlux + Db; src += Db * phib; }
+    else { dg += Db; src += (Db - mFlux) * phib; }
+}
+
+// assembleMomentum SIMPLESolver.cpp:56-203 for one component.  Requires
+// current grad U in M->g[0..8] (transpose stress) and grad p in M->gp.
+RSG_D void assemble_momentum(Ctx& C, int comp) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
+    const int nC = C.nC, nIF = C.nIF;
+    const double nu = C.nu;
+    // column `comp` of grad U: (dUx/dx_comp, dUy/dx_comp, dUz/dx_comp)
+    const double* gc0 = M->g[0 * 3 + comp];
+    const double* gc1 = M->g[1 * 3 + comp];
+    const double* gc2 = M->g[2 * 3 + comp];
+    FOR_IFACES(f) {
+        const int o = m->own[f], n = m->nbr[f];
+        double up, lo;
+        conv_diff_face(m, M, f, nu + M->nuT[o], nu + M->nuT[n], up, lo);
+        M->upper[f] = up; M->lower[f] = lo;
+        // transpose-stress face flux, StressOperators.hpp:50-59
+        const double w = m->w[f], wm = 1.0 - w;
+        const double cf = w * M->nuT[o] + (1.0 - w) * M->nuT[n];
+        const double gx = gc0[o] * w + gc0[n] * wm;
+        const double gy = gc1[o] * w + gc1[n] * wm;
+        const double gz = gc2[o] * w + gc2[n] * wm;
+        M->fscr[f] = cf * (gx * m->nx[f] + gy * m->ny[f] + gz * m->nz[f]) * m->area[f];
+    }
+    RSG_SYNC();
+    const double* phiOld = ucomp(M, comp);
+    const double* Ub = ubcomp(M, comp);
+    const double fb = C.s->bf[comp];
+    const double a = C.s->aU;
+    const int wallFn = C.s->wallFn;
+    const double* gpc = M->gp[comp];
+    FOR_CELLS(c) {
+        double dg = 0.0, src = 0.0, ts = 0.0;
+        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+            const int code = m->cfIdx[e], f = code >> 1;
+            if (f < nIF) {
+                if (code & 1) { dg += -M->upper[f]; ts -= M->fscr[f]; }
+                else          { dg += -M->lower[f]; ts += M->fscr[f]; }
+            } else {
+                const int b = f - nIF;
+                const int isWall = m->bFlags[b] & RSG_BF_WALL;
+                const double nuEff = (isWall && !wallFn) ? nu : nu + M->nuT[c];
+                conv_diff_bface(m, M, f, b, nuEff, Ub[b], dg, src);
+                // transpose stress boundary term with the wall coefficient zeroed
+                const double cfb = isWall ? 0.0 : M->nuT[c];
+                ts += cfb * (gc0[c] * m->nx[f] + gc1[c] * m->ny[f] + gc2[c] * m->nz[f]) * m->area[f];
+            }
+        }
+        const double vol = m->vol[c];
+        src -= gpc[c] * vol;
+        if (fb != 0.0) src += fb * vol;
+        src += ts;
+        src += (1.0 - a) / a * dg * phiOld[c];
+        dg /= a;
+        M->diag[c] = dg;
+        M->src[c] = src;
+        M->aPc[c] = dg;
+        if (comp == 0) M->aPx[c] = dg;
+    }
+    RSG_SYNC();
+}
+
+// assemblePressureCorrection SIMPLESolver.cpp:322-450 (aP = relaxed x-diag).
+// grad p must be current in M->gp when the Rhie-Chow term is active.
+RSG_D void assemble_pcorr(Ctx& C) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
+    const int nC = C.nC, nIF = C.nIF;
+    const bool rc = !m->hasOutlet || C.s->rcAll;
+    const double* aP = M->aPx;
+    FOR_IFACES(f) {
+        const int o = m->own[f], n = m->nbr[f];
+        const double Sf = m->area[f], delta = m->dlt[f], w = m->w[f], wm = 1.0 - w;
+        const double Vo = m->vol[o], Vn = m->vol[n];
+        const double dP_o = (fabs(aP[o]) > 1e-30) ? Vo / aP[o] : 0.0;
+        const double dP_n = (fabs(aP[n]) > 1e-30) ? Vn / aP[n] : 0.0;
+        const double dP_f = w * dP_o + (1.0 - w) * dP_n;
+        const double coeff = dP_f * Sf / delta;
+        M->upper[f] = -coeff; M->lower[f] = -coeff;
+        const double ufx = M->Ux[o] * w + M->Ux[n] * wm;
+        const double ufy = M->Uy[o] * w + M->Uy[n] * wm;
+        const double ufz = M->Uz[o] * w + M->Uz[n] * wm;
+        double massFlux = (ufx * m->nx[f] + ufy * m->ny[f] + ufz * m->nz[f]) * Sf;
+        if (rc) {
+            const double gbx = M->gp[0][o] * w + M->gp[0][n] * wm;
+            const double gby = M->gp[1][o] * w + M->gp[1][n] * wm;
+            const double gbz = M->gp[2][o] * w + M->gp[2][n] * wm;
+            const double ge = gbx * m->ex[f] + gby * m->ey[f] + gbz * m->ez[f];
+            massFlux += -dP_f * Sf * ((M->p[n] - M->p[o]) / delta - ge);
+        }
+        M->fscr[f] = massFlux;
+    }
+    RSG_SYNC();
+    FOR_CELLS(c) {
+        double dg = 0.0, src = 0.0;
+        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+            const int code = m->cfIdx[e], f = code >> 1;
+            if (f < nIF) {
+                dg += -M->upper[f];
+                if (code & 1) src += M->fscr[f]; else src -= M->fscr[f];
+            } else {
+                const int b = f - nIF;
+                const double Sf = m->area[f];
+                const double mf = (M->Ubx[b] * m->nx[f] + M->Uby[b] * m->ny[f] + M->Ubz[b] * m->nz[f]) * Sf;
+                src -= mf;
+                if (m->bFlags[b] & RSG_BF_OUTLET) {
+                    const double dP_o = (fabs(aP[c]) > 1e-30) ? m->vol[c] / aP[c] : 0.0;
+                    dg += dP_o * Sf / m->dlt[f];
+                }
+            }
+        }
+        if (c == 0 && !m->hasOutlet) dg += fabs(dg) > 1e-30 ? dg : 1.0;
+        M->diag[c] = dg;
+        M->src[c] = src;
+    }
+    RSG_SYNC();
+}
+
+// assembleKEquation (:483-560) when isOmega == false, assembleOmegaEquation
+// (:567-669) when true.
+RSG_D void assemble_turb(Ctx& C, bool isOmega) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
+    const int nC = C.nC, nIF = C.nIF;
+    const double nu = C.nu;
+    const double s1 = isOmega ? C.sw1 : C.sk1, s2 = isOmega ? C.sw2 : C.sk2;
+    FOR_IFACES(f) {
+        const int o = m->own[f], n = m->nbr[f];
+        const double w = m->w[f];
+        const double F1f = w * M->F1[o] + (1.0 - w) * M->F1[n];
+        const double sg = blend(s1, s2, F1f);
+        double up, lo;
+        conv_diff_face(m, M, f, nu + sg * M->nuT[o], nu + sg * M->nuT[n], up, lo);
+        M->upper[f] = up; M->lower[f] = lo;
+    }
+    RSG_SYNC();
+    const double* phi = isOmega ? M->om : M->k;
+    const double* phib = isOmega ? M->omb : M->kb;
+    const double a = isOmega ? C.s->aOm : C.s->aK;
+    FOR_CELLS(c) {
+        double dg = 0.0, src = 0.0;
+        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+            const int code = m->cfIdx[e], f = code >> 1;
+            if (f < nIF) {
+                dg += (code & 1) ? -M->upper[f] : -M->lower[f];
+            } else {
+                const int b = f - nIF;
+                const double sg = blend(s1, s2, M->F1[c]);
+                conv_diff_bface(m, M, f, b, nu + sg * M->nuT[c], phib[b], dg, src);
+            }
+        }
+        const double vol = m->vol[c];
+        if (!isOmega) {
+            src += M->Pk[c] * vol;
+            const double destruction = C.bs * dmax(M->om[c], 1e-20);
+            dg += destruction * vol;
+        } else {
+            const double F1 = M->F1[c];
+            const double omC = dmax(M->om[c], 1e-20);
+            const double alphaB = blend(C.al1, C.al2, F1);
+            src += alphaB * sst_prod_om(C, M->nuT[c], M->Smag[c], M->k[c], M->om[c]) * vol;
+            const double betaB = blend(C.b1, C.b2, F1);
+            dg += betaB * omC * vol;
+            src += (1.0 - F1) * M->CDkw[c] * vol;
+        }
+        src += (1.0 - a) / a * dg * phi[c];
+        dg /= a;
+        M->diag[c] = dg;
+        M->src[c] = src;
+        if (isOmega) M->omOld[c] = phi[c]; else M->kOld[c] = phi[c];
+    }
+    RSG_SYNC();
+}
+
+// velocity gradients of the listed components into M->g
+RSG_D void grad_u(Ctx& C, int c0, int c1) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const int nC = C.nC;
+    for (int comp = c0; comp < c1; ++comp) {
+        const double* const phi[1] = {ucomp(M, comp)};
+        const double* const phib[1] = {ubcomp(M, comp)};
+        FOR_CELLS(c) {
+            double g[1][3];
+            gg_cell<1>(m, c, phi, phib, g);
+            M->g[comp * 3 + 0][c] = g[0][0]; M->g[comp * 3 + 1][c] = g[0][1]; M->g[comp * 3 + 2][c] = g[0][2];
+        }
+    }
+}
+
+RSG_D void grad_scalar(Ctx& C, const double* phi_, const double* phib_, double** out) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; const int nC = C.nC;
+    const double* const phi[1] = {phi_};
+    const double* const phib[1] = {phib_};
+    FOR_CELLS(c) {
+        double g[1][3];
+        gg_cell<1>(m, c, phi, phib, g);
+        out[0][c] = g[0][0]; out[1][c] = g[0][1]; out[2][c] = g[0][2];
+    }
+}
+
+// --------------------------------------------------------------- init
+// initUniform SIMPLESolver.cpp:23-51 (+ applyAllBCs with beta1 = 0.075)
+RSG_D void init_uniform(Ctx& C) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
+    const int nC = C.nC, nB = C.nB;
+    const double* I = M->init;
+    FOR_CELLS(c) {
+        M->Ux[c] = I[0]; M->Uy[c] = I[1]; M->Uz[c] = I[2];
+        M->p[c] = I[3]; M->k[c] = I[4];
+        M->F1[c] = 1.0; M->F2[c] = 1.0; M->Pk[c] = 0.0; M->CDkw[c] = 0.0;
+        const double y = dmax(m->wd[c], 1e-20);
+        const double omWall = 60.0 * C.nu / (C.b1 * y * y);
+        const double om = dmax(omWall, I[5]);
+        M->om[c] = om;
+        M->nuT[c] = C.a1 * I[4] / dmax(C.a1 * om, 1e-20);
+    }
+    FOR_BFACES(b) {
+        M->Ubx[b] = I[0]; M->Uby[b] = I[1]; M->Ubz[b] = I[2];
+        M->pb[b] = I[3]; M->kb[b] = I[4]; M->omb[b] = I[5];
+    }
+    RSG_SYNC();
+    vel_bc(C); p_bc(C); k_bc(C); om_bc(C, 0.075);
+    RSG_SYNC();
+}
+
+// ------------------------------------------------------- outer iteration
+struct Scal {
+    int iter, status, turbEstablished, finalIter, amgBuiltIter, coarseIter;
+    double normUx0, normUy0, normP0, lastK, lastOm;
+};
+
+RSG_D void simple_iteration(Ctx& C, Scal& S) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const RsgSettingsDev* st = C.s;
+    const int nC = C.nC, nIF = C.nIF, nB = C.nB;
+    const double nu = C.nu;
+    const int iter = S.iter;
+    const bool turbActive = iter >= st->turbStart;
+    const bool turbUpdate = turbActive && ((iter - st->turbStart) % st->turbInterval == 0);
+
+    // 1. SST closure (computeFields) + frozen |S| + startup nuT floor
+    if (turbUpdate) {
+        if (!S.turbEstablished && iter >= st->turbStart + st->nutFloorIters) S.turbEstablished = 1;
+        const double nuTMin = S.turbEstablished ? 0.0 : 0.1 * nu;
+        const double* const phi[5] = {M->Ux, M->Uy, M->Uz, M->k, M->om};
+        const double* const phib[5] = {M->Ubx, M->Uby, M->Ubz, M->kb, M->omb};
+        FOR_CELLS(c) {
+            double g[5][3];
+            gg_cell<5>(m, c, phi, phib, g);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) { M->g[q * 3][c] = g[q][0]; M->g[q * 3 + 1][c] = g[q][1]; M->g[q * 3 + 2][c] = g[q][2]; }
+            const double S11 = g[0][0], S22 = g[1][1], S33 = g[2][2];
+            const double S12 = 0.5 * (g[0][1] + g[1][0]);
+            const double S13 = 0.5 * (g[0][2] + g[2][0]);
+            const double S23 = 0.5 * (g[1][2] + g[2][1]);
+            const double Sm = sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33
+                                          + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+            M->Smag[c] = Sm;
+            const double kc = dmax(M->k[c], 0.0);
+            const double wc = dmax(M->om[c], 1e-20);
+            const double y = dmax(m->wd[c], 1e-20);
+            const double dotkw = g[3][0] * g[4][0] + g[3][1] * g[4][1] + g[3][2] * g[4][2];
+            const double CDkw = 2.0 * C.sw2 / wc * dotkw;
+            M->CDkw[c] = CDkw;
+            const double CDpos = dmax(CDkw, 1e-20);
+            const double F1 = (st->variant == 2) ? 1.0 : sst_F1(C, kc, wc, y, nu, CDpos);
+            const double F2 = sst_F2(C, kc, wc, y, nu);
+            M->F1[c] = F1; M->F2[c] = F2;
+            const double nuT = sst_nut(C, kc, wc, Sm, F2);
+            M->Pk[c] = sst_prod(C, nuT, Sm, kc, wc);
+            M->nuT[c] = dmax(nuT, nuTMin);
+        }
+    } else {
+        grad_u(C, 0, 3);
+    }
+    grad_scalar(C, M->p, M->pb, M->gp);
+    RSG_SYNC();
+
+    // 2. momentum components, each assembled from the latest U
+    SolveRes rU[3];
+    for (int comp = 0; comp < 3; ++comp) {
+        if (comp > 0) { grad_u(C, comp - 1, comp); RSG_SYNC(); }
+        assemble_momentum(C, comp);
+        rU[comp] = lsolve(C, st->mSolver, ucomp(M, comp), false);
+        RSG_SYNC();
+    }
+    vel_bc(C);
+    RSG_SYNC();
+
+    // 3. pressure correction
+    assemble_pcorr(C);
+    bool amgReady = false;
+    if (st->pSolver == 0) {
+        amg_refresh_values(C);
+        if (S.amgBuiltIter < 0 || iter - S.amgBuiltIter >= st->amgRefresh) {
+            if (t.tid == 0) amg_setup_serial(m, M->amg);
+            RSG_SYNC();
+            S.amgBuiltIter = iter;
+        }
+        if (S.amgBuiltIter == iter || iter - S.coarseIter >= st->coarseRefresh) {
+            amg_coarse_inverse(C);
+            S.coarseIter = iter;
+        }
+        amgReady = true;
+    }
+    FOR_CELLS(c) M->pp[c] = 0.0;
+    RSG_SYNC();
+    SolveRes rP = lsolve(C, st->pSolver, M->pp, amgReady);
+    RSG_SYNC();
+    FOR_BFACES(b) M->ppb[b] = (m->bFlags[b] & RSG_BF_OUTLET) ? 0.0 : M->pp[m->own[nIF + b]];
+    RSG_SYNC();
+
+    // 4. correctVelocity / correctPressure
+    grad_scalar(C, M->pp, M->ppb, M->gp);
+    FOR_CELLS(c) {
+        const double rAP = (fabs(M->aPx[c]) > 1e-30) ? 1.0 / M->aPx[c] : 0.0;
+        const double vol = m->vol[c];
+        M->Ux[c] -= rAP * M->gp[0][c] * vol;
+        M->Uy[c] -= rAP * M->gp[1][c] * vol;
+        M->Uz[c] -= rAP * M->gp[2][c] * vol;
+        M->p[c] += st->aP * M->pp[c];
+    }
+    RSG_SYNC();
+    vel_bc(C); p_bc(C);
+    RSG_SYNC();
+
+    // 5. turbulence transport
+    SolveRes rK = {0, 0.0, 0.0, 0}, rOm = {0, 0.0, 0.0, 0};
+    if (turbUpdate) {
+        assemble_turb(C, false);
+        rK = lsolve(C, st->tSolver, M->k, false);
+        RSG_SYNC();
+        FOR_CELLS(c) M->k[c] = dmax(st->kMin, dmin(st->kMax, M->k[c]));
+        RSG_SYNC();
+        k_bc(C);
+        RSG_SYNC();
+        assemble_turb(C, true);
+        rOm = lsolve(C, st->tSolver, M->om, false);
+        RSG_SYNC();
+        FOR_CELLS(c) M->om[c] = dmax(st->omMin, dmin(1e15, M->om[c]));
+        RSG_SYNC();
+        om_bc(C, C.b1);
+        RSG_SYNC();
+        // wall-cell omega pin (:884-908) and field-change norms (:910-922)
+        double mx[4] = {1e-30, 1e-30, 0.0, 0.0};
+        const double sbs = sqrt(C.bs);
+        FOR_CELLS(c) {
+            const double y1 = m->pinY[c];
+            if (y1 > 0.0) {
+                const double omRes = 60.0 * nu / (C.b1 * y1 * y1);
+                if (st->wallFn) {
+                    const double k_p = dmax(M->k[c], 1e-30);
+                    const double uTau = sqrt(sqrt(C.bs) * k_p);
+                    const double omLog = uTau / (sbs * st->vonK * y1);
+                    M->om[c] = sqrt(omRes * omRes + omLog * omLog);
+                } else {
+                    M->om[c] = omRes;
+                }
+            }
+            mx[0] = dmax(mx[0], fabs(M->k[c]));
+            mx[1] = dmax(mx[1], fabs(M->om[c]));
+            mx[2] = dmax(mx[2], fabs(M->k[c] - M->kOld[c]));
+            mx[3] = dmax(mx[3], fabs(M->om[c] - M->omOld[c]));
+        }
+        bmax(t, mx);
+        S.lastK = mx[2] / mx[0];
+        S.lastOm = mx[3] / mx[1];
+    }
+
+    // 6. residual bookkeeping and convergence / divergence tests (:934-1035)
+    const double eUx = rU[0].initialRes, eUy = rU[1].initialRes, eUz = rU[2].initialRes;
+    const double eP = rP.initialRes;
+    const double eK = turbActive ? S.lastK : 0.0, eOm = turbActive ? S.lastOm : 0.0;
+    if (t.tid == 0) {
+        RsgScalars* sc = M->sc;
+        sc->innerIters[0] += rU[0].iterations; sc->innerIters[1] += rU[1].iterations;
+        sc->innerIters[2] += rU[2].iterations; sc->innerIters[3] += rP.iterations;
+        sc->innerIters[4] += rK.iterations; sc->innerIters[5] += rOm.iterations;
+        if (st->recordHistory && M->hist && iter < M->histCap) {
+            double* h = M->hist + 6 * (long)iter;
+            h[0] = eUx; h[1] = eUy; h[2] = eUz; h[3] = eP; h[4] = eK; h[5] = eOm;
+            sc->histLen = iter + 1;
+        }
+    }
+    if (iter == 0) {
+        S.normUx0 = dmax(eUx, 1e-30); S.normUy0 = dmax(eUy, 1e-30); S.normP0 = dmax(eP, 1e-30);
+    } else if (iter < 50) {
+        S.normP0 = dmax(S.normP0, eP);
+    }
+    const double normMom0 = dmax(S.normUx0, S.normUy0);
+    const double nUx = normMom0 < 1e-20 ? 0.0 : eUx / normMom0;
+    const double nUy = normMom0 < 1e-20 ? 0.0 : eUy / normMom0;
+    const double nP = S.normP0 < 1e-20 ? 0.0 : eP / S.normP0;
+    double maxRes = dmax(dmax(nUx, nUy), nP);
+    if (turbActive) maxRes = dmax(dmax(maxRes, S.lastK), S.lastOm);
+    if (!isfinite(eUx) || !isfinite(eUy) || !isfinite(eUz) || !isfinite(eP) || !isfinite(eK) ||
+        !isfinite(eOm) || !isfinite(S.lastK) || !isfinite(S.lastOm))
+        maxRes = INFINITY;
+    const bool turbScheduled = st->turbStart < st->maxIter;
+    S.iter = iter + 1;
+    if (maxRes < st->tol && iter > 0 && (!turbScheduled || (turbActive && S.turbEstablished))) {
+        S.status = RSGM_CONVERGED; S.finalIter = iter;
+    } else if (isnan(maxRes) || isinf(maxRes) || maxRes > st->divLimit) {
+        S.status = RSGM_DIVERGED; S.finalIter = iter;
+    } else if (S.iter >= st->maxIter) {
+        S.status = RSGM_MAXITER; S.finalIter = st->maxIter;
+    }
+    RSG_SYNC();
+}
+
+// Runs up to M->iterBudget outer iterations of one member.
+RSG_D void run_member(Thr t, RsgMemberDev* M) {
+    Ctx C;
+    C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
+    C.nC = C.m->nC; C.nIF = C.m->nIF; C.nB = C.m->nB; C.nu = M->nu;
+    C.sk1 = M->coeffs[0]; C.sw1 = M->coeffs[1]; C.b1 = M->coeffs[2]; C.al1 = M->coeffs[3];
+    C.sk2 = M->coeffs[4]; C.sw2 = M->coeffs[5]; C.b2 = M->coeffs[6]; C.al2 = M->coeffs[7];
+    C.bs = M->coeffs[8]; C.a1 = M->coeffs[9]; C.kap = M->coeffs[10];
+    RsgScalars* sc = M->sc;
+    if (sc->status != RSGM_RUNNING) return;
+    if (!sc->initialized) {
+        if (!M->initFromFields) init_uniform(C);
+        // level-0 pattern of the AMG hierarchy
+        RsgAmgDev& A = M->amg;
+        const RsgMeshDev* m = C.m;
+        for (int i = C.t.tid; i <= m->nC; i += C.t.bs) A.rp[i] = m->rp0[i];
+        for (int e = C.t.tid; e < m->nnz0; e += C.t.bs) A.ci[e] = m->ci0[e];
+        if (C.t.tid == 0) {
+            A.lvN[0] = m->nC; A.lvRow[0] = 0; A.lvRp[0] = 0; A.lvNz[0] = 0;
+            A.lvN[RSG_MAX_LEVELS] = 1;
+        }
+        RSG_SYNC();
+        if (C.t.tid == 0) { sc->initialized = 1; sc->amgBuiltIter = -1; }
+        RSG_SYNC();
+    }
+    Scal S;
+    S.iter = sc->iter; S.status = sc->status; S.turbEstablished = sc->turbEstablished;
+    S.finalIter = sc->finalIter; S.amgBuiltIter = sc->amgBuiltIter; S.coarseIter = sc->coarseIter;
+    S.normUx0 = sc->normUx0; S.normUy0 = sc->normUy0; S.normP0 = sc->normP0;
+    S.lastK = sc->lastK; S.lastOm = sc->lastOm;
+    RSG_SYNC();
+    for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) simple_iteration(C, S);
+    if (C.t.tid == 0) {
+        sc->iter = S.iter; sc->status = S.status; sc->turbEstablished = S.turbEstablished;
+        sc->finalIter = S.finalIter; sc->amgBuiltIter = S.amgBuiltIter; sc->coarseIter = S.coarseIter;
+        sc->normUx0 = S.normUx0; sc->normUy0 = S.normUy0; sc->normP0 = S.normP0;
+        sc->lastK = S.lastK; sc->lastOm = S.lastOm;
+        sc->nLevels = M->amg.lvN[RSG_MAX_LEVELS];
+    }
+}
+
+#undef FOR_CELLS
+#undef FOR_IFACES
+#undef FOR_BFACES
+}  // namespace rsg

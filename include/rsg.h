@@ -1,0 +1,192 @@
+/* rsg.h -- C-ABI of the MI355X RANS forward-model engine (librsg.so).
+ *
+ * The engine runs the reference's incompressible SIMPLE + SST k-omega solve
+ * (reference: incompressible/src/SIMPLESolver.cpp:696-1043) on one GPU,
+ * one workgroup per solve, many solves per launch.  The pybind11 module
+ * rans_sst_py (host C++) is the drop-in replacement of the reference's
+ * module of the same name (python/Bindings.cpp:273-519) and reaches the GPU
+ * only through the functions below.  Plain C types, caller-owned host
+ * buffers, int status codes; no exceptions cross this boundary.
+ *
+ * Reference interfaces each group replaces:
+ *   rsg_mesh_upload        Mesh as consumed by SIMPLESolver (core/include/Mesh.hpp:84-197)
+ *   rsg_problem_create     SIMPLESolver(mesh, sst, bcs, nu, settings) ctor
+ *                          (SIMPLESolver.cpp:10-19) minus the coefficients
+ *   rsg_submit/poll/wait   SIMPLESolver::initUniform + solve as called from
+ *                          ForwardModel::evaluate (incompressible/src/ForwardModel.cpp:47-66)
+ *   rsg_fetch              FlowFields + ConvergenceHistory returned by solve
+ *                          (core/include/FlowFields.hpp:9-49)
+ *   rsg_ldu_solve          ILinearSolver::solve (core/include/LinearSolver.hpp:115-120)
+ *   rsg_amg_aggregate      buildAggregates (core/include/LinearSolver.hpp:407-443)
+ */
+#ifndef RSG_H
+#define RSG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RSG_OK 0
+#define RSG_ERR_ARG (-1)        /* invalid argument (maps to ValueError) */
+#define RSG_ERR_NOMEM (-2)      /* HBM budget or host allocation exhausted */
+#define RSG_ERR_DEVICE (-3)     /* HIP runtime error */
+#define RSG_ERR_STATE (-4)      /* unknown id / ticket, or wrong state */
+#define RSG_ERR_TIMEOUT (-5)
+#define RSG_ERR_UNSUPPORTED (-6)
+
+/* patch types (Patch::type strings, Mesh.hpp:75-80) */
+#define RSG_PATCH_WALL 0
+#define RSG_PATCH_INLET 1
+#define RSG_PATCH_OUTLET 2
+#define RSG_PATCH_SYMMETRY 3
+#define RSG_PATCH_OTHER 4
+
+/* BC types: same numbering as BCType (core/include/BoundaryCondition.hpp:12-18) */
+#define RSG_BC_DIRICHLET 0
+#define RSG_BC_NEUMANN 1
+#define RSG_BC_INLET_VELOCITY 2
+#define RSG_BC_OUTLET_PRESSURE 3
+#define RSG_BC_WALL_NOSLIP 4
+#define RSG_BC_WALL_KOMEGA 5
+#define RSG_BC_WALL_MOVING 6
+#define RSG_BC_SYMMETRY 7
+#define RSG_BC_CYCLIC 8
+#define RSG_BC_KEEP 9            /* face in no patch: value never rewritten */
+
+/* linear solver codes (makeSolver names, LinearSolver.hpp:659-665) */
+#define RSG_SOLVER_AMG 0
+#define RSG_SOLVER_PCG 1
+#define RSG_SOLVER_BICGSTAB 2
+
+/* precision of a member's inner linear solves */
+#define RSG_PREC_FP64 0
+#define RSG_PREC_MIXED 1          /* FP32 inner Krylov, FP64 defect/refinement */
+
+/* ticket states */
+#define RSG_TICKET_PENDING 0
+#define RSG_TICKET_RUNNING 1
+#define RSG_TICKET_DONE 2
+#define RSG_TICKET_FAILED 3
+
+typedef struct rsg_context rsg_context;
+
+/* Mesh geometry and topology exactly as the reference stores it: internal
+ * faces first [0, n_internal_faces), boundary faces after.  All arrays are
+ * host memory, read during the call only. */
+typedef struct rsg_mesh_desc {
+    int32_t n_cells, n_faces, n_internal_faces, n_patches;
+    const int32_t* owner;          /* [n_faces] */
+    const int32_t* neighbor;       /* [n_internal_faces] */
+    const double* face_normal;     /* [3 n_faces] unit normal, owner -> neighbour */
+    const double* face_area;       /* [n_faces] */
+    const double* face_delta;      /* [n_faces] |d| */
+    const double* face_d;          /* [3 n_faces] owner->neighbour (owner->face centre on boundaries) */
+    const double* face_weight;     /* [n_faces] (internal faces used) */
+    const double* cell_volume;     /* [n_cells] */
+    const double* wall_distance;   /* [n_cells] (Mesh::computeWallDistance) */
+    const int32_t* patch_type;     /* [n_patches] RSG_PATCH_* */
+    const int32_t* patch_face_ptr; /* [n_patches + 1] */
+    const int32_t* patch_faces;    /* face ids of each patch, in patch order */
+} rsg_mesh_desc;
+
+/* Boundary conditions resolved per boundary face b = face - n_internal_faces
+ * (per-patch BCType plus any per-face profile, BoundaryCondition.hpp:20-32). */
+typedef struct rsg_bc_desc {
+    const int32_t* vel_type; const double* vel_value;   /* [nB], [3 nB] */
+    const int32_t* p_type;   const double* p_value;     /* [nB], [nB] */
+    const int32_t* k_type;   const double* k_value;
+    const int32_t* om_type;  const double* om_value;
+} rsg_bc_desc;
+
+/* SolverSettings (core/include/FlowFields.hpp:53-132) + engine knobs */
+typedef struct rsg_settings {
+    int32_t max_iterations;
+    double convergence_tol, divergence_limit;
+    double alpha_u, alpha_p, alpha_k, alpha_omega;
+    int32_t inner_iterations;
+    double inner_tolerance;
+    int32_t turb_start_iter, turb_update_interval, nut_floor_iters;
+    int32_t rhie_chow_all_meshes;
+    double body_force[3];
+    double k_min, k_max, omega_min;
+    int32_t use_wall_functions;
+    double von_karman;
+    int32_t sst_variant;
+    int32_t pressure_solver, momentum_solver, turbulence_solver;   /* RSG_SOLVER_* */
+    /* engine knobs (no reference counterpart) */
+    int32_t amg_refresh;      /* re-aggregate the p' hierarchy every N outer iterations (>=1) */
+    int32_t coarse_refresh;   /* refresh the coarsest dense inverse every N outer iterations */
+    int32_t record_history;   /* keep the per-iteration residual history */
+} rsg_settings;
+
+/* Initial state.  kind 0: uniform (SIMPLESolver::initUniform); kind 1: full
+ * fields (warm start from a cached converged state, ForwardModel.cpp:56-60). */
+typedef struct rsg_init {
+    int32_t kind;
+    double U[3], p, k, omega;
+    const double *U_cells, *p_cells, *k_cells, *om_cells;   /* [3n], [n] ... */
+    const double *nuT, *F1, *F2, *Pk, *CDkw;                  /* [n] */
+    const double *U_bnd, *p_bnd, *k_bnd, *om_bnd;             /* [3nB], [nB] ... */
+    int32_t turb_established;
+} rsg_init;
+
+/* Result of a finished solve; every pointer is caller-owned and may be NULL. */
+typedef struct rsg_result {
+    int32_t status;           /* RSG_TICKET_DONE or RSG_TICKET_FAILED */
+    int32_t final_iter, converged, diverged, turb_established;
+    double *U, *p, *k, *omega, *nuT, *F1, *F2, *Pk, *CDkw;
+    double *U_bnd, *p_bnd, *k_bnd, *om_bnd;
+    double* history;          /* 6 doubles (Ux Uy Uz p k omega) per outer iteration */
+    int32_t history_cap;      /* entries available in history */
+    int32_t history_len;      /* entries written */
+    double inner_iters[6];    /* cumulative inner iterations per equation */
+    double t_submit, t_start, t_end;   /* engine clock, seconds */
+} rsg_result;
+
+int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** out);
+int rsg_context_destroy(rsg_context* ctx);
+const char* rsg_last_error(rsg_context* ctx);
+/* engine clock (seconds) used by t_submit/t_start/t_end */
+double rsg_now(rsg_context* ctx);
+/* device properties: out[0] = compute units, out[1] = wavefront size */
+int rsg_device_info(rsg_context* ctx, int32_t* out);
+
+int rsg_mesh_upload(rsg_context* ctx, const rsg_mesh_desc* mesh, int32_t* mesh_id);
+int rsg_mesh_release(rsg_context* ctx, int32_t mesh_id);
+int rsg_problem_create(rsg_context* ctx, int32_t mesh_id, const rsg_bc_desc* bc, double nu,
+                       const rsg_settings* settings, int32_t* problem_id);
+int rsg_problem_release(rsg_context* ctx, int32_t problem_id);
+
+/* Enqueue one solve; returns immediately.  coeffs: the 11 SST coefficients
+ * in InferenceParameterSet order (core/include/InferenceParameters.hpp:12-14). */
+int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const rsg_init* init,
+               int32_t precision, int64_t* ticket);
+int rsg_poll(rsg_context* ctx, const int64_t* tickets, int32_t n, int32_t* state_out);
+/* Blocks until every ticket is DONE/FAILED or timeout_s elapses (<0: forever). */
+int rsg_wait(rsg_context* ctx, const int64_t* tickets, int32_t n, double timeout_s);
+int rsg_fetch(rsg_context* ctx, int64_t ticket, rsg_result* out);
+int rsg_release(rsg_context* ctx, int64_t ticket);
+/* Engine tuning: outer iterations per persistent launch and workgroup size. */
+int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_size);
+/* Engine counters: out[0] launches, out[1] members admitted, out[2] members retired,
+ * out[3] kernel milliseconds (HIP events), out[4] max concurrent members. */
+int rsg_stats(rsg_context* ctx, double* out);
+
+/* One LDU linear solve on the device (x in/out, host buffers).
+ * stats = {iterations, finalRes, initialRes, converged}. */
+int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                  const double* diag, const double* upper, const double* lower, const double* b,
+                  double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
+                  double* stats);
+
+/* Greedy aggregation of one CSR matrix (host, bit-exact to the reference).
+ * Returns the number of aggregates, or a negative status. */
+int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                      const double* values, const double* diag, int32_t* map);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
