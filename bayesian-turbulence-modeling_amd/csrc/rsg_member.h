@@ -36,10 +36,12 @@
 #ifdef RSG_EMULATE
 #include <math.h>
 #define RSG_D static inline
+#define RSG_HD static inline
 #define RSG_SYNC() ((void)0)
 #else
 #include <hip/hip_runtime.h>
 #define RSG_D __device__ static __attribute__((always_inline)) inline
+#define RSG_HD __host__ __device__ static inline
 #define RSG_SYNC() __syncthreads()
 #endif
 
@@ -389,7 +391,7 @@ RSG_D SolveRes bicgstab(Ctx& C, double* x, int maxIter, double tol) {
 // LinearSolver.hpp:407-443); the Galerkin pattern and the gather lists
 // reproduce galerkinProduct's summation order (:446-485: coarse entry (I,J)
 // sums fine entries in ascending fine row, then ascending entry).
-RSG_D int amg_aggregate(int n, const int* rp, const int* ci, const double* v, const double* dg, int* map) {
+RSG_HD int amg_aggregate(int n, const int* rp, const int* ci, const double* v, const double* dg, int* map) {
     int nc = 0;
     for (int i = 0; i < n; ++i) map[i] = -1;
     for (int i = 0; i < n; ++i) {

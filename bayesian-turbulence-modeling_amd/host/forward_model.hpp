@@ -1,0 +1,112 @@
+// forward_model.hpp -- theta -> EvaluationResult on the MI355X engine.
+//
+// Same contract as the reference ForwardModel (incompressible/include/
+// ForwardModel.hpp:75-139, incompressible/src/ForwardModel.cpp:29-122):
+// bounds check, unpack to the 11 SST coefficients, warm start from the
+// nearest cached converged state or a uniform initial state, SIMPLE solve,
+// status classification, predictions and Gaussian log-likelihood.  The solve
+// itself runs in librsg.so (include/rsg.h); this class only marshals.
+#pragma once
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mesh.hpp"
+#include "model.hpp"
+#include "rsg.h"
+
+namespace rsh {
+
+enum class EvaluationStatus { Converged, Unconverged, Diverged, InvalidParameters, Unknown };
+
+struct EvaluationResult {
+    EvaluationStatus status = EvaluationStatus::Unknown;
+    double loglik = -1e30;
+    std::vector<double> predictions;
+    int simpleIters = 0;
+};
+
+// Process-wide engine context on one GPU (created on first use).
+class Engine {
+public:
+    static Engine& get();
+    rsg_context* ctx();           // throws if the GPU engine cannot start
+    void shutdown();
+    void setLaunch(int itersPerLaunch, int blockSize);
+    std::vector<double> stats();
+private:
+    std::mutex mu_;
+    rsg_context* ctx_ = nullptr;
+};
+
+std::string engineError(rsg_context* ctx, int rc);
+
+// WarmStartCache, ForwardModel.hpp:18-72 (value copies under the lock)
+class WarmStartCache {
+public:
+    struct Entry { std::vector<double> theta; FlowFields fields; };
+    explicit WarmStartCache(int maxSize = 50, double threshold = 0.5) : maxSize_(maxSize), threshold_(threshold) {}
+    bool findNearest(const std::vector<double>& theta, Entry& out) const;
+    void store(const std::vector<double>& theta, const FlowFields& f);
+    int size() const { std::lock_guard<std::mutex> l(mu_); return (int)cache_.size(); }
+    void clear() { std::lock_guard<std::mutex> l(mu_); cache_.clear(); }
+private:
+    int maxSize_;
+    double threshold_;
+    mutable std::mutex mu_;
+    std::deque<Entry> cache_;
+};
+
+class ForwardModel {
+public:
+    ForwardModel(const Mesh& mesh, const InferenceParameterSet& paramSet, const ObservationOperator& obsOp,
+                 const FlowBoundaryConditions& bcs, double nu, const SolverSettings& settings = {},
+                 const Vec3& Uinit = Vec3(1, 0, 0), double pInit = 0.0, double kInit = 1e-4,
+                 double omegaInit = 1.0);
+    ~ForwardModel();
+    ForwardModel(const ForwardModel&) = delete;
+    ForwardModel& operator=(const ForwardModel&) = delete;
+
+    EvaluationResult evaluate(const std::vector<double>& theta);
+    // Many thetas solved concurrently on the device (one workgroup each);
+    // each result equals what evaluate() returns for that theta alone, given
+    // the same warm-start cache contents.
+    std::vector<EvaluationResult> evaluateBatch(const std::vector<std::vector<double>>& thetas);
+    double penalizedLogLikelihood(const std::vector<double>& theta);
+
+    WarmStartCache& cache() { return cache_; }
+    const InferenceParameterSet& paramSet() const { return paramSet_; }
+    const FlowFields& lastFields() const { return lastFields_; }
+    bool hasLastFields() const { return hasLastFields_; }
+    const ConvergenceInfo& lastConvergence() const { return lastConv_; }
+    const Mesh& mesh() const { return mesh_; }
+    void setPrecision(int p) { precision_ = p; }
+    int precision() const { return precision_; }
+
+    // Submission split for servers: returns an engine ticket.
+    int64_t submit(const std::vector<double>& theta, bool useCache);
+    // Finishes a ticket into a result (and the retained state).
+    EvaluationResult complete(int64_t ticket, const std::vector<double>& theta, bool storeCache);
+
+private:
+    const Mesh& mesh_;
+    InferenceParameterSet paramSet_;
+    ObservationOperator obsOp_;
+    FlowBoundaryConditions bcs_;
+    double nu_;
+    SolverSettings settings_;
+    Vec3 Uinit_;
+    double pInit_, kInit_, omegaInit_;
+    WarmStartCache cache_;
+    FlowFields lastFields_;
+    bool hasLastFields_ = false;
+    ConvergenceInfo lastConv_;
+    int precision_ = RSG_PREC_FP64;
+    int meshId_ = -1, problemId_ = -1;
+    std::mutex mu_;
+    void ensureProblem();
+};
+
+}  // namespace rsh

@@ -324,6 +324,52 @@ int ref_residual(const char* path, int bcKind, const double* bcVals, double nu,
     } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
 }
 
+// Writes a mesh built by one of the reference factories to MSH1.
+// kind 0 makeChannel2D(nx,ny,Lx,Ly); 1 makeChannel2D(...,Re,y+);
+// 2 makePlate2D(nx,ny,Lx,Ly,Re,y+); 3 makeBackwardFacingStep2D(8 args);
+// 4 makeBackwardFacingStep2D(...,Re,y+);
+// 5 makeCurvedChannelPeriodic2D(x[nx+1], yb[nx+1], yTop, ny, Re, y+).
+// ip / dp carry the integer / real arguments in signature order (kind 5:
+// ip = {nx, ny}, dp = {yTop, Re, y+, x..., yb...}).
+int ref_make_mesh(int kind, const int* ip, const double* dp, const char* path) {
+    try {
+        Mesh m;
+        switch (kind) {
+            case 0: m = Mesh::makeChannel2D(ip[0], ip[1], dp[0], dp[1]); break;
+            case 1: m = Mesh::makeChannel2D(ip[0], ip[1], dp[0], dp[1], dp[2], dp[3]); break;
+            case 2: m = Mesh::makePlate2D(ip[0], ip[1], dp[0], dp[1], dp[2], dp[3]); break;
+            case 3: m = Mesh::makeBackwardFacingStep2D(ip[0], ip[1], ip[2], ip[3], dp[0], dp[1], dp[2], dp[3]); break;
+            case 4: m = Mesh::makeBackwardFacingStep2D(ip[0], ip[1], ip[2], ip[3], dp[0], dp[1], dp[2], dp[3], dp[4], dp[5]); break;
+            case 5: {
+                const int nx = ip[0];
+                std::vector<double> x(dp + 3, dp + 3 + nx + 1), yb(dp + 4 + nx, dp + 4 + 2 * nx + 1);
+                m = Mesh::makeCurvedChannelPeriodic2D(x, yb, dp[0], ip[1], dp[1], dp[2]);
+                break;
+            }
+            default: return -1;
+        }
+        m.saveBinary(path);
+        return 0;
+    } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
+}
+
+// Transpose-stress source (StressOperators.hpp:31-84) for comp, with the
+// wall-zero coefficient when wallZero != 0.
+int ref_transpose_stress(const char* path, const double* coeff, const double* U, const double* Ub,
+                         int comp, int wallZero, double* src) {
+    try {
+        Mesh m = loadMesh(path);
+        ScalarField c(m, "c");
+        VectorField u(m, "U");
+        for (int i = 0; i < m.nCells(); ++i) { c[i] = coeff[i]; u[i] = Vec3(U[3*i], U[3*i+1], U[3*i+2]); }
+        for (int i = 0; i < u.nBnd(); ++i) u.bnd(i) = Vec3(Ub[3*i], Ub[3*i+1], Ub[3*i+2]);
+        std::vector<double> zero(m.nCells(), 0.0);
+        std::vector<double> s = transposeStressSource(m, c, u, comp, wallZero ? &zero : nullptr);
+        for (int i = 0; i < m.nCells(); ++i) src[i] = s[i];
+        return 0;
+    } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
+}
+
 // Pointwise SST functions (SSTModel.cpp:9-118) for closed-form checks.
 // which: 0 F1(k,w,y,nu,CDpos) 1 F2(k,w,y,nu) 2 eddyViscosity(k,w,S,F2)
 //        3 production(nuT,S,k,w) 4 productionOmega(nuT,S,k,w)

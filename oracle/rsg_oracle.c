@@ -965,6 +965,48 @@ static void transpose_src(solver_t* S, const double* coeff, const fields_t* f, i
 #undef GCOL
 }
 
+int om_transpose_stress(const om_mesh* m, const double* coeff, const double* U, const double* Ub,
+                        int comp, int wallZero, double* src) {
+    /* transposeStressSource with (wallZero) or without the wall coefficient */
+    double zeroBc[3] = {0.0, 0.0, 0.0};
+    (void)zeroBc;
+    solver_t S;
+    memset(&S, 0, sizeof S);
+    S.m = m;
+    int n = m->nC, nB = m->nF - m->nIF;
+    S.vg.dudx = calloc(n + 1, sizeof(v3)); S.vg.dvdx = calloc(n + 1, sizeof(v3)); S.vg.dwdx = calloc(n + 1, sizeof(v3));
+    S.scC = calloc(n + 1, 8); S.scB = calloc(nB + 1, 8);
+    fields_t f; memset(&f, 0, sizeof f);
+    f.U = (v3*)U; f.Ub = (v3*)Ub;
+    if (wallZero) {
+        transpose_src(&S, coeff, &f, comp, src);
+    } else {
+        /* owner extrapolation on every boundary face (StressOperators.hpp:60-67) */
+        vel_grads(m, f.U, f.Ub, &S.vg, S.scC, S.scB);
+        velgrad* vg = &S.vg;
+#define GC(ci) (comp == 0 ? v3make(vg->dudx[ci].x, vg->dvdx[ci].x, vg->dwdx[ci].x) : \
+                comp == 1 ? v3make(vg->dudx[ci].y, vg->dvdx[ci].y, vg->dwdx[ci].y) : \
+                            v3make(vg->dudx[ci].z, vg->dvdx[ci].z, vg->dwdx[ci].z))
+        for (int c = 0; c < n; ++c) src[c] = 0.0;
+        for (int fi = 0; fi < m->nIF; ++fi) {
+            int o = m->own[fi], nb = m->nbr[fi];
+            double w = m->weight[fi];
+            double cf = w * coeff[o] + (1.0 - w) * coeff[nb];
+            v3 g = v3add(v3mul(GC(o), w), v3mul(GC(nb), 1.0 - w));
+            double flux = cf * v3dot(g, m->fn[fi]) * m->area[fi];
+            src[o] += flux; src[nb] -= flux;
+        }
+        for (int fi = m->nIF; fi < m->nF; ++fi) {
+            int o = m->own[fi];
+            double flux = coeff[o] * v3dot(GC(o), m->fn[fi]) * m->area[fi];
+            src[o] += flux;
+        }
+#undef GC
+    }
+    free(S.vg.dudx); free(S.vg.dvdx); free(S.vg.dwdx); free(S.scC); free(S.scB);
+    return 0;
+}
+
 /* assembleMomentum SIMPLESolver.cpp:56-203 (injection source excluded) */
 static void assemble_momentum(solver_t* S, ldu* A, const fields_t* f, int comp, double* aP) {
     const om_mesh* m = S->m;

@@ -98,6 +98,11 @@ int om_residual(const om_mesh* m, int bcKind, const double* bcVals, double nu,
                 const double* k, const double* kb, const double* om, const double* omb,
                 double* R);
 
+/* transposeStressSource (StressOperators.hpp:31-84); wallZero selects the
+ * wall-coefficient-zero boundary treatment the momentum assembly uses. */
+int om_transpose_stress(const om_mesh* m, const double* coeff, const double* U, const double* Ub,
+                        int comp, int wallZero, double* src);
+
 /* Observation operator (ObservationOperator.hpp:159-378).
  * obs types: 0 Drag 1 PressureTap 2 VelocityProfile 3 SkinFriction
  *            4 SeparationPoint 5 ReattachmentLength

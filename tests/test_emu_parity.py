@@ -1,0 +1,68 @@
+"""The product member solver (csrc/rsg_member.h) compiled sequentially
+(tests/emu) against the oracle.  With a Jacobi-PCG pressure solve every
+algorithm on the path is the reference's, so the whole SIMPLE trajectory must
+agree bit for bit; with the device AMG (damped-Jacobi V-cycle, exact dense
+coarsest solve) the trajectory differs only through the pressure
+preconditioner."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import rsg_abi as A
+from conftest import CASES, oracle_solve, settings_for
+
+D = lambda a: (C.c_double * len(a))(*a)
+
+
+def emu_solve(emu, path, case, s):
+    kind, vals, nu, _ = CASES[case]
+    msh = A.Msh(path)
+    r, arr = A.new_result(msh.nC, msh.nF - msh.nIF, s.max_iterations if s.record_history else 0)
+    rc = emu.emu_solve(C.byref(msh.desc()), C.byref(msh.bc(kind, vals)), C.c_double(nu), C.byref(s),
+                       D(A.DEFAULT_COEFFS), D([1, 0, 0, 0, 1e-4, 1.0]), 0, C.byref(r))
+    assert rc == 0
+    return r, arr
+
+
+@pytest.mark.parametrize("case,iters", [("c0", 700), ("channel_bounded", 150), ("couette", 150),
+                                        ("bfs", 80), ("plate", 80), ("hill", 120)])
+def test_emulated_device_trajectory_bit_exact(oracle, emu, meshes, case, iters):
+    s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=1,
+                     nut_floor_iters=min(500, iters // 2), record_history=1)
+    r, a = emu_solve(emu, meshes[case], case, s)
+    o, ho, io = oracle_solve(oracle, meshes[case], case, s, hist_cap=iters)
+    assert r.final_iter == io[0] and r.converged == io[1] and r.diverged == io[2]
+    he = a["history"][:6 * r.history_len].reshape(-1, 6)
+    assert np.array_equal(he, ho)
+    for k_emu, k_or in (("U", "U"), ("p", "p"), ("k", "k"), ("omega", "om"), ("nuT", "nuT"),
+                        ("F1", "F1"), ("F2", "F2"), ("Pk", "Pk"), ("CDkw", "CDkw"),
+                        ("U_bnd", "Ub"), ("p_bnd", "pb"), ("k_bnd", "kb"), ("om_bnd", "omb")):
+        assert np.array_equal(a[k_emu], o[k_or]), k_emu
+
+
+def test_emulated_amg_tracks_oracle(oracle, emu, meshes):
+    # 700 iterations of C0 with the device AMG vs the oracle's reference AMG:
+    # same discretisation, different p' preconditioner
+    s = settings_for("c0", max_iterations=700, convergence_tol=1e-12, pressure_solver=0, amg_refresh=1)
+    r, a = emu_solve(emu, meshes["c0"], "c0", s)
+    o, _, _ = oracle_solve(oracle, meshes["c0"], "c0", s)
+    for k_emu, k_or in (("U", "U"), ("k", "k"), ("omega", "om"), ("nuT", "nuT")):
+        rel = np.linalg.norm(a[k_emu] - o[k_or]) / np.linalg.norm(o[k_or])
+        assert rel < 1e-4, (k_emu, rel)
+    assert r.inner_iters[3] < 0.5 * 700 * 30   # the V-cycle keeps p' solves short
+
+
+def test_emulated_warm_restart_is_a_continuation(oracle, emu, meshes):
+    # a solve split at an arbitrary outer iteration equals the unsplit solve
+    # (state, normalisation and carried norms all persist across launches)
+    msh = A.Msh(meshes["c0"])
+    s = settings_for("c0", max_iterations=300, convergence_tol=1e-12, pressure_solver=1)
+    kind, vals, nu, _ = CASES["c0"]
+    r1, a1 = A.new_result(msh.nC, msh.nF - msh.nIF)
+    emu.emu_solve(C.byref(msh.desc()), C.byref(msh.bc(kind, vals)), C.c_double(nu), C.byref(s),
+                  D(A.DEFAULT_COEFFS), D([1, 0, 0, 0, 1e-4, 1.0]), 37, C.byref(r1))
+    r2, a2 = A.new_result(msh.nC, msh.nF - msh.nIF)
+    emu.emu_solve(C.byref(msh.desc()), C.byref(msh.bc(kind, vals)), C.c_double(nu), C.byref(s),
+                  D(A.DEFAULT_COEFFS), D([1, 0, 0, 0, 1e-4, 1.0]), 300, C.byref(r2))
+    assert np.array_equal(a1["U"], a2["U"]) and np.array_equal(a1["k"], a2["k"])
