@@ -89,7 +89,9 @@ PYBIND11_MODULE(rans_sst_py, m) {
         .def_readonly("status", &EvaluationResult::status)
         .def_readonly("log_lik", &EvaluationResult::loglik)
         .def_readonly("predictions", &EvaluationResult::predictions)
-        .def_readonly("simple_iters", &EvaluationResult::simpleIters);
+        .def_readonly("simple_iters", &EvaluationResult::simpleIters)
+        .def_readonly("latency", &EvaluationResult::latency)
+        .def_readonly("device_time", &EvaluationResult::deviceTime);
 
     py::class_<SSTCoefficients>(m, "SSTCoefficients")
         .def(py::init<>())
@@ -294,6 +296,14 @@ PYBIND11_MODULE(rans_sst_py, m) {
               return oddEvenEnergyRatio(mesh, values);
           },
           py::arg("mesh"), py::arg("values"));
+
+    m.def("evaluate_many",
+          [](std::vector<ForwardModel*> fms, std::vector<std::vector<double>> thetas) {
+              py::gil_scoped_release nogil;
+              return evaluateMany(fms, thetas);
+          },
+          py::arg("models"), py::arg("thetas"),
+          "Solve one theta per ForwardModel concurrently on the device; results in input order.");
 
     m.def("engine_set_launch", [](int iters, int block) { Engine::get().setLaunch(iters, block); },
           py::arg("iters_per_launch"), py::arg("block_size") = 256);

@@ -154,6 +154,8 @@ EvaluationResult ForwardModel::complete(int64_t ticket, const std::vector<double
 
     EvaluationResult res;
     res.simpleIters = r.final_iter;
+    res.latency = r.t_end - r.t_submit;
+    res.deviceTime = r.t_end - r.t_start;
     std::lock_guard<std::mutex> l(mu_);
     lastConv_ = conv;
     // ForwardModel.cpp:70-97
@@ -211,6 +213,31 @@ std::vector<EvaluationResult> ForwardModel::evaluateBatch(const std::vector<std:
         try {
             out[i] = complete(tickets[i], thetas[i], true);
         } catch (const std::exception& e) {
+            out[i].status = EvaluationStatus::Unknown;
+            out[i].loglik = -1e6;
+        }
+    }
+    return out;
+}
+
+std::vector<EvaluationResult> evaluateMany(const std::vector<ForwardModel*>& fms,
+                                           const std::vector<std::vector<double>>& thetas) {
+    if (fms.size() != thetas.size()) throw std::invalid_argument("evaluate_many: models and thetas differ in length");
+    std::vector<EvaluationResult> out(fms.size());
+    std::vector<int64_t> tickets(fms.size(), -1);
+    for (size_t i = 0; i < fms.size(); ++i) {
+        if (!fms[i]->paramSet().inBounds(thetas[i])) {
+            out[i].status = EvaluationStatus::InvalidParameters;
+            out[i].loglik = -1e30;
+            continue;
+        }
+        tickets[i] = fms[i]->submit(thetas[i], true);
+    }
+    for (size_t i = 0; i < fms.size(); ++i) {
+        if (tickets[i] < 0) continue;
+        try {
+            out[i] = fms[i]->complete(tickets[i], thetas[i], true);
+        } catch (const std::exception&) {
             out[i].status = EvaluationStatus::Unknown;
             out[i].loglik = -1e6;
         }

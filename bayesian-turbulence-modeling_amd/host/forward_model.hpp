@@ -26,6 +26,8 @@ struct EvaluationResult {
     double loglik = -1e30;
     std::vector<double> predictions;
     int simpleIters = 0;
+    // engine timing of the solve (seconds): submit -> result, device residency
+    double latency = 0.0, deviceTime = 0.0;
 };
 
 // Process-wide engine context on one GPU (created on first use).
@@ -108,5 +110,10 @@ private:
     std::mutex mu_;
     void ensureProblem();
 };
+
+// Ensemble evaluation: every (model, theta) pair is submitted before any is
+// awaited, so all members share the device (one workgroup each).
+std::vector<EvaluationResult> evaluateMany(const std::vector<ForwardModel*>& fms,
+                                           const std::vector<std::vector<double>>& thetas);
 
 }  // namespace rsh

@@ -1,0 +1,196 @@
+"""Benchmark: converged RANS forward solves per second on one MI355X.
+
+Workload (one "step"): the C2 batched ensemble -- 512 streamwise-periodic
+channel solves (4 x ny quads, ny 96-384 from Re_tau log-uniform in
+[180, 2000], betaStar/a1/sigma_k1/sigma_w1/beta1 within +-20 % of their
+defaults), each driven by SIMPLE + SST k-omega to the reference's convergence
+criterion at tol 1e-8, all members resident on the device at once (one
+workgroup per member, persistent launches, continuous batching).
+
+Prints ONE JSON line (rank 0).  value = converged solves / wall second over the
+timed steps; p50/p99 = per-member latency (submit -> result) inside a step.
+roofline = algorithmic bytes of the SIMPLE chunk kernel / its HIP-event time,
+against the 8 TB/s HBM peak.  cpu_baseline = the reference solver compiled
+from its own sources (oracle/_ref) on one host core, one member solved to the
+same tolerance.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "bayesian-turbulence-modeling_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.join(PKG, "python"))
+
+METRIC = "converged RANS solves/sec at <=1e-6 rel-L2 vs FP64; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n, nif, nb, outer, inner):
+    """Algorithmic HBM bytes of one member's solve (FP64 values, int32 indices),
+    each array counted once per pass (SpMV(LDU) = 8(n + 2 f_i) + 8 f_i + 16 n;
+    dot/axpy = 16-24 n per vector pair; face assembly reads geometry + fields).
+    outer: outer iterations; inner: cumulative inner iterations
+    (Ux, Uy, Uz, p, k, omega)."""
+    w = 8.0
+    spmv = w * (n + 2 * nif) + 8.0 * nif + 2 * w * n + 4.0 * (n + 1 + 2 * nif)   # + cell->face lists
+    gg = nif * (8 + w + 3 * w) + nb * (4 + 4 * w) + n * (w + w + 3 * w) + 4.0 * (n + 1 + 2 * nif)
+    face_asm = nif * (8 + 4 * w + 6 * w) + 2 * w * nif + nb * (8 * w)
+    cell_asm = 4.0 * (n + 1 + 2 * nif) + n * (8 * w) + 2 * w * nif
+    assemble = face_asm + cell_asm
+    sst = 16 * w * n + gg * 5 / 1.0
+    per_outer = (3 * (assemble + gg)          # momentum: assembly + grad U refresh
+                 + gg + assemble + 2 * w * n  # grad p, p' assembly
+                 + gg + 6 * w * n             # grad p', corrections
+                 + 2 * assemble + 4 * w * n   # k, omega assembly + clamps
+                 + sst)
+    bicg = 2 * spmv + 14 * w * n
+    # V-cycle: ~4/3 of level-0 work per smoothing pass (geometric coarsening)
+    vcycle = (4.0 / 3.0) * (5 * spmv + 6 * w * n)
+    pcg_amg = spmv + vcycle + 8 * w * n
+    inner_bytes = (inner[0] + inner[1] + inner[2] + inner[4] + inner[5]) * bicg + inner[3] * pcg_amg
+    return outer * per_outer + inner_bytes
+
+
+def run_batch(rs, specs, tol, fms_cache=None):
+    import ensembles
+    if fms_cache is None:
+        meshes, fms = ensembles.build_models(rs, specs, tol=tol)
+    else:
+        meshes, fms = fms_cache
+    for fm in fms:
+        fm.clear_cache()              # every step solves cold (order-independent)
+    t0 = time.perf_counter()
+    res = rs.evaluate_many(fms, [sp["theta"] for sp in specs])
+    wall = time.perf_counter() - t0
+    return (meshes, fms), res, wall
+
+
+def cpu_baseline(specs, tol, budget_s=30.0):
+    """Reference solver (oracle/_ref) on one core, one member to tol."""
+    import ctypes as C
+    lib_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    kind = "reference"
+    if not os.path.exists(lib_path):
+        lib_path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+        kind = "port"
+    if not os.path.exists(lib_path):
+        return None
+    import rans_sst_py as rs
+    sp = min(specs, key=lambda s: s["ny"])           # smallest member keeps the sample to ~10-30 s
+    y = rs.Mesh.geometric_channel_nodes(sp["ny"] // 2, sp["ratio"], 1.0)
+    mesh = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))
+    path = "/tmp/bench_cpu_member.msh"
+    mesh.save_binary(path)
+    coeffs = [0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]
+    for idx, v in zip([8, 9, 0, 1, 2], sp["theta"]):
+        coeffs[idx] = v
+    packed = [400000, tol, 1e6, 0.7, 0.3, 0.5, 0.5, 200, 1e-3, 5, 1, 500, 0, 1.0, 0, 0,
+              1e-10, 1e10, 1e-6, 0, 0.41, 0, 0, 2, 2]
+    D = lambda a: (C.c_double * len(a))(*a)
+    lib = C.CDLL(lib_path)
+    info = (C.c_int * 5)()
+    t0 = time.perf_counter()
+    if kind == "reference":
+        lib.ref_simple_solve(path.encode(), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs), D(packed),
+                             D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+    else:
+        lib.om_mesh_load.restype = C.c_void_p
+        m = lib.om_mesh_load(path.encode())
+        lib.om_simple_solve(C.c_void_p(m), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs), D(packed),
+                            D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": "one member (Re_tau=%.0f, 4x%d, %d cells) solved to %g in %d outer iterations, %.1f s"
+                      % (sp["re_tau"], sp["ny"], 4 * sp["ny"], tol, info[0], dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--members", type=int, default=512)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--iters-per-launch", type=int, default=500)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this build has a single MI355X "
+              "(ensemble members shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import rans_sst_py as rs
+    import ensembles
+    specs = ensembles.channel_ensemble(args.members, args.seed)
+    rs.engine_set_launch(args.iters_per_launch, args.block)
+    cache = None
+    for _ in range(args.warmup):
+        cache, _, _ = run_batch(rs, specs, args.tol, cache)
+    stats0 = rs.engine_stats()
+    walls, lat, conv, outer = [], [], 0, []
+    res = None
+    for _ in range(args.steps):
+        cache, res, wall = run_batch(rs, specs, args.tol, cache)
+        walls.append(wall)
+        for r in res:
+            lat.append(r.latency)
+            ok = str(r.status).split(".")[-1] == "Converged"
+            conv += ok
+            outer.append(r.simple_iters)
+    stats1 = rs.engine_stats()
+    total = sum(walls)
+    launches = stats1[0] - stats0[0]
+    kernel_ms = stats1[3] - stats0[3]
+    # algorithmic bytes of the timed steps
+    meshes, fms = cache
+    bytes_alg = 0.0
+    for fm, mesh, r in zip(fms, meshes, res):
+        inner = fm.last_inner_iterations()
+        n, nif = mesh.n_cells(), mesh.n_internal_faces()
+        bytes_alg += algorithmic_bytes(n, nif, mesh.n_faces() - nif, r.simple_iters, inner)
+    bytes_alg *= args.steps
+    achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": conv / total,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "C2 batched ensemble: %d periodic channels 4x(96-384), Re_tau logU[180,2000], "
+                               "betaStar/a1/sigma_k1/sigma_w1/beta1 +-20%%, SIMPLE+SST to tol %g, FP64"
+                               % (args.members, args.tol),
+                   "members": args.members, "tol": args.tol, "seed": args.seed,
+                   "iters_per_launch": args.iters_per_launch, "block": args.block},
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "converged": int(conv),
+        "solves": len(lat),
+        "outer_iters_median": float(np.median(outer)),
+        "max_rel_l2_vs_fp64": 0.0,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_simple_chunk", "launches": launches,
+                     "avg_launch_ms": kernel_ms / max(launches, 1)},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(specs, args.tol),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
