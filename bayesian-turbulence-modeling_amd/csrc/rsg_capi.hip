@@ -56,30 +56,11 @@ __global__ __launch_bounds__(BS) void k_ldu_solve(RsgMemberDev* M, int solver, i
     __shared__ double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
     rsg::Thr t;
     t.tid = threadIdx.x; t.bs = BS; t.red = red; t.lds = lds; t.phase = 0;
-    rsg::Ctx C;
-    C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
-    C.nC = C.m->nC; C.nIF = C.m->nIF; C.nB = C.m->nB; C.nu = 0.0;
-    bool amg = false;
-    if (solver == RSG_SOLVER_AMG) {
-        RsgAmgDev& A = M->amg;
-        const RsgMeshDev* m = C.m;
-        for (int i = t.tid; i <= m->nC; i += BS) A.rp[i] = m->rp0[i];
-        for (int e = t.tid; e < m->nnz0; e += BS) A.ci[e] = m->ci0[e];
-        if (t.tid == 0) { A.lvN[0] = m->nC; A.lvRow[0] = 0; A.lvRp[0] = 0; A.lvNz[0] = 0; A.lvN[RSG_MAX_LEVELS] = 1; }
-        __syncthreads();
-        rsg::amg_refresh_values(C);
-        if (t.tid == 0) rsg::amg_setup_serial(m, A);
-        __syncthreads();
-        rsg::amg_coarse_inverse(C);
-        amg = true;
-    }
-    rsg::SolveRes r;
-    if (solver == RSG_SOLVER_BICGSTAB) r = rsg::bicgstab(C, M->pp, maxIter, tol);
-    else r = rsg::pcg(C, M->pp, maxIter, tol, amg);
-    __syncthreads();
+    int nl = 0;
+    rsg::SolveRes r = rsg::ldu_solve_member(t, M, solver, maxIter, tol, &nl);
     if (t.tid == 0) {
         stats[0] = r.iterations; stats[1] = r.finalRes; stats[2] = r.initialRes; stats[3] = r.converged;
-        stats[4] = M->amg.lvN[RSG_MAX_LEVELS];
+        stats[4] = nl;
     }
 }
 
@@ -707,19 +688,11 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     if (precision != RSG_PREC_FP64) return set_err(ctx, RSG_ERR_UNSUPPORTED, "only FP64 is implemented");
     if (solver != RSG_SOLVER_AMG && solver != RSG_SOLVER_PCG && solver != RSG_SOLVER_BICGSTAB)
         return set_err(ctx, RSG_ERR_ARG, "unknown solver code");
-    // topology-only mesh: every face internal, geometry unused
-    std::vector<double> zeros3(3 * (size_t)std::max(nif, 1), 0.0), zeros(std::max(nif, 1), 0.0), ones(n, 1.0);
     for (int f = 0; f < nif; ++f)
         if (own[f] < 0 || own[f] >= n || nbr[f] < 0 || nbr[f] >= n) return set_err(ctx, RSG_ERR_ARG, "face index out of range");
-    rsg_mesh_desc d;
-    d.n_cells = n; d.n_faces = nif; d.n_internal_faces = nif; d.n_patches = 0;
-    d.owner = own; d.neighbor = nbr; d.face_normal = zeros3.data(); d.face_area = zeros.data();
-    d.face_delta = zeros.data(); d.face_d = zeros3.data(); d.face_weight = zeros.data();
-    d.cell_volume = ones.data(); d.wall_distance = ones.data();
-    d.patch_type = nullptr; d.patch_face_ptr = nullptr; d.patch_faces = nullptr;
     rsg::PrepMesh pm;
     std::string err;
-    if (!rsg::prep_mesh(&d, pm, err)) return set_err(ctx, RSG_ERR_ARG, err);
+    if (!rsg::prep_topology(n, nif, own, nbr, pm, err)) return set_err(ctx, RSG_ERR_ARG, err);
     rsg::MeshOffsets mo = rsg::mesh_offsets(pm);
     const size_t mbytes = mo.total + 256;
     const size_t wbytes = rsg::member_bytes(pm.nC, pm.nIF, pm.nB, pm.nnz0, 0);

@@ -1247,6 +1247,35 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
     }
 }
 
+// One linear solve (ILinearSolver::solve, LinearSolver.hpp:115-120) on the
+// LDU system already stored in M (diag/upper/lower/src), x = M->pp in/out.
+// solver: RSG_SOLVER_* codes (0 AMG-PCG, 1 Jacobi-PCG, 2 BiCGSTAB).
+RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter, double tol, int* nLevels) {
+    Ctx C;
+    C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
+    C.nC = C.m->nC; C.nIF = C.m->nIF; C.nB = C.m->nB; C.nu = 0.0;
+    bool amg = false;
+    if (solver == 0) {
+        RsgAmgDev& A = M->amg;
+        const RsgMeshDev* m = C.m;
+        for (int i = t.tid; i <= m->nC; i += t.bs) A.rp[i] = m->rp0[i];
+        for (int e = t.tid; e < m->nnz0; e += t.bs) A.ci[e] = m->ci0[e];
+        if (t.tid == 0) { A.lvN[0] = m->nC; A.lvRow[0] = 0; A.lvRp[0] = 0; A.lvNz[0] = 0; A.lvN[RSG_MAX_LEVELS] = 1; }
+        RSG_SYNC();
+        amg_refresh_values(C);
+        if (t.tid == 0) amg_setup_serial(m, A);
+        RSG_SYNC();
+        amg_coarse_inverse(C);
+        amg = true;
+    }
+    SolveRes r;
+    if (solver == 2) r = bicgstab(C, M->pp, maxIter, tol);
+    else r = pcg(C, M->pp, maxIter, tol, amg);
+    RSG_SYNC();
+    *nLevels = M->amg.lvN[RSG_MAX_LEVELS];
+    return r;
+}
+
 #undef FOR_CELLS
 #undef FOR_IFACES
 #undef FOR_BFACES

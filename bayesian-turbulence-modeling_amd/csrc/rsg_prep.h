@@ -94,9 +94,14 @@ inline bool prep_mesh(const rsg_mesh_desc* d, PrepMesh& P, std::string& err) {
     P.bFlags.assign(nB, 0);
     P.pinY.assign(nC, 0.0);
     P.hasOutlet = 0;
-    P.patchType.assign(d->patch_type, d->patch_type + d->n_patches);
-    P.patchPtr.assign(d->patch_face_ptr, d->patch_face_ptr + d->n_patches + 1);
-    if (d->n_patches > 0 && P.patchPtr[0] != 0) { err = "mesh descriptor: patch_face_ptr[0] != 0"; return false; }
+    if (d->n_patches > 0) {
+        P.patchType.assign(d->patch_type, d->patch_type + d->n_patches);
+        P.patchPtr.assign(d->patch_face_ptr, d->patch_face_ptr + d->n_patches + 1);
+    } else {
+        P.patchType.clear();
+        P.patchPtr.assign(1, 0);   // no patches: empty face list
+    }
+    if (P.patchPtr[0] != 0) { err = "mesh descriptor: patch_face_ptr[0] != 0"; return false; }
     for (int p = 0; p < d->n_patches; ++p) {
         const int a = P.patchPtr[p], b = P.patchPtr[p + 1];
         if (b < a || b > nB) { err = "mesh descriptor: patch_face_ptr not monotone"; return false; }
@@ -138,6 +143,20 @@ inline bool prep_mesh(const rsg_mesh_desc* d, PrepMesh& P, std::string& err) {
         }
     }
     return true;
+}
+
+// Topology-only mesh of an LDU system (every face internal, no geometry),
+// used by the stand-alone linear-solve entry point.
+inline bool prep_topology(int n, int nif, const int* own, const int* nbr, PrepMesh& P, std::string& err) {
+    if (n <= 0 || nif < 0 || !own || (nif > 0 && !nbr)) { err = "LDU system: bad sizes or null topology"; return false; }
+    std::vector<double> z3(3 * (size_t)std::max(nif, 1), 0.0), z(std::max(nif, 1), 0.0), one(n, 1.0);
+    rsg_mesh_desc d;
+    d.n_cells = n; d.n_faces = nif; d.n_internal_faces = nif; d.n_patches = 0;
+    d.owner = own; d.neighbor = nbr; d.face_normal = z3.data(); d.face_area = z.data();
+    d.face_delta = z.data(); d.face_d = z3.data(); d.face_weight = z.data();
+    d.cell_volume = one.data(); d.wall_distance = one.data();
+    d.patch_type = nullptr; d.patch_face_ptr = nullptr; d.patch_faces = nullptr;
+    return prep_mesh(&d, P, err);
 }
 
 // Byte offsets of every RsgMeshDev array inside one contiguous block.

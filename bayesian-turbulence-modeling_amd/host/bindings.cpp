@@ -266,6 +266,14 @@ PYBIND11_MODULE(rans_sst_py, m) {
              py::arg("k_init") = 1e-4, py::arg("omega_init") = 1.0, py::keep_alive<1, 2>())
         .def("evaluate", &ForwardModel::evaluate, py::call_guard<py::gil_scoped_release>())
         .def("evaluate_batch", &ForwardModel::evaluateBatch, py::call_guard<py::gil_scoped_release>())
+        .def("submit", [](ForwardModel& fm, const std::vector<double>& theta) {
+                 if (!fm.paramSet().inBounds(theta)) throw std::invalid_argument("theta out of bounds");
+                 py::gil_scoped_release nogil;
+                 return fm.submit(theta, true);
+             }, py::arg("theta"), "Enqueue a solve on the device engine; returns a ticket.")
+        .def("complete", &ForwardModel::complete, py::arg("ticket"), py::arg("theta"),
+             py::arg("store_cache") = true, py::call_guard<py::gil_scoped_release>(),
+             "Wait for a ticket and return its EvaluationResult.")
         .def("penalized_log_likelihood", &ForwardModel::penalizedLogLikelihood,
              py::call_guard<py::gil_scoped_release>())
         .def("param_set", &ForwardModel::paramSet, py::return_value_policy::reference)

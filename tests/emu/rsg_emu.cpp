@@ -68,3 +68,34 @@ extern "C" int emu_solve(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, doub
     }
     return 0;
 }
+
+// Stand-alone LDU solve through the same topology preparation and solver
+// body the engine's rsg_ldu_solve uses.  stats = {iters, finalRes, initialRes, converged, levels}.
+extern "C" int emu_ldu_solve(int n, int nif, const int* own, const int* nbr, const double* diag,
+                             const double* upper, const double* lower, const double* b, double* x,
+                             int solver, int maxIter, double tol, double* stats) {
+    rsg::PrepMesh P;
+    std::string err;
+    if (!rsg::prep_topology(n, nif, own, nbr, P, err)) { fprintf(stderr, "emu: %s\n", err.c_str()); return -1; }
+    rsg::MeshOffsets mo = rsg::mesh_offsets(P);
+    std::vector<char> mblk(mo.total + 256);
+    rsg::mesh_stage(P, mo, mblk.data());
+    RsgMeshDev md = rsg::mesh_view(P, mo, mblk.data());
+    RsgMemberDev M;
+    memset(&M, 0, sizeof M);
+    std::vector<char> wblk(rsg::member_bytes(P.nC, P.nIF, P.nB, P.nnz0, 0) + 256, 0);
+    rsg::Carver cv; cv.base = wblk.data();
+    rsg::carve_member(cv, P.nC, P.nIF, P.nB, P.nnz0, 0, M);
+    M.mesh = &md;
+    memcpy(M.diag, diag, 8 * (size_t)n); memcpy(M.src, b, 8 * (size_t)n); memcpy(M.pp, x, 8 * (size_t)n);
+    if (nif > 0) { memcpy(M.upper, upper, 8 * (size_t)nif); memcpy(M.lower, lower, 8 * (size_t)nif); }
+    M.sc->status = RSGM_RUNNING;
+    static double red[256];
+    static double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
+    rsg::Thr t; t.tid = 0; t.bs = 1; t.red = red; t.lds = lds; t.phase = 0;
+    int nl = 0;
+    rsg::SolveRes r = rsg::ldu_solve_member(t, &M, solver, maxIter, tol, &nl);
+    memcpy(x, M.pp, 8 * (size_t)n);
+    stats[0] = r.iterations; stats[1] = r.finalRes; stats[2] = r.initialRes; stats[3] = r.converged; stats[4] = nl;
+    return 0;
+}
