@@ -160,7 +160,7 @@ struct rsg_context {
     int nextMesh = 1, nextProblem = 1;
     int64_t nextTicket = 1;
     int itersPerLaunch = 200;
-    int blockSize = 256;
+    int blockSize = 512;
     int maxSlots = 4096;
     std::thread engine;
     bool stop = false;
@@ -747,5 +747,17 @@ int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
             if (col_idx[j] < 0 || col_idx[j] >= n) return RSG_ERR_ARG;
     return rsg::amg_aggregate(n, row_ptr, col_idx, values, diag, map);
 }
+
+#ifdef RSG_PROF
+// Section timers of the instrumented build: seconds per slot, then reset.
+int rsg_prof_read(double* out) {
+    unsigned long long t[32];
+    if (hipMemcpyFromSymbol(t, HIP_SYMBOL(rsg_prof_ticks), sizeof t) != hipSuccess) return RSG_ERR_DEVICE;
+    for (int i = 0; i < 32; ++i) out[i] = 1e-8 * (double)t[i];
+    const unsigned long long z[32] = {};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(rsg_prof_ticks), z, sizeof z) != hipSuccess) return RSG_ERR_DEVICE;
+    return RSG_OK;
+}
+#endif
 
 }  // extern "C"

@@ -14,6 +14,7 @@
 #define RSG_MAX_LEVELS 20     // AMGSolver::maxLevels_ (LinearSolver.hpp:508)
 #define RSG_MIN_COARSE 50     // AMGSolver::minCoarseSize_ (LinearSolver.hpp:509)
 #define RSG_COARSE_DENSE 64   // coarsest levels up to this size are inverted densely
+#define RSG_ELL_K 4           // internal faces per cell held in registers (resident solves)
 
 // boundary-face flag bits (from the face's patch type)
 #define RSG_BF_WALL 1
@@ -58,6 +59,12 @@ struct RsgMeshDev {
     const int* rp0;          // [nC+1]
     const int* ci0;          // [nnz0]
     const int* src0;         // [nnz0]: <0 -> diag[-1-v]; <nIF -> upper[v]; else lower[v-nIF]
+    // cell-major ELL image of the off-diagonal pattern: the k-th internal
+    // face of cell c (ascending face order) at [k * nC + c]
+    int ellK;                // max internal faces of a cell (0 if > RSG_ELL_K)
+    const int* ellCnt;       // [nC]
+    const int* ellNb;        // [RSG_ELL_K * nC] neighbouring cell (padding: c)
+    const int* ellFace;      // [RSG_ELL_K * nC] (f << 1) | isNeighbour (padding: -1)
 };
 
 // per-problem boundary conditions, resolved per boundary face
@@ -139,6 +146,7 @@ struct RsgMemberDev {
     // LDU system in use
     double *diag, *upper, *lower, *src;
     double *fscr;          // [nIF] face scratch
+    double *ellA;          // [RSG_ELL_K * nC] off-diagonal of the LDU system in ELL order
     double *kv[8];         // Krylov vectors
     RsgAmgDev amg;
     double* hist;          // [6 * histCap] (optional)

@@ -45,6 +45,23 @@
 #define RSG_SYNC() __syncthreads()
 #endif
 
+// Section timers (built only with RSG_PROF): thread 0 of every member adds the
+// wall-clock ticks (100 MHz) between consecutive marks to rsg_prof_ticks[slot].
+#if defined(RSG_PROF) && !defined(RSG_EMULATE)
+__device__ unsigned long long rsg_prof_ticks[32];
+#define RSG_TMARK(slot)                                                          \
+    do {                                                                         \
+        __syncthreads();                                                         \
+        if (C.t.tid == 0) {                                                      \
+            const unsigned long long now_ = wall_clock64();                      \
+            atomicAdd(&rsg_prof_ticks[slot], now_ - C.tmark);                    \
+            C.tmark = now_;                                                      \
+        }                                                                        \
+    } while (0)
+#else
+#define RSG_TMARK(slot) ((void)0)
+#endif
+
 namespace rsg {
 
 // ------------------------------------------------------------ primitives
@@ -130,6 +147,7 @@ struct Ctx {
     double nu;
     // SST coefficients, InferenceParameters.hpp:12-14 order
     double sk1, sw1, b1, al1, sk2, sw2, b2, al2, bs, a1, kap;
+    unsigned long long tmark;   // section timer (RSG_PROF builds)
 };
 
 RSG_D double blend(double p1, double p2, double F1) { return F1 * p1 + (1.0 - F1) * p2; }
@@ -1073,31 +1091,38 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     }
     grad_scalar(C, M->p, M->pb, M->gp);
     RSG_SYNC();
+    RSG_TMARK(0);
 
     // 2. momentum components, each assembled from the latest U
     SolveRes rU[3];
     for (int comp = 0; comp < 3; ++comp) {
         if (comp > 0) { grad_u(C, comp - 1, comp); RSG_SYNC(); }
         assemble_momentum(C, comp);
+        RSG_TMARK(1);
         rU[comp] = lsolve(C, st->mSolver, ucomp(M, comp), false);
         RSG_SYNC();
+        RSG_TMARK(2);
     }
     vel_bc(C);
     RSG_SYNC();
 
     // 3. pressure correction
     assemble_pcorr(C);
+    RSG_TMARK(3);
     bool amgReady = false;
     if (st->pSolver == 0) {
         amg_refresh_values(C);
+        RSG_TMARK(4);
         if (S.amgBuiltIter < 0 || iter - S.amgBuiltIter >= st->amgRefresh) {
             if (t.tid == 0) amg_setup_serial(m, M->amg);
             RSG_SYNC();
             S.amgBuiltIter = iter;
+            RSG_TMARK(5);
         }
         if (S.amgBuiltIter == iter || iter - S.coarseIter >= st->coarseRefresh) {
             amg_coarse_inverse(C);
             S.coarseIter = iter;
+            RSG_TMARK(6);
         }
         amgReady = true;
     }
@@ -1105,6 +1130,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     RSG_SYNC();
     SolveRes rP = lsolve(C, st->pSolver, M->pp, amgReady);
     RSG_SYNC();
+    RSG_TMARK(7);
     FOR_BFACES(b) M->ppb[b] = (m->bFlags[b] & RSG_BF_OUTLET) ? 0.0 : M->pp[m->own[nIF + b]];
     RSG_SYNC();
 
@@ -1121,20 +1147,25 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     RSG_SYNC();
     vel_bc(C); p_bc(C);
     RSG_SYNC();
+    RSG_TMARK(8);
 
     // 5. turbulence transport
     SolveRes rK = {0, 0.0, 0.0, 0}, rOm = {0, 0.0, 0.0, 0};
     if (turbUpdate) {
         assemble_turb(C, false);
+        RSG_TMARK(9);
         rK = lsolve(C, st->tSolver, M->k, false);
         RSG_SYNC();
+        RSG_TMARK(10);
         FOR_CELLS(c) M->k[c] = dmax(st->kMin, dmin(st->kMax, M->k[c]));
         RSG_SYNC();
         k_bc(C);
         RSG_SYNC();
         assemble_turb(C, true);
+        RSG_TMARK(11);
         rOm = lsolve(C, st->tSolver, M->om, false);
         RSG_SYNC();
+        RSG_TMARK(12);
         FOR_CELLS(c) M->om[c] = dmax(st->omMin, dmin(1e15, M->om[c]));
         RSG_SYNC();
         om_bc(C, C.b1);
@@ -1204,6 +1235,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
         S.status = RSGM_MAXITER; S.finalIter = st->maxIter;
     }
     RSG_SYNC();
+    RSG_TMARK(13);
 }
 
 // Runs up to M->iterBudget outer iterations of one member.
@@ -1237,6 +1269,9 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
     S.normUx0 = sc->normUx0; S.normUy0 = sc->normUy0; S.normP0 = sc->normP0;
     S.lastK = sc->lastK; S.lastOm = sc->lastOm;
     RSG_SYNC();
+#if defined(RSG_PROF) && !defined(RSG_EMULATE)
+    C.tmark = wall_clock64();
+#endif
     for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) simple_iteration(C, S);
     if (C.t.tid == 0) {
         sc->iter = S.iter; sc->status = S.status; sc->turbEstablished = S.turbEstablished;

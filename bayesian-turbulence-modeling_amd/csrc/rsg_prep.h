@@ -31,7 +31,8 @@ struct Carver {
 // Host image of one mesh, in the order RsgMeshDev expects.
 struct PrepMesh {
     int nC = 0, nIF = 0, nF = 0, nB = 0, hasOutlet = 0, nnz0 = 0, nP = 0;
-    std::vector<int> own, nbr, cfPtr, cfIdx, bFlags, rp0, ci0, src0;
+    std::vector<int> own, nbr, cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace;
+    int ellK = 0;
     std::vector<double> w, dlt, area, nx, ny, nz, sfx, sfy, sfz, ex, ey, ez, vol, wd, pinY;
     std::vector<int> patchType, patchPtr, patchFaces;
 };
@@ -117,6 +118,27 @@ inline bool prep_mesh(const rsg_mesh_desc* d, PrepMesh& P, std::string& err) {
         if (P.patchType[p] == RSG_PATCH_OUTLET && b > a) P.hasOutlet = 1;
     }
     P.patchFaces.assign(d->patch_faces, d->patch_faces + (d->n_patches > 0 ? P.patchPtr[d->n_patches] : 0));
+    // ELL image of the internal faces of each cell, in ascending face order
+    P.ellCnt.assign(nC, 0);
+    P.ellNb.assign((size_t)RSG_ELL_K * nC, 0);
+    P.ellFace.assign((size_t)RSG_ELL_K * nC, -1);
+    int kmax = 0;
+    for (int c = 0; c < nC; ++c) {
+        int k = 0;
+        for (int e = P.cfPtr[c]; e < P.cfPtr[c + 1]; ++e) {
+            const int code = P.cfIdx[e], f = code >> 1;
+            if (f >= nIF) continue;
+            if (k < RSG_ELL_K) {
+                P.ellNb[(size_t)k * nC + c] = (code & 1) ? P.own[f] : P.nbr[f];
+                P.ellFace[(size_t)k * nC + c] = code;
+            }
+            ++k;
+        }
+        P.ellCnt[c] = std::min(k, RSG_ELL_K);
+        for (int j = k; j < RSG_ELL_K; ++j) P.ellNb[(size_t)j * nC + c] = c;
+        kmax = std::max(kmax, k);
+    }
+    P.ellK = kmax <= RSG_ELL_K ? kmax : 0;
     // level-0 CSR pattern, built as lduToCSR does (LinearSolver.hpp:344-404):
     // diagonal first, then faces in order, then a stable insertion sort per row
     P.nnz0 = nC + 2 * nIF;
@@ -162,7 +184,7 @@ inline bool prep_topology(int n, int nif, const int* own, const int* nbr, PrepMe
 // Byte offsets of every RsgMeshDev array inside one contiguous block.
 struct MeshOffsets {
     size_t own, nbr, w, dlt, area, nx, ny, nz, sfx, sfy, sfz, ex, ey, ez, vol, wd, pinY,
-        cfPtr, cfIdx, bFlags, rp0, ci0, src0, total;
+        cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace, total;
 };
 
 inline MeshOffsets mesh_offsets(const PrepMesh& P) {
@@ -177,6 +199,7 @@ inline MeshOffsets mesh_offsets(const PrepMesh& P) {
     o.vol = at(8 * P.vol.size()); o.wd = at(8 * P.wd.size()); o.pinY = at(8 * P.pinY.size());
     o.cfPtr = at(4 * P.cfPtr.size()); o.cfIdx = at(4 * P.cfIdx.size()); o.bFlags = at(4 * P.bFlags.size());
     o.rp0 = at(4 * P.rp0.size()); o.ci0 = at(4 * P.ci0.size()); o.src0 = at(4 * P.src0.size());
+    o.ellCnt = at(4 * P.ellCnt.size()); o.ellNb = at(4 * P.ellNb.size()); o.ellFace = at(4 * P.ellFace.size());
     o.total = (cv.off + 255) & ~size_t(255);
     return o;
 }
@@ -199,6 +222,8 @@ inline void mesh_stage(const PrepMesh& P, const MeshOffsets& o, char* base) {
     put(o.bFlags, P.bFlags.data(), 4 * P.bFlags.size());
     put(o.rp0, P.rp0.data(), 4 * P.rp0.size()); put(o.ci0, P.ci0.data(), 4 * P.ci0.size());
     put(o.src0, P.src0.data(), 4 * P.src0.size());
+    put(o.ellCnt, P.ellCnt.data(), 4 * P.ellCnt.size()); put(o.ellNb, P.ellNb.data(), 4 * P.ellNb.size());
+    put(o.ellFace, P.ellFace.data(), 4 * P.ellFace.size());
 }
 
 // RsgMeshDev whose pointers address `base` (host staging or device copy).
@@ -216,6 +241,9 @@ inline RsgMeshDev mesh_view(const PrepMesh& P, const MeshOffsets& o, char* base)
     m.cfPtr = (const int*)(base + o.cfPtr); m.cfIdx = (const int*)(base + o.cfIdx);
     m.bFlags = (const int*)(base + o.bFlags);
     m.rp0 = (const int*)(base + o.rp0); m.ci0 = (const int*)(base + o.ci0); m.src0 = (const int*)(base + o.src0);
+    m.ellK = P.ellK;
+    m.ellCnt = (const int*)(base + o.ellCnt); m.ellNb = (const int*)(base + o.ellNb);
+    m.ellFace = (const int*)(base + o.ellFace);
     return m;
 }
 
@@ -258,6 +286,7 @@ inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int hist
     M.upper = cv.take<double>(nIF > 0 ? nIF : 1);
     M.lower = cv.take<double>(nIF > 0 ? nIF : 1);
     M.fscr = cv.take<double>(nIF > 0 ? nIF : 1);
+    M.ellA = cv.take<double>((size_t)RSG_ELL_K * nC);
     RsgAmgDev& A = M.amg;
     A.rowCap = 3 * nC + 64;
     A.nnzCap = 3 * nnz0 + 64;

@@ -314,7 +314,19 @@ PYBIND11_MODULE(rans_sst_py, m) {
           "Solve one theta per ForwardModel concurrently on the device; results in input order.");
 
     m.def("engine_set_launch", [](int iters, int block) { Engine::get().setLaunch(iters, block); },
-          py::arg("iters_per_launch"), py::arg("block_size") = 256);
+          py::arg("iters_per_launch"), py::arg("block_size") = 512);
+    m.def("ticket_states",
+          [](const std::vector<int64_t>& tickets) {
+              std::vector<int32_t> st(tickets.size(), 0);
+              if (!tickets.empty()) {
+                  rsg_context* c = Engine::get().ctx();
+                  const int rc = rsg_poll(c, tickets.data(), (int32_t)tickets.size(), st.data());
+                  if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+              }
+              return st;
+          },
+          py::arg("tickets"),
+          "Engine state of each ticket: 0 pending, 1 running, 2 done, 3 failed (does not block).");
     m.def("engine_stats", []() { return Engine::get().stats(); });
     m.def("_shutdown", []() { Engine::get().shutdown(); });
     py::module_::import("atexit").attr("register")(m.attr("_shutdown"));

@@ -116,6 +116,31 @@ void copyScalar(const ScalarField& f, double* out, double* bout) {
     if (bout) for (int i = 0; i < f.nBnd(); ++i) bout[i] = f.bnd(i);
 }
 
+// Runs SIMPLESolver::solve on f and exports the state (shared by the cold
+// and the warm entry points).
+int solveAndExport(const Mesh& m, SIMPLESolver& solver, FlowFields& f,
+                   double* U, double* p, double* k, double* om, double* nuT,
+                   double* F1, double* F2, double* Pk, double* CDkw,
+                   double* Ub, double* pb, double* kb, double* omb,
+                   double* hist, int histCap, int* info) {
+    ConvergenceHistory h = solver.solve(f);
+    const int n = m.nCells();
+    if (U) for (int i = 0; i < n; ++i) { U[3*i] = f.U[i].x; U[3*i+1] = f.U[i].y; U[3*i+2] = f.U[i].z; }
+    if (Ub) for (int i = 0; i < f.U.nBnd(); ++i) { Ub[3*i] = f.U.bnd(i).x; Ub[3*i+1] = f.U.bnd(i).y; Ub[3*i+2] = f.U.bnd(i).z; }
+    copyScalar(f.p, p, pb); copyScalar(f.k, k, kb); copyScalar(f.omega, om, omb);
+    copyScalar(f.nuT, nuT, nullptr); copyScalar(f.F1, F1, nullptr);
+    copyScalar(f.F2, F2, nullptr); copyScalar(f.Pk, Pk, nullptr);
+    copyScalar(f.CDkw, CDkw, nullptr);
+    int ne = (int)h.entries.size();
+    if (hist) for (int e = 0; e < ne && e < histCap; ++e) {
+        const auto& r = h.entries[e];
+        hist[6*e] = r.Ux; hist[6*e+1] = r.Uy; hist[6*e+2] = r.Uz;
+        hist[6*e+3] = r.p; hist[6*e+4] = r.k; hist[6*e+5] = r.omega;
+    }
+    info[0] = h.finalIter; info[1] = h.converged; info[2] = h.diverged; info[3] = ne;
+    info[4] = f.turbEstablished ? 1 : 0;
+    return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -160,23 +185,42 @@ int ref_simple_solve(const char* path, int bcKind, const double* bcVals, double 
         SIMPLESolver solver(m, sst, bcs, nu, s);
         FlowFields f(m);
         solver.initUniform(f, Vec3(init[0], init[1], init[2]), init[3], init[4], init[5]);
-        ConvergenceHistory h = solver.solve(f);
-        const int n = m.nCells();
-        if (U) for (int i = 0; i < n; ++i) { U[3*i] = f.U[i].x; U[3*i+1] = f.U[i].y; U[3*i+2] = f.U[i].z; }
-        if (Ub) for (int i = 0; i < f.U.nBnd(); ++i) { Ub[3*i] = f.U.bnd(i).x; Ub[3*i+1] = f.U.bnd(i).y; Ub[3*i+2] = f.U.bnd(i).z; }
-        copyScalar(f.p, p, pb); copyScalar(f.k, k, kb); copyScalar(f.omega, om, omb);
-        copyScalar(f.nuT, nuT, nullptr); copyScalar(f.F1, F1, nullptr);
-        copyScalar(f.F2, F2, nullptr); copyScalar(f.Pk, Pk, nullptr);
-        copyScalar(f.CDkw, CDkw, nullptr);
-        int ne = (int)h.entries.size();
-        if (hist) for (int e = 0; e < ne && e < histCap; ++e) {
-            const auto& r = h.entries[e];
-            hist[6*e] = r.Ux; hist[6*e+1] = r.Uy; hist[6*e+2] = r.Uz;
-            hist[6*e+3] = r.p; hist[6*e+4] = r.k; hist[6*e+5] = r.omega;
+        return solveAndExport(m, solver, f, U, p, k, om, nuT, F1, F2, Pk, CDkw, Ub, pb, kb, omb,
+                              hist, histCap, info);
+    } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
+}
+
+// Warm start (ForwardModel.cpp:56-60: fields = cached.fields; solve(fields)).
+// state: U[3n] Ub[3nB] p pb k kb omega omegab nuT F1 F2 Pk CDkw (13 arrays).
+int ref_simple_solve_warm(const char* path, int bcKind, const double* bcVals, double nu,
+                          const double* coeffs, const double* settings,
+                          const double* const* state, int turbEstablished,
+                          double* U, double* p, double* k, double* om, double* nuT,
+                          double* F1, double* F2, double* Pk, double* CDkw,
+                          double* Ub, double* pb, double* kb, double* omb,
+                          double* hist, int histCap, int* info) {
+    try {
+        Mesh m = loadMesh(path);
+        FlowBoundaryConditions bcs = makeBCs(m, bcKind, bcVals);
+        SolverSettings s = unpackSettings(settings);
+        SSTModel sst(unpackCoeffs(coeffs));
+        sst.variant = static_cast<SSTVariant>(s.sstVariant);
+        SIMPLESolver solver(m, sst, bcs, nu, s);
+        FlowFields f(m);
+        const int n = m.nCells(), nB = f.U.nBnd();
+        for (int i = 0; i < n; ++i) f.U[i] = Vec3(state[0][3*i], state[0][3*i+1], state[0][3*i+2]);
+        for (int i = 0; i < nB; ++i) f.U.bnd(i) = Vec3(state[1][3*i], state[1][3*i+1], state[1][3*i+2]);
+        ScalarField* sf[] = {&f.p, &f.k, &f.omega};
+        for (int q = 0; q < 3; ++q) {
+            for (int i = 0; i < n; ++i) (*sf[q])[i] = state[2 + 2*q][i];
+            for (int i = 0; i < nB; ++i) sf[q]->bnd(i) = state[3 + 2*q][i];
         }
-        info[0] = h.finalIter; info[1] = h.converged; info[2] = h.diverged; info[3] = ne;
-        info[4] = f.turbEstablished ? 1 : 0;
-        return 0;
+        ScalarField* cl[] = {&f.nuT, &f.F1, &f.F2, &f.Pk, &f.CDkw};
+        for (int q = 0; q < 5; ++q)
+            for (int i = 0; i < n; ++i) (*cl[q])[i] = state[8 + q][i];
+        f.turbEstablished = turbEstablished != 0;
+        return solveAndExport(m, solver, f, U, p, k, om, nuT, F1, F2, Pk, CDkw, Ub, pb, kb, omb,
+                              hist, histCap, info);
     } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
 }
 

@@ -1431,6 +1431,32 @@ static void load_state(const om_mesh* m, fields_t* f, const double* U, const dou
     memcpy(f->om, om, 8 * n); memcpy(f->omb, omb, 8 * nB);
 }
 
+/* Warm start (ForwardModel.cpp:56-60: fields = cached.fields; solve(fields)).
+ * state: U[3n] Ub[3nB] p pb k kb omega omegab nuT F1 F2 Pk CDkw (13 arrays). */
+int om_simple_solve_warm(const om_mesh* m, int bcKind, const double* bcVals, double nu,
+                         const double* coeffs, const double* settings,
+                         const double* const* state, int turbEstablished,
+                         double* U, double* p, double* k, double* om, double* nuT,
+                         double* F1, double* F2, double* Pk, double* CDkw,
+                         double* Ub, double* pb, double* kb, double* omb,
+                         double* hist, int histCap, int* info) {
+    solver_t S;
+    solver_init(&S, m, bcKind, bcVals, nu, coeffs, settings);
+    fields_t f = fields_alloc(m);
+    int n = m->nC;
+    load_state(m, &f, state[0], state[1], state[2], state[3], state[4], state[5], state[6], state[7]);
+    memcpy(f.nuT, state[8], 8 * n); memcpy(f.F1, state[9], 8 * n); memcpy(f.F2, state[10], 8 * n);
+    memcpy(f.Pk, state[11], 8 * n); memcpy(f.CDkw, state[12], 8 * n);
+    f.turbEstablished = turbEstablished != 0;
+    hist_t H = simple_solve(&S, &f, hist, histCap);
+    export_fields(m, &f, U, p, k, om, nuT, F1, F2, Pk, CDkw, Ub, pb, kb, omb);
+    info[0] = H.finalIter; info[1] = H.converged; info[2] = H.diverged; info[3] = H.nEntries;
+    info[4] = f.turbEstablished;
+    fields_free(&f);
+    solver_free(&S);
+    return 0;
+}
+
 /* SIMPLESolver::recomputeClosure :1054-1064 */
 static void recompute_closure(solver_t* S, fields_t* f, double* Smag) {
     compute_fields(S, f, Smag);

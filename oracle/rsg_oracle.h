@@ -84,6 +84,17 @@ int om_simple_solve(const om_mesh* m, int bcKind, const double* bcVals, double n
                     double* Ub, double* pb, double* kb, double* omb,
                     double* hist, int histCap, int* info);
 
+/* The same solve from a given state (ForwardModel.cpp:56-60 warm start).
+ * state: U[3n] Ub[3nB] p pb k kb omega omegab nuT F1 F2 Pk CDkw (13 arrays);
+ * turbEstablished: the flag carried with the state (FlowFields.hpp:20-26). */
+int om_simple_solve_warm(const om_mesh* m, int bcKind, const double* bcVals, double nu,
+                         const double* coeffs, const double* settings,
+                         const double* const* state, int turbEstablished,
+                         double* U, double* p, double* k, double* om, double* nuT,
+                         double* F1, double* F2, double* Pk, double* CDkw,
+                         double* Ub, double* pb, double* kb, double* omb,
+                         double* hist, int histCap, int* info);
+
 /* Operator blocks at a fixed state (SIMPLESolver::assemblePreconBlocks). */
 int om_precon_blocks(const om_mesh* m, int bcKind, const double* bcVals, double nu,
                      const double* coeffs, const double* settings,

@@ -160,6 +160,39 @@ def oracle_solve(oracle, path, case, settings, hist_cap=0):
     return out, hist[:6 * info[3]].reshape(-1, 6) if hist_cap else None, list(info)
 
 
+STATE_ORDER = ("U", "Ub", "p", "pb", "k", "kb", "om", "omb", "nuT", "F1", "F2", "Pk", "CDkw")
+
+
+def state_ptrs(state):
+    """The 13-array warm-start state in the order om_simple_solve_warm takes."""
+    arrs = [np.ascontiguousarray(state[k], dtype=np.float64) for k in STATE_ORDER]
+    return arrs, (C.POINTER(C.c_double) * 13)(*[rsg_abi.dptr(a) for a in arrs])
+
+
+def oracle_solve_warm(oracle, path, case, settings, state, turb, coeffs=None, hist_cap=0):
+    """om_simple_solve_warm: the SIMPLE solve from a given state (warm start)."""
+    kind, vals, nu, _ = CASES[case]
+    D = lambda a: (C.c_double * len(a))(*a)
+    m = oracle.om_mesh_load(path.encode())
+    info5 = (C.c_int * 5)()
+    oracle.om_mesh_info(C.c_void_p(m), info5)
+    n, nB = info5[0], info5[1] - info5[2]
+    out = {k: np.zeros(3 * n if k == "U" else n) for k in FIELD_KEYS}
+    bnd = {"Ub": np.zeros(3 * nB), "pb": np.zeros(nB), "kb": np.zeros(nB), "omb": np.zeros(nB)}
+    hist = np.zeros(6 * max(hist_cap, 1))
+    info = (C.c_int * 5)()
+    keep, ptrs = state_ptrs(state)
+    oracle.om_simple_solve_warm(C.c_void_p(m), kind, D(list(vals)), C.c_double(nu),
+                                D(coeffs or rsg_abi.DEFAULT_COEFFS), D(rsg_abi.packed_settings(settings)),
+                                ptrs, int(turb),
+                                *[rsg_abi.dptr(out[k]) for k in FIELD_KEYS],
+                                *[rsg_abi.dptr(bnd[k]) for k in ("Ub", "pb", "kb", "omb")],
+                                rsg_abi.dptr(hist), hist_cap, info)
+    oracle.om_mesh_free(C.c_void_p(m))
+    out.update(bnd)
+    return out, hist[:6 * info[3]].reshape(-1, 6) if hist_cap else None, list(info)
+
+
 def settings_for(case, **kw):
     _, _, _, bf = CASES[case]
     kw.setdefault("body_force", bf)

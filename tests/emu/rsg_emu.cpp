@@ -10,9 +10,12 @@
 #include "rsg_prep.h"
 #include "rsg_member.h"
 
-extern "C" int emu_solve(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
-                         const rsg_settings* s, const double* coeffs, const double* init,
-                         int iterBudget, rsg_result* out) {
+// state == NULL: uniform start from init[6]; otherwise a warm start from
+// U[3n] Ub[3nB] p pb k kb omega omegab nuT F1 F2 Pk CDkw, staged into the
+// member arrays as the engine's admission does.
+static int emu_run(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
+                   const rsg_settings* s, const double* coeffs, const double* init,
+                   const double* const* state, int turbEstablished, int iterBudget, rsg_result* out) {
     rsg::PrepMesh P;
     std::string err;
     if (!rsg::prep_mesh(mesh, P, err)) { fprintf(stderr, "emu: %s\n", err.c_str()); return -1; }
@@ -36,12 +39,24 @@ extern "C" int emu_solve(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, doub
     M.mesh = &md; M.bc = &bc; M.set = &sd; M.nu = nu;
     for (int i = 0; i < 11; ++i) M.coeffs[i] = coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = init[i];
-    M.initFromFields = 0;
+    M.initFromFields = state != nullptr;
     M.iterBudget = iterBudget > 0 ? iterBudget : s->max_iterations;
     M.sc->status = RSGM_RUNNING;
     M.sc->amgBuiltIter = -1;
+    M.sc->coarseIter = -1000000000;
     M.sc->normUx0 = M.sc->normUy0 = M.sc->normP0 = 1.0;
     M.sc->lastK = M.sc->lastOm = 1.0;
+    if (state) {
+        const int n = P.nC, nB = P.nB;
+        M.sc->turbEstablished = turbEstablished;
+        for (int c = 0; c < n; ++c) { M.Ux[c] = state[0][3*c]; M.Uy[c] = state[0][3*c+1]; M.Uz[c] = state[0][3*c+2]; }
+        for (int b = 0; b < nB; ++b) { M.Ubx[b] = state[1][3*b]; M.Uby[b] = state[1][3*b+1]; M.Ubz[b] = state[1][3*b+2]; }
+        memcpy(M.p, state[2], 8 * (size_t)n); memcpy(M.pb, state[3], 8 * (size_t)nB);
+        memcpy(M.k, state[4], 8 * (size_t)n); memcpy(M.kb, state[5], 8 * (size_t)nB);
+        memcpy(M.om, state[6], 8 * (size_t)n); memcpy(M.omb, state[7], 8 * (size_t)nB);
+        double* cl[] = {M.nuT, M.F1, M.F2, M.Pk, M.CDkw};
+        for (int q = 0; q < 5; ++q) memcpy(cl[q], state[8 + q], 8 * (size_t)n);
+    }
     static double red[256];
     static double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
     rsg::Thr t; t.tid = 0; t.bs = 1; t.red = red; t.lds = lds; t.phase = 0;
@@ -67,6 +82,19 @@ extern "C" int emu_solve(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, doub
         out->history_len = len;
     }
     return 0;
+}
+
+extern "C" int emu_solve(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
+                         const rsg_settings* s, const double* coeffs, const double* init,
+                         int iterBudget, rsg_result* out) {
+    return emu_run(mesh, bcd, nu, s, coeffs, init, nullptr, 0, iterBudget, out);
+}
+
+extern "C" int emu_solve_warm(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
+                              const rsg_settings* s, const double* coeffs,
+                              const double* const* state, int turbEstablished, rsg_result* out) {
+    const double init[6] = {0, 0, 0, 0, 0, 0};
+    return emu_run(mesh, bcd, nu, s, coeffs, init, state, turbEstablished, 0, out);
 }
 
 // Stand-alone LDU solve through the same topology preparation and solver

@@ -94,3 +94,29 @@ def test_emulated_standalone_linear_solve(oracle, emu, meshes, solver):
         assert np.array_equal(x, xo) and np.array_equal(st[:4], so)
     else:
         assert st[4] >= 2
+
+
+@pytest.mark.parametrize("case", ["c0", "bfs"])
+def test_emulated_warm_start_bit_exact(oracle, emu, meshes, case):
+    """A warm start (cached state + its turbEstablished flag, other
+    coefficients) on the device algorithm equals the oracle's bit for bit."""
+    from conftest import oracle_solve_warm, state_ptrs
+    s0 = settings_for(case, max_iterations=120, convergence_tol=1e-12, pressure_solver=1, nut_floor_iters=40)
+    base, _, _ = oracle_solve(oracle, meshes[case], case, s0)
+    coeffs = list(A.DEFAULT_COEFFS)
+    coeffs[9] *= 1.05; coeffs[8] *= 0.97
+    s = settings_for(case, max_iterations=90, convergence_tol=1e-12, pressure_solver=1, nut_floor_iters=40,
+                     record_history=1)
+    kind, vals, nu, _ = CASES[case]
+    msh = A.Msh(meshes[case])
+    for turb in (1, 0):
+        o, ho, io = oracle_solve_warm(oracle, meshes[case], case, s, base, turb, coeffs, hist_cap=90)
+        r, a = A.new_result(msh.nC, msh.nF - msh.nIF, 90)
+        keep, ptrs = state_ptrs(base)
+        assert emu.emu_solve_warm(C.byref(msh.desc()), C.byref(msh.bc(kind, vals)), C.c_double(nu), C.byref(s),
+                                  D(coeffs), ptrs, turb, C.byref(r)) == 0
+        assert r.final_iter == io[0] and r.turb_established == io[4]
+        assert np.array_equal(a["history"][:6 * r.history_len].reshape(-1, 6), ho)
+        for k_emu, k_or in (("U", "U"), ("p", "p"), ("k", "k"), ("omega", "om"), ("nuT", "nuT"),
+                            ("U_bnd", "Ub"), ("p_bnd", "pb"), ("k_bnd", "kb"), ("om_bnd", "omb")):
+            assert np.array_equal(a[k_emu], o[k_or]), (turb, k_emu)

@@ -153,3 +153,37 @@ def test_assembly_bit_exact(oracle, ref, meshes, case):
             ref.ref_transpose_stress(path.encode(), A.dptr(coeff), A.dptr(st[0]), A.dptr(st[1]), comp, wz, A.dptr(sr))
             assert np.array_equal(so, sr)
     oracle.om_mesh_free(C.c_void_p(m))
+
+
+@pytest.mark.parametrize("case", ["c0", "bfs"])
+def test_warm_start_bit_exact(oracle, ref, meshes, case):
+    """ForwardModel's warm start (ForwardModel.cpp:56-60): the solve restarted
+    from a cached state with other coefficients, the state's turbEstablished
+    flag carried along, equals the reference's bit for bit."""
+    from conftest import oracle_solve_warm, state_ptrs
+    s0 = settings_for(case, max_iterations=120, convergence_tol=1e-12, pressure_solver=1, nut_floor_iters=40)
+    base, _, _ = oracle_solve(oracle, meshes[case], case, s0)
+    coeffs = list(A.DEFAULT_COEFFS)
+    coeffs[9] *= 1.05; coeffs[8] *= 0.97
+    s = settings_for(case, max_iterations=90, convergence_tol=1e-12, pressure_solver=1, nut_floor_iters=40)
+    for turb in (1, 0):
+        o, ho, io = oracle_solve_warm(oracle, meshes[case], case, s, base, turb, coeffs, hist_cap=90)
+        kind, vals, nu, _ = CASES[case]
+        msh = A.Msh(meshes[case])
+        n, nB = msh.nC, msh.nF - msh.nIF
+        r = {k: np.zeros(3 * n if k == "U" else n) for k in FIELD_KEYS}
+        rb = {"Ub": np.zeros(3 * nB), "pb": np.zeros(nB), "kb": np.zeros(nB), "omb": np.zeros(nB)}
+        hist = np.zeros(6 * 90)
+        info = (C.c_int * 5)()
+        keep, ptrs = state_ptrs(base)
+        assert ref.ref_simple_solve_warm(meshes[case].encode(), kind, D(list(vals)), C.c_double(nu), D(coeffs),
+                                         D(A.packed_settings(s)), ptrs, turb,
+                                         *[A.dptr(r[k]) for k in FIELD_KEYS],
+                                         *[A.dptr(rb[k]) for k in ("Ub", "pb", "kb", "omb")],
+                                         A.dptr(hist), 90, info) == 0
+        assert io == list(info)
+        assert np.array_equal(ho, hist[:6 * info[3]].reshape(-1, 6))
+        for k in FIELD_KEYS:
+            assert np.array_equal(o[k], r[k]), (turb, k)
+        for k in ("Ub", "pb", "kb", "omb"):
+            assert np.array_equal(o[k], rb[k]), (turb, k)
