@@ -36,14 +36,44 @@
 // ------------------------------------------------------------------ kernels
 namespace {
 
+// Widest cells-per-thread instantiation of the resident solvers for a block
+// size (it bounds the kernel's register allocation; RSG_RES_MAXN cells at most).
+constexpr int cmax_for(int bs) { return bs >= 1024 ? 1 : (bs >= 512 ? 3 : 6); }
+
+// Workgroup LDS: reduction scratch, coarsest dense inverse, the resident
+// solvers' two exchange buffers and coarse-level vectors (~75 KB, so two
+// member workgroups share a CU).
+struct LdsArea {
+    double red[128];
+    double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
+    double xb[2 * RSG_RES_MAXN];
+    double cv[3 * RSG_RES_MAXC];
+};
+
+__device__ rsg::Thr make_thr(LdsArea& L, int bs) {
+    rsg::Thr t;
+    t.tid = threadIdx.x; t.bs = bs; t.red = L.red; t.lds = L.lds; t.phase = 0;
+    t.xb0 = L.xb; t.xb1 = L.xb + RSG_RES_MAXN; t.xsel = 0;
+    t.cvec = L.cv; t.cvecCap = RSG_RES_MAXC;
+    return t;
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_simple_chunk(RsgMemberDev* members, RsgScalars* statusOut) {
-    __shared__ double red[128];
-    __shared__ double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
-    rsg::Thr t;
-    t.tid = threadIdx.x; t.bs = BS; t.red = red; t.lds = lds; t.phase = 0;
+    __shared__ LdsArea L;
+    rsg::Thr t = make_thr(L, BS);
     RsgMemberDev* M = members + blockIdx.x;
-    rsg::run_member(t, M);
+#ifdef RSG_PROF
+    const unsigned long long t0 = wall_clock64();
+#endif
+    rsg::run_member<cmax_for(BS)>(t, M);
+#ifdef RSG_PROF
+    if (threadIdx.x == 0 && blockIdx.x < 4096) {
+        rsg_prof_wg[3 * blockIdx.x] = t0;
+        rsg_prof_wg[3 * blockIdx.x + 1] = wall_clock64();
+        rsg_prof_wg[3 * blockIdx.x + 2] = (unsigned long long)__smid();
+    }
+#endif
     __syncthreads();
     if (threadIdx.x == 0) statusOut[blockIdx.x] = *M->sc;
 }
@@ -52,12 +82,10 @@ __global__ __launch_bounds__(BS) void k_simple_chunk(RsgMemberDev* members, RsgS
 template <int BS>
 __global__ __launch_bounds__(BS) void k_ldu_solve(RsgMemberDev* M, int solver, int maxIter, double tol,
                                                   double* stats) {
-    __shared__ double red[128];
-    __shared__ double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
-    rsg::Thr t;
-    t.tid = threadIdx.x; t.bs = BS; t.red = red; t.lds = lds; t.phase = 0;
+    __shared__ LdsArea L;
+    rsg::Thr t = make_thr(L, BS);
     int nl = 0;
-    rsg::SolveRes r = rsg::ldu_solve_member(t, M, solver, maxIter, tol, &nl);
+    rsg::SolveRes r = rsg::ldu_solve_member<cmax_for(BS)>(t, M, solver, maxIter, tol, &nl);
     if (t.tid == 0) {
         stats[0] = r.iterations; stats[1] = r.finalRes; stats[2] = r.initialRes; stats[3] = r.converged;
         stats[4] = nl;
@@ -160,7 +188,7 @@ struct rsg_context {
     int nextMesh = 1, nextProblem = 1;
     int64_t nextTicket = 1;
     int itersPerLaunch = 200;
-    int blockSize = 512;
+    int blockSize = 256;
     int maxSlots = 4096;
     std::thread engine;
     bool stop = false;
@@ -756,6 +784,13 @@ int rsg_prof_read(double* out) {
     for (int i = 0; i < 32; ++i) out[i] = 1e-8 * (double)t[i];
     const unsigned long long z[32] = {};
     if (hipMemcpyToSymbol(HIP_SYMBOL(rsg_prof_ticks), z, sizeof z) != hipSuccess) return RSG_ERR_DEVICE;
+    return RSG_OK;
+}
+
+// Per-workgroup start / end ticks and CU id of the last SIMPLE launch.
+int rsg_prof_wg_read(uint64_t* out, int n) {
+    if (n > 4096) n = 4096;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(rsg_prof_wg), 24 * (size_t)n) != hipSuccess) return RSG_ERR_DEVICE;
     return RSG_OK;
 }
 #endif

@@ -15,6 +15,15 @@
 #define RSG_MIN_COARSE 50     // AMGSolver::minCoarseSize_ (LinearSolver.hpp:509)
 #define RSG_COARSE_DENSE 64   // coarsest levels up to this size are inverted densely
 #define RSG_ELL_K 4           // internal faces per cell held in registers (resident solves)
+// Resident solves: at most RSG_RES_MAXN cells per member (LDS exchange
+// buffers) and RSG_RES_MAXC coarse AMG rows (LDS level vectors).
+#ifdef RSG_EMULATE
+#define RSG_RES_MAXN 2048
+#define RSG_RES_MAXC 2048
+#else
+#define RSG_RES_MAXN 1536
+#define RSG_RES_MAXC 768
+#endif
 
 // boundary-face flag bits (from the face's patch type)
 #define RSG_BF_WALL 1

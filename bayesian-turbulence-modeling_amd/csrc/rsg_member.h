@@ -49,6 +49,7 @@
 // wall-clock ticks (100 MHz) between consecutive marks to rsg_prof_ticks[slot].
 #if defined(RSG_PROF) && !defined(RSG_EMULATE)
 __device__ unsigned long long rsg_prof_ticks[32];
+__device__ unsigned long long rsg_prof_wg[3 * 4096];   // per workgroup: start, end, CU id
 #define RSG_TMARK(slot)                                                          \
     do {                                                                         \
         __syncthreads();                                                         \
@@ -74,6 +75,11 @@ struct Thr {
     double* red;    // LDS reduction scratch: 2 phases x 16 waves x 4 values
     double* lds;    // LDS tile for the coarsest-level dense inverse [64 x 64]
     int phase;
+    // resident solves (null xb0: not available)
+    double *xb0, *xb1;   // LDS exchange buffers [RSG_RES_MAXN] each
+    int xsel;
+    double* cvec;        // LDS coarse-level vectors x | b | r, cvecCap rows each
+    int cvecCap;
 };
 
 #ifdef RSG_EMULATE
@@ -528,8 +534,9 @@ RSG_D void amg_setup_serial(const RsgMeshDev* m, RsgAmgDev& A) {
     A.lvN[RSG_MAX_LEVELS] = L;   // level count
 }
 
-// Parallel refresh of every level's values from the current LDU matrix.
-RSG_D void amg_refresh_values(Ctx& C) {
+// Parallel refresh of level 0's CSR values from the current LDU matrix and,
+// with coarse set, of every coarser level's Galerkin values.
+RSG_D void amg_refresh_values(Ctx& C, bool coarse) {
     Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
     RsgAmgDev& A = M->amg;
     const int nC = C.nC, nIF = C.nIF, nnz0 = m->nnz0;
@@ -539,6 +546,7 @@ RSG_D void amg_refresh_values(Ctx& C) {
     }
     FOR_CELLS(c) A.dg[c] = M->diag[c];
     RSG_SYNC();
+    if (!coarse) return;
     const int L = A.lvN[RSG_MAX_LEVELS];
     for (int l = 0; l + 1 < L; ++l) {
         const int nc = A.lvN[l + 1];
@@ -780,8 +788,19 @@ RSG_D SolveRes pcg(Ctx& C, double* x, int maxIter, double tol, bool useAmg) {
     return res;
 }
 
+}  // namespace rsg
+#include "rsg_resident.h"
+namespace rsg {
+
+// One linear solve: the resident solvers when the member's cells fit the
+// workgroup's registers (and, for the V-cycle, its coarse vectors fit LDS),
+// the global-memory solvers otherwise.
+template <int CMAX>
 RSG_D SolveRes lsolve(Ctx& C, int code, double* x, bool amgReady) {
     const RsgSettingsDev* s = C.s;
+    const bool amg = code == 0 && amgReady;
+    if (res_fits(C, CMAX) && (!amg || res_amg_fits(C)))
+        return lsolve_res<CMAX>(C, code, x, amg, s->innerIter, s->innerTol);
     if (code == 0) return pcg(C, x, s->innerIter, s->innerTol, amgReady);
     if (code == 1) return pcg(C, x, s->innerIter, s->innerTol, false);
     return bicgstab(C, x, s->innerIter, s->innerTol);
@@ -1046,6 +1065,7 @@ struct Scal {
     double normUx0, normUy0, normP0, lastK, lastOm;
 };
 
+template <int CMAX>
 RSG_D void simple_iteration(Ctx& C, Scal& S) {
     Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const RsgSettingsDev* st = C.s;
     const int nC = C.nC, nIF = C.nIF, nB = C.nB;
@@ -1094,12 +1114,13 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     RSG_TMARK(0);
 
     // 2. momentum components, each assembled from the latest U
-    SolveRes rU[3];
+    SolveRes rU0 = {0, 0.0, 0.0, 0}, rU1 = rU0, rU2 = rU0;   // named, not indexed: no private-memory array
     for (int comp = 0; comp < 3; ++comp) {
         if (comp > 0) { grad_u(C, comp - 1, comp); RSG_SYNC(); }
         assemble_momentum(C, comp);
         RSG_TMARK(1);
-        rU[comp] = lsolve(C, st->mSolver, ucomp(M, comp), false);
+        const SolveRes r = lsolve<CMAX>(C, st->mSolver, ucomp(M, comp), false);
+        if (comp == 0) rU0 = r; else if (comp == 1) rU1 = r; else rU2 = r;
         RSG_SYNC();
         RSG_TMARK(2);
     }
@@ -1111,15 +1132,26 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     RSG_TMARK(3);
     bool amgReady = false;
     if (st->pSolver == 0) {
-        amg_refresh_values(C);
-        RSG_TMARK(4);
-        if (S.amgBuiltIter < 0 || iter - S.amgBuiltIter >= st->amgRefresh) {
+        // Aggregation every amgRefresh iterations; the coarse Galerkin values
+        // and the coarsest inverse every coarseRefresh iterations (between
+        // refreshes the V-cycle's coarse levels lag the current matrix, which
+        // changes the preconditioner, not the system being solved).
+        const bool rebuild = S.amgBuiltIter < 0 || iter - S.amgBuiltIter >= st->amgRefresh;
+        const bool coarse = rebuild || iter - S.coarseIter >= st->coarseRefresh;
+        if (rebuild) {
+            amg_refresh_values(C, false);
+            RSG_TMARK(4);
             if (t.tid == 0) amg_setup_serial(m, M->amg);
             RSG_SYNC();
             S.amgBuiltIter = iter;
             RSG_TMARK(5);
+        } else {
+            // level 0's CSR copy feeds only the global-memory V-cycle and the Galerkin products
+            const bool resident = res_fits(C, CMAX) && res_amg_fits(C);
+            if (coarse || !resident) amg_refresh_values(C, coarse);
+            RSG_TMARK(4);
         }
-        if (S.amgBuiltIter == iter || iter - S.coarseIter >= st->coarseRefresh) {
+        if (coarse) {
             amg_coarse_inverse(C);
             S.coarseIter = iter;
             RSG_TMARK(6);
@@ -1128,7 +1160,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     }
     FOR_CELLS(c) M->pp[c] = 0.0;
     RSG_SYNC();
-    SolveRes rP = lsolve(C, st->pSolver, M->pp, amgReady);
+    SolveRes rP = lsolve<CMAX>(C, st->pSolver, M->pp, amgReady);
     RSG_SYNC();
     RSG_TMARK(7);
     FOR_BFACES(b) M->ppb[b] = (m->bFlags[b] & RSG_BF_OUTLET) ? 0.0 : M->pp[m->own[nIF + b]];
@@ -1154,7 +1186,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     if (turbUpdate) {
         assemble_turb(C, false);
         RSG_TMARK(9);
-        rK = lsolve(C, st->tSolver, M->k, false);
+        rK = lsolve<CMAX>(C, st->tSolver, M->k, false);
         RSG_SYNC();
         RSG_TMARK(10);
         FOR_CELLS(c) M->k[c] = dmax(st->kMin, dmin(st->kMax, M->k[c]));
@@ -1163,7 +1195,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
         RSG_SYNC();
         assemble_turb(C, true);
         RSG_TMARK(11);
-        rOm = lsolve(C, st->tSolver, M->om, false);
+        rOm = lsolve<CMAX>(C, st->tSolver, M->om, false);
         RSG_SYNC();
         RSG_TMARK(12);
         FOR_CELLS(c) M->om[c] = dmax(st->omMin, dmin(1e15, M->om[c]));
@@ -1197,13 +1229,13 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     }
 
     // 6. residual bookkeeping and convergence / divergence tests (:934-1035)
-    const double eUx = rU[0].initialRes, eUy = rU[1].initialRes, eUz = rU[2].initialRes;
+    const double eUx = rU0.initialRes, eUy = rU1.initialRes, eUz = rU2.initialRes;
     const double eP = rP.initialRes;
     const double eK = turbActive ? S.lastK : 0.0, eOm = turbActive ? S.lastOm : 0.0;
     if (t.tid == 0) {
         RsgScalars* sc = M->sc;
-        sc->innerIters[0] += rU[0].iterations; sc->innerIters[1] += rU[1].iterations;
-        sc->innerIters[2] += rU[2].iterations; sc->innerIters[3] += rP.iterations;
+        sc->innerIters[0] += rU0.iterations; sc->innerIters[1] += rU1.iterations;
+        sc->innerIters[2] += rU2.iterations; sc->innerIters[3] += rP.iterations;
         sc->innerIters[4] += rK.iterations; sc->innerIters[5] += rOm.iterations;
         if (st->recordHistory && M->hist && iter < M->histCap) {
             double* h = M->hist + 6 * (long)iter;
@@ -1239,6 +1271,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
 }
 
 // Runs up to M->iterBudget outer iterations of one member.
+template <int CMAX>
 RSG_D void run_member(Thr t, RsgMemberDev* M) {
     Ctx C;
     C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
@@ -1272,7 +1305,14 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
 #if defined(RSG_PROF) && !defined(RSG_EMULATE)
     C.tmark = wall_clock64();
 #endif
-    for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) simple_iteration(C, S);
+    // the coarsest dense inverse lives in LDS: restore it for this launch
+    if (S.amgBuiltIter >= 0 && sc->coarseDense) {
+        const RsgAmgDev& A = M->amg;
+        const int nL = A.lvN[A.lvN[RSG_MAX_LEVELS] - 1];
+        for (int idx = C.t.tid; idx < nL * RSG_COARSE_DENSE; idx += C.t.bs) C.t.lds[idx] = A.cinv[idx];
+        RSG_SYNC();
+    }
+    for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) simple_iteration<CMAX>(C, S);
     if (C.t.tid == 0) {
         sc->iter = S.iter; sc->status = S.status; sc->turbEstablished = S.turbEstablished;
         sc->finalIter = S.finalIter; sc->amgBuiltIter = S.amgBuiltIter; sc->coarseIter = S.coarseIter;
@@ -1285,6 +1325,7 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
 // One linear solve (ILinearSolver::solve, LinearSolver.hpp:115-120) on the
 // LDU system already stored in M (diag/upper/lower/src), x = M->pp in/out.
 // solver: RSG_SOLVER_* codes (0 AMG-PCG, 1 Jacobi-PCG, 2 BiCGSTAB).
+template <int CMAX>
 RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter, double tol, int* nLevels) {
     Ctx C;
     C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
@@ -1297,14 +1338,15 @@ RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter,
         for (int e = t.tid; e < m->nnz0; e += t.bs) A.ci[e] = m->ci0[e];
         if (t.tid == 0) { A.lvN[0] = m->nC; A.lvRow[0] = 0; A.lvRp[0] = 0; A.lvNz[0] = 0; A.lvN[RSG_MAX_LEVELS] = 1; }
         RSG_SYNC();
-        amg_refresh_values(C);
+        amg_refresh_values(C, false);
         if (t.tid == 0) amg_setup_serial(m, A);
         RSG_SYNC();
         amg_coarse_inverse(C);
         amg = true;
     }
     SolveRes r;
-    if (solver == 2) r = bicgstab(C, M->pp, maxIter, tol);
+    if (res_fits(C, CMAX) && (!amg || res_amg_fits(C))) r = lsolve_res<CMAX>(C, solver, M->pp, amg, maxIter, tol);
+    else if (solver == 2) r = bicgstab(C, M->pp, maxIter, tol);
     else r = pcg(C, M->pp, maxIter, tol, amg);
     RSG_SYNC();
     *nLevels = M->amg.lvN[RSG_MAX_LEVELS];

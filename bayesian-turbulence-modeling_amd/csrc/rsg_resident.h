@@ -1,0 +1,535 @@
+// rsg_resident.h -- register/LDS-resident Krylov solvers and V-cycle for
+// members whose cells fit in the workgroup's registers.  Included by
+// rsg_member.h after the global-memory solvers, whose helpers it uses.
+//
+// Thread t owns cells c = t + j*bs, j < CPT.  For the duration of one linear
+// solve each thread keeps, in registers, its cells' matrix rows (diagonal and
+// the internal-face coefficients in ascending face order, from the mesh's ELL
+// image) and every Krylov vector.  The only cross-thread traffic is the
+// vector a matrix-vector product gathers, published through one of two LDS
+// buffers (double-buffered, so each product costs one workgroup barrier) and
+// the block reductions.  Per cell the arithmetic, and per thread the order of
+// partial sums, are those of the global-memory solvers above (the row product
+// adds diag*x first and then the face terms in ascending face order, as
+// ldu_row does), so the results are bit-identical to them.
+//
+// The AMG V-cycle keeps level 0 in registers and levels >= 1 in LDS vectors;
+// the coarse CSR values stay in global memory and the coarsest dense inverse
+// lives in LDS for the whole launch.
+#pragma once
+
+namespace rsg {
+
+// Matrix rows of the owned cells.
+template <int CPT> struct ResRows {
+    double dg[CPT];
+    double a[CPT][RSG_ELL_K];
+    int nb[CPT][RSG_ELL_K];
+    int cnt[CPT];
+};
+
+RSG_D double* res_buf(Thr& t) {
+    double* b = t.xsel ? t.xb1 : t.xb0;
+    t.xsel ^= 1;
+    return b;
+}
+
+template <int CPT> RSG_D void res_load_rows(const Ctx& C, ResRows<CPT>& R) {
+    const Thr& t = C.t; const RsgMeshDev* m = C.m; const RsgMemberDev* M = C.M; const int nC = C.nC;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        R.dg[j] = 0.0; R.cnt[j] = 0;
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k) { R.a[j][k] = 0.0; R.nb[j][k] = 0; }
+        if (c < nC) {
+            R.dg[j] = M->diag[c];
+            R.cnt[j] = m->ellCnt[c];
+#pragma unroll
+            for (int k = 0; k < RSG_ELL_K; ++k) {
+                const int code = m->ellFace[k * nC + c];
+                R.nb[j][k] = m->ellNb[k * nC + c];
+                if (code >= 0) R.a[j][k] = (code & 1) ? M->lower[code >> 1] : M->upper[code >> 1];
+            }
+        }
+    }
+}
+
+// y = (A x)_c for owned cell j: diag first, then the face terms in face order.
+template <int CPT> RSG_D double res_row(const ResRows<CPT>& R, int j, double xc, const double* xs) {
+    double y = R.dg[j] * xc;
+#pragma unroll
+    for (int k = 0; k < RSG_ELL_K; ++k)
+        if (k < R.cnt[j]) y += R.a[j][k] * xs[R.nb[j][k]];
+    return y;
+}
+
+template <int CPT> RSG_D void res_put(const Thr& t, int nC, const double (&v)[CPT], double* xs) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        if (c < nC) xs[c] = v[j];
+    }
+}
+
+template <int CPT> RSG_D void res_get(const Thr& t, int nC, const double* g, double (&v)[CPT]) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        v[j] = (c < nC) ? g[c] : 0.0;
+    }
+}
+
+template <int CPT> RSG_D void res_store(const Thr& t, int nC, const double (&v)[CPT], double* g) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        if (c < nC) g[c] = v[j];
+    }
+}
+
+// BiCGSTAB (the recurrence of bicgstab() above, LinearSolver.hpp:193-245).
+template <int CPT> RSG_D SolveRes bicgstab_res(Ctx& C, double* xg, int maxIter, double tol) {
+    Thr& t = C.t; const int nC = C.nC; const double* b = C.M->src;
+    ResRows<CPT> R;
+    res_load_rows(C, R);
+    double x[CPT], r[CPT], rh[CPT], p[CPT], v[CPT], s[CPT], tt[CPT], ph[CPT], sh[CPT];
+    res_get(t, nC, xg, x);
+    double* xs = res_buf(t);
+    res_put(t, nC, x, xs);
+    RSG_SYNC();
+    SolveRes res = {0, 0.0, 0.0, 0};
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        r[j] = rh[j] = p[j] = v[j] = s[j] = tt[j] = ph[j] = sh[j] = 0.0;
+        if (c < nC) {
+            const double rc = b[c] - res_row(R, j, x[j], xs);
+            r[j] = rc; rh[j] = rc;
+            acc[0] += rc * rc;
+        }
+    }
+    bsum(t, acc);
+    const double r0 = sqrt(acc[0]);
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return res; }
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    double rhoN = acc[0];
+    bool updated = false;
+    for (int it = 0; it < maxIter; ++it) {
+        if (fabs(rhoN) < 1e-30) break;
+        const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const double pc = r[j] + beta * (p[j] - omega * v[j]);
+            p[j] = pc;
+            ph[j] = jac(pc, R.dg[j]);
+        }
+        xs = res_buf(t);
+        res_put(t, nC, ph, xs);
+        RSG_SYNC();
+        double a1[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            if (c < nC) {
+                const double vc = res_row(R, j, ph[j], xs);
+                v[j] = vc;
+                a1[0] += rh[j] * vc;
+            }
+        }
+        bsum(t, a1);
+        const double rv = a1[0];
+        if (fabs(rv) < 1e-30) break;
+        alpha = rhoN / rv;
+        double a2[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            const double sc = r[j] - alpha * v[j];
+            s[j] = sc;
+            sh[j] = jac(sc, R.dg[j]);
+            if (c < nC) a2[0] += sc * sc;
+        }
+        xs = res_buf(t);
+        res_put(t, nC, sh, xs);       // published by the reduction's barrier
+        bsum(t, a2);
+        const double sn = sqrt(a2[0]);
+        if (sn / r0 < tol) {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) x[j] += alpha * ph[j];
+            res.iterations = it + 1; res.finalRes = sn / r0; res.initialRes = r0; res.converged = 1;
+            updated = true;
+            break;
+        }
+        double a3[2] = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            if (c < nC) {
+                const double tc = res_row(R, j, sh[j], xs);
+                tt[j] = tc;
+                a3[0] += tc * tc;
+                a3[1] += tc * s[j];
+            }
+        }
+        bsum(t, a3);
+        omega = (a3[0] > 1e-30) ? a3[1] / a3[0] : 0.0;
+        double a4[2] = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            x[j] += alpha * ph[j] + omega * sh[j];
+            const double rc = s[j] - omega * tt[j];
+            r[j] = rc;
+            if (c < nC) { a4[0] += rc * rc; a4[1] += rh[j] * rc; }
+        }
+        updated = true;
+        bsum(t, a4);
+        const double rn = sqrt(a4[0]);
+        res.iterations = it + 1;
+        res.finalRes = rn / r0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+        if (fabs(omega) < 1e-30) break;
+        rho = rhoN;
+        rhoN = a4[1];
+    }
+    if (updated) res_store(t, nC, x, xg);
+    return res;
+}
+
+// Damped-Jacobi sweep on level 0: xo = xi + wJ (b - A xi) / d (xi gathered from xs).
+template <int CPT> RSG_D void res_jacobi0(const Thr& t, int nC, const ResRows<CPT>& R, const double (&b)[CPT],
+                                          const double (&xi)[CPT], const double* xs, double (&xo)[CPT]) {
+    const double wJ = 2.0 / 3.0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        double o = xi[j];
+        if (c < nC) {
+            const double s = b[j] - res_row(R, j, xi[j], xs);
+            const double d = R.dg[j];
+            o = (fabs(d) > 1e-30) ? xi[j] + wJ * s / d : xi[j];
+        }
+        xo[j] = o;
+    }
+}
+
+// Damped-Jacobi sweep on a coarse level held in LDS (CSR from global memory).
+RSG_D void res_jacobiL(const Thr& t, int n, const int* rp, const int* ci, const double* v, const double* dg,
+                       const double* b, const double* xin, double* xout, bool xzero) {
+    const double wJ = 2.0 / 3.0;
+    for (int i = t.tid; i < n; i += t.bs) {
+        double s = b[i];
+        if (!xzero)
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xin[ci[e]];
+        const double d = dg[i];
+        const double x0 = xzero ? 0.0 : xin[i];
+        xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+    }
+}
+
+// z = V-cycle(rin): the cycle of amg_vcycle() with level 0 in registers.
+template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const double (&rin)[CPT],
+                                         double (&z)[CPT]) {
+    Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg; const int nC = C.nC;
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    const double wJ = 2.0 / 3.0;
+    const int base = A.lvRow[1];                  // first coarse row in the level arrays
+    double* cx = t.cvec;                          // coarse x / b / r, rows base.. in LDS
+    double* cb = t.cvec + t.cvecCap;
+    double* cr = t.cvec + 2 * t.cvecCap;
+    double x0[CPT], tmp[CPT];
+    if (L == 1) {
+        // level 0 is the coarsest: dense inverse (or Jacobi sweeps) on the whole vector
+        double* xs = res_buf(t);
+        res_put(t, nC, rin, xs);
+        RSG_SYNC();
+        if (M->sc->coarseDense) {
+            const int DN = RSG_COARSE_DENSE;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const int c = t.tid + j * t.bs;
+                double s = 0.0;
+                if (c < nC)
+                    for (int q = 0; q < nC; ++q) s += t.lds[c * DN + q] * xs[q];
+                z[j] = s;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) z[j] = (fabs(R.dg[j]) > 1e-30) ? wJ * rin[j] / R.dg[j] : 0.0;
+            for (int sweep = 0; sweep < 50; ++sweep) {
+                xs = res_buf(t);
+                res_put(t, nC, z, xs);
+                RSG_SYNC();
+                res_jacobi0(t, nC, R, rin, z, xs, tmp);
+#pragma unroll
+                for (int j = 0; j < CPT; ++j) z[j] = tmp[j];
+            }
+        }
+        return;
+    }
+    // level 0 down: two sweeps from zero, residual, restriction
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) x0[j] = (fabs(R.dg[j]) > 1e-30) ? wJ * rin[j] / R.dg[j] : 0.0;
+    double* xs = res_buf(t);
+    res_put(t, nC, x0, xs);
+    RSG_SYNC();
+    res_jacobi0(t, nC, R, rin, x0, xs, tmp);
+    xs = res_buf(t);
+    res_put(t, nC, tmp, xs);
+    RSG_SYNC();
+    double* rs = res_buf(t);
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        x0[j] = tmp[j];
+        if (c < nC) rs[c] = rin[j] - res_row(R, j, tmp[j], xs);
+    }
+    RSG_SYNC();
+    {
+        const int nc = A.lvN[1];
+        const int* aP = A.aggPtr + A.lvRp[1];
+        const int* aI = A.aggIdx + A.lvRow[0];
+        double* b1 = cb + (A.lvRow[1] - base);
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += rs[aI[q]];
+            b1[I] = s;
+        }
+    }
+    // coarse levels down (row I of a level is owned by thread I % bs throughout)
+    for (int l = 1; l + 1 < L; ++l) {
+        const int n = A.lvN[l];
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
+        const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
+        const int off = A.lvRow[l] - base;
+        double* x = cx + off; double* bb = cb + off; double* r = cr + off;
+        res_jacobiL(t, n, rp, ci, v, dg, bb, x, r, true);
+        RSG_SYNC();
+        res_jacobiL(t, n, rp, ci, v, dg, bb, r, x, false);
+        RSG_SYNC();
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = 0.0;
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+            r[i] = bb[i] - s;
+        }
+        RSG_SYNC();
+        const int nc = A.lvN[l + 1];
+        const int* aP = A.aggPtr + A.lvRp[l + 1];
+        const int* aI = A.aggIdx + A.lvRow[l];
+        double* b1 = cb + (A.lvRow[l + 1] - base);
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
+            b1[I] = s;
+        }
+    }
+    RSG_SYNC();
+    {   // coarsest level
+        const int l = L - 1, n = A.lvN[l];
+        const int off = A.lvRow[l] - base;
+        double* x = cx + off; double* bb = cb + off;
+        if (M->sc->coarseDense) {
+            const int DN = RSG_COARSE_DENSE;
+            for (int i = t.tid; i < n; i += t.bs) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j) s += t.lds[i * DN + j] * bb[j];
+                x[i] = s;
+            }
+        } else {
+            const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
+            const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
+            double* r = cr + off;
+            res_jacobiL(t, n, rp, ci, v, dg, bb, x, x, true);
+            RSG_SYNC();
+            for (int s = 0; s < 25; ++s) {
+                res_jacobiL(t, n, rp, ci, v, dg, bb, x, r, false);
+                RSG_SYNC();
+                res_jacobiL(t, n, rp, ci, v, dg, bb, r, x, false);
+                RSG_SYNC();
+            }
+        }
+        RSG_SYNC();
+    }
+    // coarse levels up
+    for (int l = L - 2; l >= 1; --l) {
+        const int n = A.lvN[l];
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
+        const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
+        const int off = A.lvRow[l] - base;
+        double* x = cx + off; double* bb = cb + off; double* r = cr + off;
+        const int* map = A.map + A.lvRow[l];
+        const double* xc = cx + (A.lvRow[l + 1] - base);
+        for (int i = t.tid; i < n; i += t.bs) x[i] += xc[map[i]];
+        RSG_SYNC();
+        res_jacobiL(t, n, rp, ci, v, dg, bb, x, r, false);
+        RSG_SYNC();
+        res_jacobiL(t, n, rp, ci, v, dg, bb, r, x, false);
+        RSG_SYNC();
+    }
+    // level 0 up: prolongation and two sweeps
+    {
+        const int* map = A.map + A.lvRow[0];
+        const double* xc = cx + (A.lvRow[1] - base);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            if (c < nC) x0[j] += xc[map[c]];
+        }
+    }
+    xs = res_buf(t);
+    res_put(t, nC, x0, xs);
+    RSG_SYNC();
+    res_jacobi0(t, nC, R, rin, x0, xs, tmp);
+    xs = res_buf(t);
+    res_put(t, nC, tmp, xs);
+    RSG_SYNC();
+    res_jacobi0(t, nC, R, rin, tmp, xs, z);
+}
+
+// PCG with Jacobi or V-cycle preconditioning (the recurrence of pcg() above).
+template <int CPT> RSG_D SolveRes pcg_res(Ctx& C, double* xg, int maxIter, double tol, bool useAmg) {
+    Thr& t = C.t; const int nC = C.nC; const double* b = C.M->src;
+    ResRows<CPT> R;
+    res_load_rows(C, R);
+    double x[CPT], r[CPT], z[CPT], p[CPT], q[CPT];
+    res_get(t, nC, xg, x);
+    double* xs = res_buf(t);
+    res_put(t, nC, x, xs);
+    RSG_SYNC();
+    SolveRes res = {0, 0.0, 0.0, 0};
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = t.tid + j * t.bs;
+        r[j] = 0.0;
+        if (c < nC) {
+            const double rc = b[c] - res_row(R, j, x[j], xs);
+            r[j] = rc;
+            acc[0] += rc * rc;
+        }
+    }
+    bsum(t, acc);
+    const double r0 = sqrt(acc[0]);
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return res; }
+    double rzv[1] = {0.0};
+    if (useAmg) {
+        vcycle_res(C, R, r, z);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            p[j] = z[j];
+            if (c < nC) rzv[0] += r[j] * z[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            const double zc = jac(r[j], R.dg[j]);
+            z[j] = zc; p[j] = zc;
+            if (c < nC) rzv[0] += r[j] * zc;
+        }
+    }
+    bsum(t, rzv);
+    double rz = rzv[0];
+    bool updated = false;
+    for (int it = 0; it < maxIter; ++it) {
+        xs = res_buf(t);
+        res_put(t, nC, p, xs);
+        RSG_SYNC();
+        double a1[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            q[j] = 0.0;
+            if (c < nC) {
+                const double qc = res_row(R, j, p[j], xs);
+                q[j] = qc;
+                a1[0] += p[j] * qc;
+            }
+        }
+        bsum(t, a1);
+        const double pAp = a1[0];
+        if (fabs(pAp) < 1e-30) break;
+        const double alpha = rz / pAp;
+        double a2[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int c = t.tid + j * t.bs;
+            x[j] += alpha * p[j];
+            const double rc = r[j] - alpha * q[j];
+            r[j] = rc;
+            if (c < nC) a2[0] += rc * rc;
+        }
+        updated = true;
+        bsum(t, a2);
+        const double rn = sqrt(a2[0]);
+        res.iterations = it + 1;
+        res.finalRes = rn / r0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+        double a3[1] = {0.0};
+        if (useAmg) {
+            vcycle_res(C, R, r, z);
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const int c = t.tid + j * t.bs;
+                if (c < nC) a3[0] += r[j] * z[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const int c = t.tid + j * t.bs;
+                const double zc = jac(r[j], R.dg[j]);
+                z[j] = zc;
+                if (c < nC) a3[0] += r[j] * zc;
+            }
+        }
+        bsum(t, a3);
+        const double rzNew = a3[0];
+        const double beta = rzNew / (rz + 1e-30);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) p[j] = z[j] + beta * p[j];
+        rz = rzNew;
+    }
+    if (updated) res_store(t, nC, x, xg);
+    return res;
+}
+
+// Whether the resident path applies to this member's system.
+RSG_D bool res_fits(const Ctx& C, int cptMax) {
+    return C.m->ellK > 0 && C.nC <= cptMax * C.t.bs && C.nC <= RSG_RES_MAXN && C.t.xb0 != nullptr;
+}
+
+// Whether the V-cycle's coarse vectors fit the LDS area.
+RSG_D bool res_amg_fits(const Ctx& C) {
+    const RsgAmgDev& A = C.M->amg;
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    if (L <= 1) return true;
+    const int rows = A.lvRow[L - 1] + A.lvN[L - 1] - A.lvRow[1];
+    return rows <= C.t.cvecCap;
+}
+
+template <int CPT> RSG_D SolveRes lsolve_cpt(Ctx& C, int code, double* x, bool amg, int maxIter, double tol) {
+    if (code == 2) return bicgstab_res<CPT>(C, x, maxIter, tol);
+    return pcg_res<CPT>(C, x, maxIter, tol, amg);
+}
+
+// Cells per thread rounded up to an instantiated width (CMAX: the widest one
+// the kernel is built with; it bounds the kernel's register allocation).
+template <int CMAX> RSG_D SolveRes lsolve_res(Ctx& C, int code, double* x, bool amg, int maxIter, double tol) {
+#ifdef RSG_EMULATE
+    return lsolve_cpt<RSG_RES_MAXN>(C, code, x, amg, maxIter, tol);
+#else
+    const int cpt = (C.nC + C.t.bs - 1) / C.t.bs;
+    if constexpr (CMAX >= 2) { if (cpt == 2) return lsolve_cpt<2>(C, code, x, amg, maxIter, tol); }
+    if constexpr (CMAX >= 3) { if (cpt == 3) return lsolve_cpt<3>(C, code, x, amg, maxIter, tol); }
+    if constexpr (CMAX >= 4) { if (cpt == 4) return lsolve_cpt<4>(C, code, x, amg, maxIter, tol); }
+    if constexpr (CMAX >= 6) { if (cpt == 5 || cpt == 6) return lsolve_cpt<6>(C, code, x, amg, maxIter, tol); }
+    return lsolve_cpt<1>(C, code, x, amg, maxIter, tol);
+#endif
+}
+
+}  // namespace rsg

@@ -314,7 +314,7 @@ PYBIND11_MODULE(rans_sst_py, m) {
           "Solve one theta per ForwardModel concurrently on the device; results in input order.");
 
     m.def("engine_set_launch", [](int iters, int block) { Engine::get().setLaunch(iters, block); },
-          py::arg("iters_per_launch"), py::arg("block_size") = 512);
+          py::arg("iters_per_launch"), py::arg("block_size") = 256);
     m.def("ticket_states",
           [](const std::vector<int64_t>& tickets) {
               std::vector<int32_t> st(tickets.size(), 0);

@@ -61,7 +61,8 @@ def emu():
     out_dir = os.path.join(TESTS, "emu", "_build")
     path = os.path.join(out_dir, "librsg_emu.so")
     srcs = [os.path.join(TESTS, "emu", "rsg_emu.cpp"), os.path.join(PKG, "csrc", "rsg_member.h"),
-            os.path.join(PKG, "csrc", "rsg_prep.h"), os.path.join(PKG, "csrc", "rsg_layout.h")]
+            os.path.join(PKG, "csrc", "rsg_prep.h"), os.path.join(PKG, "csrc", "rsg_layout.h"),
+            os.path.join(PKG, "csrc", "rsg_resident.h")]
     if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(s) for s in srcs):
         os.makedirs(out_dir, exist_ok=True)
         subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-shared", "-fPIC", "-w",

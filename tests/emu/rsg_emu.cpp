@@ -10,6 +10,24 @@
 #include "rsg_prep.h"
 #include "rsg_member.h"
 
+// One emulated "workgroup" of a single thread.  The resident solvers (LDS
+// exchange buffers, registers) are used unless switched off, so both solver
+// paths can be held against the oracle.
+static int g_resident = 1;
+extern "C" void emu_set_resident(int on) { g_resident = on; }
+
+static rsg::Thr emu_thr() {
+    static double red[256];
+    static double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
+    static double xb[2 * RSG_RES_MAXN];
+    static double cv[3 * RSG_RES_MAXC];
+    rsg::Thr t;
+    t.tid = 0; t.bs = 1; t.red = red; t.lds = lds; t.phase = 0;
+    t.xb0 = g_resident ? xb : nullptr; t.xb1 = xb + RSG_RES_MAXN; t.xsel = 0;
+    t.cvec = cv; t.cvecCap = RSG_RES_MAXC;
+    return t;
+}
+
 // state == NULL: uniform start from init[6]; otherwise a warm start from
 // U[3n] Ub[3nB] p pb k kb omega omegab nuT F1 F2 Pk CDkw, staged into the
 // member arrays as the engine's admission does.
@@ -57,10 +75,8 @@ static int emu_run(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
         double* cl[] = {M.nuT, M.F1, M.F2, M.Pk, M.CDkw};
         for (int q = 0; q < 5; ++q) memcpy(cl[q], state[8 + q], 8 * (size_t)n);
     }
-    static double red[256];
-    static double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
-    rsg::Thr t; t.tid = 0; t.bs = 1; t.red = red; t.lds = lds; t.phase = 0;
-    while (M.sc->status == RSGM_RUNNING) rsg::run_member(t, &M);
+    rsg::Thr t = emu_thr();
+    while (M.sc->status == RSGM_RUNNING) rsg::run_member<RSG_RES_MAXN>(t, &M);
     const int n = P.nC, nB = P.nB;
     RsgScalars* sc = M.sc;
     out->status = RSG_TICKET_DONE;
@@ -118,11 +134,9 @@ extern "C" int emu_ldu_solve(int n, int nif, const int* own, const int* nbr, con
     memcpy(M.diag, diag, 8 * (size_t)n); memcpy(M.src, b, 8 * (size_t)n); memcpy(M.pp, x, 8 * (size_t)n);
     if (nif > 0) { memcpy(M.upper, upper, 8 * (size_t)nif); memcpy(M.lower, lower, 8 * (size_t)nif); }
     M.sc->status = RSGM_RUNNING;
-    static double red[256];
-    static double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
-    rsg::Thr t; t.tid = 0; t.bs = 1; t.red = red; t.lds = lds; t.phase = 0;
+    rsg::Thr t = emu_thr();
     int nl = 0;
-    rsg::SolveRes r = rsg::ldu_solve_member(t, &M, solver, maxIter, tol, &nl);
+    rsg::SolveRes r = rsg::ldu_solve_member<RSG_RES_MAXN>(t, &M, solver, maxIter, tol, &nl);
     memcpy(x, M.pp, 8 * (size_t)n);
     stats[0] = r.iterations; stats[1] = r.finalRes; stats[2] = r.initialRes; stats[3] = r.converged; stats[4] = nl;
     return 0;

@@ -25,12 +25,19 @@ def emu_solve(emu, path, case, s):
     return r, arr
 
 
+@pytest.mark.parametrize("resident", [1, 0])
 @pytest.mark.parametrize("case,iters", [("c0", 700), ("channel_bounded", 150), ("couette", 150),
                                         ("bfs", 80), ("plate", 80), ("hill", 120)])
-def test_emulated_device_trajectory_bit_exact(oracle, emu, meshes, case, iters):
+def test_emulated_device_trajectory_bit_exact(oracle, emu, meshes, case, iters, resident):
+    """Both linear-solver paths of the device code: the register/LDS-resident
+    solvers and the global-memory ones."""
     s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=1,
                      nut_floor_iters=min(500, iters // 2), record_history=1)
-    r, a = emu_solve(emu, meshes[case], case, s)
+    emu.emu_set_resident(resident)
+    try:
+        r, a = emu_solve(emu, meshes[case], case, s)
+    finally:
+        emu.emu_set_resident(1)
     o, ho, io = oracle_solve(oracle, meshes[case], case, s, hist_cap=iters)
     assert r.final_iter == io[0] and r.converged == io[1] and r.diverged == io[2]
     he = a["history"][:6 * r.history_len].reshape(-1, 6)
@@ -41,11 +48,16 @@ def test_emulated_device_trajectory_bit_exact(oracle, emu, meshes, case, iters):
         assert np.array_equal(a[k_emu], o[k_or]), k_emu
 
 
-def test_emulated_amg_tracks_oracle(oracle, emu, meshes):
+@pytest.mark.parametrize("resident", [1, 0])
+def test_emulated_amg_tracks_oracle(oracle, emu, meshes, resident):
     # 700 iterations of C0 with the device AMG vs the oracle's reference AMG:
     # same discretisation, different p' preconditioner
     s = settings_for("c0", max_iterations=700, convergence_tol=1e-12, pressure_solver=0, amg_refresh=1)
-    r, a = emu_solve(emu, meshes["c0"], "c0", s)
+    emu.emu_set_resident(resident)
+    try:
+        r, a = emu_solve(emu, meshes["c0"], "c0", s)
+    finally:
+        emu.emu_set_resident(1)
     o, _, _ = oracle_solve(oracle, meshes["c0"], "c0", s)
     for k_emu, k_or in (("U", "U"), ("k", "k"), ("omega", "om"), ("nuT", "nuT")):
         rel = np.linalg.norm(a[k_emu] - o[k_or]) / np.linalg.norm(o[k_or])
