@@ -219,14 +219,15 @@ int set_err(rsg_context* c, int code, const std::string& msg) {
             return set_err(ctx, RSG_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
     } while (0)
 
+// Staging slots for the device-side descriptor structs that follow their
+// arrays in one allocation.
+constexpr size_t kMeshStructSlot = (sizeof(RsgMeshDev) + 255) & ~size_t(255);
+static_assert(sizeof(RsgBcDev) <= 256, "problem staging reserves 256 bytes for RsgBcDev");
+
+// 256 threads: two member workgroups per CU (256 VGPRs, ~75 KB LDS each);
+// 512 threads: one per CU, for the lowest single-member latency.
 ChunkFn chunk_kernel(int bs) {
-    switch (bs) {
-        case 64: return k_simple_chunk<64>;
-        case 128: return k_simple_chunk<128>;
-        case 512: return k_simple_chunk<512>;
-        case 1024: return k_simple_chunk<1024>;
-        default: return k_simple_chunk<256>;
-    }
+    return bs == 512 ? k_simple_chunk<512> : k_simple_chunk<256>;
 }
 
 // host pointers of the result region of a member workspace
@@ -477,7 +478,7 @@ int rsg_mesh_upload(rsg_context* ctx, const rsg_mesh_desc* mesh, int32_t* mesh_i
     std::string err;
     if (!rsg::prep_mesh(mesh, ME->prep, err)) return set_err(ctx, RSG_ERR_ARG, err);
     rsg::MeshOffsets mo = rsg::mesh_offsets(ME->prep);
-    const size_t bytes = mo.total + 256;
+    const size_t bytes = mo.total + kMeshStructSlot;   // arrays, then the RsgMeshDev struct
     std::vector<char> stage(bytes, 0);
     rsg::mesh_stage(ME->prep, mo, stage.data());
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -690,8 +691,8 @@ int rsg_release(rsg_context* ctx, int64_t ticket) {
 
 int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_size) {
     if (!ctx || iters_per_launch <= 0) return RSG_ERR_ARG;
-    if (block_size != 64 && block_size != 128 && block_size != 256 && block_size != 512 && block_size != 1024)
-        return set_err(ctx, RSG_ERR_ARG, "block size must be 64, 128, 256, 512 or 1024");
+    if (block_size != 256 && block_size != 512)
+        return set_err(ctx, RSG_ERR_ARG, "block size must be 256 or 512");
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->itersPerLaunch = iters_per_launch;
     ctx->blockSize = block_size;
@@ -722,7 +723,7 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     std::string err;
     if (!rsg::prep_topology(n, nif, own, nbr, pm, err)) return set_err(ctx, RSG_ERR_ARG, err);
     rsg::MeshOffsets mo = rsg::mesh_offsets(pm);
-    const size_t mbytes = mo.total + 256;
+    const size_t mbytes = mo.total + kMeshStructSlot;
     const size_t wbytes = rsg::member_bytes(pm.nC, pm.nIF, pm.nB, pm.nnz0, 0);
     const size_t total = mbytes + 256 + sizeof(RsgMemberDev) + 256 + wbytes + 256;
     std::lock_guard<std::mutex> lk(ctx->mu);

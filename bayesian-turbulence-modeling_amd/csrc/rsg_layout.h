@@ -14,7 +14,7 @@
 #define RSG_MAX_LEVELS 20     // AMGSolver::maxLevels_ (LinearSolver.hpp:508)
 #define RSG_MIN_COARSE 50     // AMGSolver::minCoarseSize_ (LinearSolver.hpp:509)
 #define RSG_COARSE_DENSE 64   // coarsest levels up to this size are inverted densely
-#define RSG_ELL_K 4           // internal faces per cell held in registers (resident solves)
+#define RSG_ELL_K 4           // internal faces per cell in the cell-major (ELL) face slots
 // Resident solves: at most RSG_RES_MAXN cells per member (LDS exchange
 // buffers) and RSG_RES_MAXC coarse AMG rows (LDS level vectors).
 #ifdef RSG_EMULATE
@@ -70,10 +70,18 @@ struct RsgMeshDev {
     const int* src0;         // [nnz0]: <0 -> diag[-1-v]; <nIF -> upper[v]; else lower[v-nIF]
     // cell-major ELL image of the off-diagonal pattern: the k-th internal
     // face of cell c (ascending face order) at [k * nC + c]
-    int ellK;                // max internal faces of a cell (0 if > RSG_ELL_K)
-    const int* ellCnt;       // [nC]
+    // Cell-major face lists, in the order of the cell-face list: the first
+    // RSG_ELL_K internal faces of cell c in ELL slots [k * nC + c], further
+    // internal faces in an overflow list, then the boundary faces.
+    int ellK;                // 1 when no cell overflows the ELL slots (resident solvers apply)
+    const int* ellCnt;       // [nC] internal faces in the ELL slots
     const int* ellNb;        // [RSG_ELL_K * nC] neighbouring cell (padding: c)
     const int* ellFace;      // [RSG_ELL_K * nC] (f << 1) | isNeighbour (padding: -1)
+    const int* ovPtr;        // [nC+1] overflow internal faces
+    const int* ovFace;       // [ovPtr[nC]] (f << 1) | isNeighbour
+    const int* ovNb;         // [ovPtr[nC]] neighbouring cell
+    const int* bPtr;         // [nC+1] boundary faces of the cell
+    const int* bFace;        // [bPtr[nC]] boundary face index f >= nIF, ascending
 };
 
 // per-problem boundary conditions, resolved per boundary face

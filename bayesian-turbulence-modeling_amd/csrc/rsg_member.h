@@ -158,27 +158,52 @@ struct Ctx {
 
 RSG_D double blend(double p1, double p2, double F1) { return F1 * p1 + (1.0 - F1) * p2; }
 
+// ------------------------------------------------------- cell face lists
+// Calls fn(code, other) for every face of cell c in the order of its
+// cell-face list -- internal faces by ascending face index, then boundary
+// faces ascending, the order in which the reference's face-loop scatter adds
+// into the cell -- with code = (f << 1) | isNeighbour and other the cell
+// across an internal face (-1 on a boundary face).  The ELL slots are read in
+// one round of independent loads; overflow and boundary faces follow.
+template <class Fn> RSG_D void cell_faces(const RsgMeshDev* m, int c, Fn&& fn) {
+    const int nC = m->nC;
+    const int cnt = m->ellCnt[c];
+    int code[RSG_ELL_K], nb[RSG_ELL_K];
+#pragma unroll
+    for (int k = 0; k < RSG_ELL_K; ++k) { code[k] = m->ellFace[k * nC + c]; nb[k] = m->ellNb[k * nC + c]; }
+    const int ov0 = m->ovPtr[c], ov1 = m->ovPtr[c + 1], b0 = m->bPtr[c], b1 = m->bPtr[c + 1];
+#pragma unroll
+    for (int k = 0; k < RSG_ELL_K; ++k)
+        if (k < cnt) fn(code[k], nb[k]);
+    for (int e = ov0; e < ov1; ++e) fn(m->ovFace[e], m->ovNb[e]);
+    for (int e = b0; e < b1; ++e) fn(m->bFace[e] << 1, -1);
+}
+
 // ------------------------------------------------------- Green-Gauss (gather)
 // greenGaussGrad Field.hpp:122-152 for NF fields at once, cell c.
 template <int NF>
 RSG_D void gg_cell(const RsgMeshDev* m, int c, const double* const (&phi)[NF],
                    const double* const (&phib)[NF], double (&g)[NF][3]) {
     const int nIF = m->nIF;
+    double pc[NF];
 #pragma unroll
-    for (int q = 0; q < NF; ++q) g[q][0] = g[q][1] = g[q][2] = 0.0;
-    for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
-        const int code = m->cfIdx[e];
+    for (int q = 0; q < NF; ++q) { g[q][0] = g[q][1] = g[q][2] = 0.0; pc[q] = phi[q][c]; }
+    cell_faces(m, c, [&](int code, int other) {
         const int f = code >> 1;
         const double sx = m->sfx[f], sy = m->sfy[f], sz = m->sfz[f];
         if (f < nIF) {
-            const int o = m->own[f], n = m->nbr[f];
+            const bool nbSide = code & 1;
             const double w = m->w[f], wm = 1.0 - w;
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
-                const double pf = phi[q][o] * w + phi[q][n] * wm;
+                const double po = nbSide ? phi[q][other] : pc[q];
+                const double pn = nbSide ? pc[q] : phi[q][other];
+                const double pf = po * w + pn * wm;
                 const double ax = sx * pf, ay = sy * pf, az = sz * pf;
-                if (code & 1) { g[q][0] = g[q][0] - ax; g[q][1] = g[q][1] - ay; g[q][2] = g[q][2] - az; }
-                else          { g[q][0] = g[q][0] + ax; g[q][1] = g[q][1] + ay; g[q][2] = g[q][2] + az; }
+                // x - a == x + (-a) exactly: select the operand, not the accumulator
+                g[q][0] = g[q][0] + (nbSide ? -ax : ax);
+                g[q][1] = g[q][1] + (nbSide ? -ay : ay);
+                g[q][2] = g[q][2] + (nbSide ? -az : az);
             }
         } else {
 #pragma unroll
@@ -187,7 +212,7 @@ RSG_D void gg_cell(const RsgMeshDev* m, int c, const double* const (&phi)[NF],
                 g[q][0] = g[q][0] + sx * pb; g[q][1] = g[q][1] + sy * pb; g[q][2] = g[q][2] + sz * pb;
             }
         }
-    }
+    });
     const double V = m->vol[c];
     if (V > 1e-30) {
 #pragma unroll
@@ -812,9 +837,8 @@ RSG_D double* ubcomp(RsgMemberDev* M, int comp) { return comp == 0 ? M->Ubx : co
 
 // Convection-diffusion face coefficients shared by the momentum, k and omega
 // assemblies: upper = -(Df + cNeg), lower = -(Df + cPos).
-RSG_D void conv_diff_face(const RsgMeshDev* m, const RsgMemberDev* M, int f, double nuEff_o,
+RSG_D void conv_diff_face(const RsgMeshDev* m, const RsgMemberDev* M, int f, int o, int n, double nuEff_o,
                           double nuEff_n, double& up, double& lo) {
-    const int o = m->own[f], n = m->nbr[f];
     const double Sf = m->area[f], delta = m->dlt[f], w = m->w[f], wm = 1.0 - w;
     const double nuEff_f = w * nuEff_o + (1.0 - w) * nuEff_n;
     const double Df = nuEff_f * Sf / delta;
@@ -830,12 +854,13 @@ RSG_D void conv_diff_face(const RsgMeshDev* m, const RsgMemberDev* M, int f, dou
 // boundary-face convection-diffusion of a scalar with face value phib
 // (outflow / inflow branches, SIMPLESolver.cpp:132-140)
 RSG_D void conv_diff_bface(const RsgMeshDev* m, const RsgMemberDev* M, int f, int b, double nuEff,
-                           double phib, double& dg, double& src) {
+                           double phib, double& ddg, double& dsrc) {
     const double Sf = m->area[f], delta = m->dlt[f];
     const double Db = nuEff * Sf / delta;
     const double mFlux = (M->Ubx[b] * m->nx[f] + M->Uby[b] * m->ny[f] + M->Ubz[b] * m->nz[f]) * Sf;
-    if (mFlux >= 0) { dg += mFlux + Db; src += Db * phib; }
-    else { dg += Db; src += (Db - mFlux) * phib; }
+    const bool out = mFlux >= 0;
+    ddg = out ? mFlux + Db : Db;
+    dsrc = out ? Db * phib : (Db - mFlux) * phib;
 }
 
 // assembleMomentum SIMPLESolver.cpp:56-203 for one component.  Requires
@@ -848,43 +873,50 @@ RSG_D void assemble_momentum(Ctx& C, int comp) {
     const double* gc0 = M->g[0 * 3 + comp];
     const double* gc1 = M->g[1 * 3 + comp];
     const double* gc2 = M->g[2 * 3 + comp];
-    FOR_IFACES(f) {
-        const int o = m->own[f], n = m->nbr[f];
-        double up, lo;
-        conv_diff_face(m, M, f, nu + M->nuT[o], nu + M->nuT[n], up, lo);
-        M->upper[f] = up; M->lower[f] = lo;
-        // transpose-stress face flux, StressOperators.hpp:50-59
-        const double w = m->w[f], wm = 1.0 - w;
-        const double cf = w * M->nuT[o] + (1.0 - w) * M->nuT[n];
-        const double gx = gc0[o] * w + gc0[n] * wm;
-        const double gy = gc1[o] * w + gc1[n] * wm;
-        const double gz = gc2[o] * w + gc2[n] * wm;
-        M->fscr[f] = cf * (gx * m->nx[f] + gy * m->ny[f] + gz * m->nz[f]) * m->area[f];
-    }
-    RSG_SYNC();
     const double* phiOld = ucomp(M, comp);
     const double* Ub = ubcomp(M, comp);
     const double fb = C.s->bf[comp];
     const double a = C.s->aU;
     const int wallFn = C.s->wallFn;
     const double* gpc = M->gp[comp];
+    // One pass per cell: each internal face's coefficients and transpose-stress
+    // flux are evaluated from the cell on either side with the same operands
+    // (so both sides see identical values); the owner side stores the LDU
+    // off-diagonals.
     FOR_CELLS(c) {
+        // Accumulators start at +0 and never become -0, so a +0 increment is
+        // exact; every branch produces increments and one add per accumulator
+        // follows (no branch-dependent accumulator address).
         double dg = 0.0, src = 0.0, ts = 0.0;
-        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
-            const int code = m->cfIdx[e], f = code >> 1;
+        cell_faces(m, c, [&](int code, int other) {
+            const int f = code >> 1;
+            double ddg, dsrc = 0.0, dts;
             if (f < nIF) {
-                if (code & 1) { dg += -M->upper[f]; ts -= M->fscr[f]; }
-                else          { dg += -M->lower[f]; ts += M->fscr[f]; }
+                const bool nbSide = code & 1;
+                const int o = nbSide ? other : c, n = nbSide ? c : other;
+                double up, lo;
+                conv_diff_face(m, M, f, o, n, nu + M->nuT[o], nu + M->nuT[n], up, lo);
+                // transpose-stress face flux, StressOperators.hpp:50-59
+                const double w = m->w[f], wm = 1.0 - w;
+                const double cf = w * M->nuT[o] + (1.0 - w) * M->nuT[n];
+                const double gx = gc0[o] * w + gc0[n] * wm;
+                const double gy = gc1[o] * w + gc1[n] * wm;
+                const double gz = gc2[o] * w + gc2[n] * wm;
+                const double fs = cf * (gx * m->nx[f] + gy * m->ny[f] + gz * m->nz[f]) * m->area[f];
+                if (!nbSide) { M->upper[f] = up; M->lower[f] = lo; }
+                ddg = nbSide ? -up : -lo;
+                dts = nbSide ? -fs : fs;
             } else {
                 const int b = f - nIF;
                 const int isWall = m->bFlags[b] & RSG_BF_WALL;
                 const double nuEff = (isWall && !wallFn) ? nu : nu + M->nuT[c];
-                conv_diff_bface(m, M, f, b, nuEff, Ub[b], dg, src);
+                conv_diff_bface(m, M, f, b, nuEff, Ub[b], ddg, dsrc);
                 // transpose stress boundary term with the wall coefficient zeroed
                 const double cfb = isWall ? 0.0 : M->nuT[c];
-                ts += cfb * (gc0[c] * m->nx[f] + gc1[c] * m->ny[f] + gc2[c] * m->nz[f]) * m->area[f];
+                dts = cfb * (gc0[c] * m->nx[f] + gc1[c] * m->ny[f] + gc2[c] * m->nz[f]) * m->area[f];
             }
-        }
+            dg += ddg; src += dsrc; ts += dts;
+        });
         const double vol = m->vol[c];
         src -= gpc[c] * vol;
         if (fb != 0.0) src += fb * vol;
@@ -906,47 +938,47 @@ RSG_D void assemble_pcorr(Ctx& C) {
     const int nC = C.nC, nIF = C.nIF;
     const bool rc = !m->hasOutlet || C.s->rcAll;
     const double* aP = M->aPx;
-    FOR_IFACES(f) {
-        const int o = m->own[f], n = m->nbr[f];
-        const double Sf = m->area[f], delta = m->dlt[f], w = m->w[f], wm = 1.0 - w;
-        const double Vo = m->vol[o], Vn = m->vol[n];
-        const double dP_o = (fabs(aP[o]) > 1e-30) ? Vo / aP[o] : 0.0;
-        const double dP_n = (fabs(aP[n]) > 1e-30) ? Vn / aP[n] : 0.0;
-        const double dP_f = w * dP_o + (1.0 - w) * dP_n;
-        const double coeff = dP_f * Sf / delta;
-        M->upper[f] = -coeff; M->lower[f] = -coeff;
-        const double ufx = M->Ux[o] * w + M->Ux[n] * wm;
-        const double ufy = M->Uy[o] * w + M->Uy[n] * wm;
-        const double ufz = M->Uz[o] * w + M->Uz[n] * wm;
-        double massFlux = (ufx * m->nx[f] + ufy * m->ny[f] + ufz * m->nz[f]) * Sf;
-        if (rc) {
-            const double gbx = M->gp[0][o] * w + M->gp[0][n] * wm;
-            const double gby = M->gp[1][o] * w + M->gp[1][n] * wm;
-            const double gbz = M->gp[2][o] * w + M->gp[2][n] * wm;
-            const double ge = gbx * m->ex[f] + gby * m->ey[f] + gbz * m->ez[f];
-            massFlux += -dP_f * Sf * ((M->p[n] - M->p[o]) / delta - ge);
-        }
-        M->fscr[f] = massFlux;
-    }
-    RSG_SYNC();
     FOR_CELLS(c) {
         double dg = 0.0, src = 0.0;
-        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
-            const int code = m->cfIdx[e], f = code >> 1;
+        cell_faces(m, c, [&](int code, int other) {
+            const int f = code >> 1;
+            double ddg, dsrc;
             if (f < nIF) {
-                dg += -M->upper[f];
-                if (code & 1) src += M->fscr[f]; else src -= M->fscr[f];
+                const bool nbSide = code & 1;
+                const int o = nbSide ? other : c, n = nbSide ? c : other;
+                const double Sf = m->area[f], delta = m->dlt[f], w = m->w[f], wm = 1.0 - w;
+                const double Vo = m->vol[o], Vn = m->vol[n];
+                const double dP_o = (fabs(aP[o]) > 1e-30) ? Vo / aP[o] : 0.0;
+                const double dP_n = (fabs(aP[n]) > 1e-30) ? Vn / aP[n] : 0.0;
+                const double dP_f = w * dP_o + (1.0 - w) * dP_n;
+                const double coeff = dP_f * Sf / delta;
+                const double ufx = M->Ux[o] * w + M->Ux[n] * wm;
+                const double ufy = M->Uy[o] * w + M->Uy[n] * wm;
+                const double ufz = M->Uz[o] * w + M->Uz[n] * wm;
+                double massFlux = (ufx * m->nx[f] + ufy * m->ny[f] + ufz * m->nz[f]) * Sf;
+                if (rc) {
+                    const double gbx = M->gp[0][o] * w + M->gp[0][n] * wm;
+                    const double gby = M->gp[1][o] * w + M->gp[1][n] * wm;
+                    const double gbz = M->gp[2][o] * w + M->gp[2][n] * wm;
+                    const double ge = gbx * m->ex[f] + gby * m->ey[f] + gbz * m->ez[f];
+                    massFlux += -dP_f * Sf * ((M->p[n] - M->p[o]) / delta - ge);
+                }
+                if (!nbSide) { M->upper[f] = -coeff; M->lower[f] = -coeff; }
+                ddg = -(-coeff);
+                dsrc = nbSide ? massFlux : -massFlux;
             } else {
                 const int b = f - nIF;
                 const double Sf = m->area[f];
                 const double mf = (M->Ubx[b] * m->nx[f] + M->Uby[b] * m->ny[f] + M->Ubz[b] * m->nz[f]) * Sf;
-                src -= mf;
+                dsrc = -mf;
+                ddg = 0.0;
                 if (m->bFlags[b] & RSG_BF_OUTLET) {
                     const double dP_o = (fabs(aP[c]) > 1e-30) ? m->vol[c] / aP[c] : 0.0;
-                    dg += dP_o * Sf / m->dlt[f];
+                    ddg = dP_o * Sf / m->dlt[f];
                 }
             }
-        }
+            dg += ddg; src += dsrc;   // +0 increments are exact (accumulators are never -0)
+        });
         if (c == 0 && !m->hasOutlet) dg += fabs(dg) > 1e-30 ? dg : 1.0;
         M->diag[c] = dg;
         M->src[c] = src;
@@ -961,31 +993,31 @@ RSG_D void assemble_turb(Ctx& C, bool isOmega) {
     const int nC = C.nC, nIF = C.nIF;
     const double nu = C.nu;
     const double s1 = isOmega ? C.sw1 : C.sk1, s2 = isOmega ? C.sw2 : C.sk2;
-    FOR_IFACES(f) {
-        const int o = m->own[f], n = m->nbr[f];
-        const double w = m->w[f];
-        const double F1f = w * M->F1[o] + (1.0 - w) * M->F1[n];
-        const double sg = blend(s1, s2, F1f);
-        double up, lo;
-        conv_diff_face(m, M, f, nu + sg * M->nuT[o], nu + sg * M->nuT[n], up, lo);
-        M->upper[f] = up; M->lower[f] = lo;
-    }
-    RSG_SYNC();
     const double* phi = isOmega ? M->om : M->k;
     const double* phib = isOmega ? M->omb : M->kb;
     const double a = isOmega ? C.s->aOm : C.s->aK;
     FOR_CELLS(c) {
         double dg = 0.0, src = 0.0;
-        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
-            const int code = m->cfIdx[e], f = code >> 1;
+        cell_faces(m, c, [&](int code, int other) {
+            const int f = code >> 1;
+            double ddg, dsrc = 0.0;
             if (f < nIF) {
-                dg += (code & 1) ? -M->upper[f] : -M->lower[f];
+                const bool nbSide = code & 1;
+                const int o = nbSide ? other : c, n = nbSide ? c : other;
+                const double w = m->w[f];
+                const double F1f = w * M->F1[o] + (1.0 - w) * M->F1[n];
+                const double sg = blend(s1, s2, F1f);
+                double up, lo;
+                conv_diff_face(m, M, f, o, n, nu + sg * M->nuT[o], nu + sg * M->nuT[n], up, lo);
+                if (!nbSide) { M->upper[f] = up; M->lower[f] = lo; }
+                ddg = nbSide ? -up : -lo;
             } else {
                 const int b = f - nIF;
                 const double sg = blend(s1, s2, M->F1[c]);
-                conv_diff_bface(m, M, f, b, nu + sg * M->nuT[c], phib[b], dg, src);
+                conv_diff_bface(m, M, f, b, nu + sg * M->nuT[c], phib[b], ddg, dsrc);
             }
-        }
+            dg += ddg; src += dsrc;   // +0 increments are exact (accumulators are never -0)
+        });
         const double vol = m->vol[c];
         if (!isOmega) {
             src += M->Pk[c] * vol;

@@ -31,7 +31,8 @@ struct Carver {
 // Host image of one mesh, in the order RsgMeshDev expects.
 struct PrepMesh {
     int nC = 0, nIF = 0, nF = 0, nB = 0, hasOutlet = 0, nnz0 = 0, nP = 0;
-    std::vector<int> own, nbr, cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace;
+    std::vector<int> own, nbr, cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace, ovPtr, ovFace, ovNb,
+        bPtr, bFace;
     int ellK = 0;
     std::vector<double> w, dlt, area, nx, ny, nz, sfx, sfy, sfz, ex, ey, ez, vol, wd, pinY;
     std::vector<int> patchType, patchPtr, patchFaces;
@@ -122,23 +123,30 @@ inline bool prep_mesh(const rsg_mesh_desc* d, PrepMesh& P, std::string& err) {
     P.ellCnt.assign(nC, 0);
     P.ellNb.assign((size_t)RSG_ELL_K * nC, 0);
     P.ellFace.assign((size_t)RSG_ELL_K * nC, -1);
-    int kmax = 0;
+    P.ovPtr.assign(nC + 1, 0); P.ovFace.clear(); P.ovNb.clear();
+    P.bPtr.assign(nC + 1, 0); P.bFace.clear();
     for (int c = 0; c < nC; ++c) {
         int k = 0;
         for (int e = P.cfPtr[c]; e < P.cfPtr[c + 1]; ++e) {
             const int code = P.cfIdx[e], f = code >> 1;
-            if (f >= nIF) continue;
+            if (f >= nIF) { P.bFace.push_back(f); continue; }
+            const int other = (code & 1) ? P.own[f] : P.nbr[f];
             if (k < RSG_ELL_K) {
-                P.ellNb[(size_t)k * nC + c] = (code & 1) ? P.own[f] : P.nbr[f];
+                P.ellNb[(size_t)k * nC + c] = other;
                 P.ellFace[(size_t)k * nC + c] = code;
+            } else {
+                P.ovFace.push_back(code); P.ovNb.push_back(other);
             }
             ++k;
         }
         P.ellCnt[c] = std::min(k, RSG_ELL_K);
         for (int j = k; j < RSG_ELL_K; ++j) P.ellNb[(size_t)j * nC + c] = c;
-        kmax = std::max(kmax, k);
+        P.ovPtr[c + 1] = (int)P.ovFace.size();
+        P.bPtr[c + 1] = (int)P.bFace.size();
     }
-    P.ellK = kmax <= RSG_ELL_K ? kmax : 0;
+    P.ellK = P.ovFace.empty() ? 1 : 0;
+    if (P.ovFace.empty()) { P.ovFace.push_back(0); P.ovNb.push_back(0); }   // keep every array non-empty
+    if (P.bFace.empty()) P.bFace.push_back(0);
     // level-0 CSR pattern, built as lduToCSR does (LinearSolver.hpp:344-404):
     // diagonal first, then faces in order, then a stable insertion sort per row
     P.nnz0 = nC + 2 * nIF;
@@ -184,7 +192,7 @@ inline bool prep_topology(int n, int nif, const int* own, const int* nbr, PrepMe
 // Byte offsets of every RsgMeshDev array inside one contiguous block.
 struct MeshOffsets {
     size_t own, nbr, w, dlt, area, nx, ny, nz, sfx, sfy, sfz, ex, ey, ez, vol, wd, pinY,
-        cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace, total;
+        cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace, ovPtr, ovFace, ovNb, bPtr, bFace, total;
 };
 
 inline MeshOffsets mesh_offsets(const PrepMesh& P) {
@@ -200,6 +208,8 @@ inline MeshOffsets mesh_offsets(const PrepMesh& P) {
     o.cfPtr = at(4 * P.cfPtr.size()); o.cfIdx = at(4 * P.cfIdx.size()); o.bFlags = at(4 * P.bFlags.size());
     o.rp0 = at(4 * P.rp0.size()); o.ci0 = at(4 * P.ci0.size()); o.src0 = at(4 * P.src0.size());
     o.ellCnt = at(4 * P.ellCnt.size()); o.ellNb = at(4 * P.ellNb.size()); o.ellFace = at(4 * P.ellFace.size());
+    o.ovPtr = at(4 * P.ovPtr.size()); o.ovFace = at(4 * P.ovFace.size()); o.ovNb = at(4 * P.ovNb.size());
+    o.bPtr = at(4 * P.bPtr.size()); o.bFace = at(4 * P.bFace.size());
     o.total = (cv.off + 255) & ~size_t(255);
     return o;
 }
@@ -224,6 +234,9 @@ inline void mesh_stage(const PrepMesh& P, const MeshOffsets& o, char* base) {
     put(o.src0, P.src0.data(), 4 * P.src0.size());
     put(o.ellCnt, P.ellCnt.data(), 4 * P.ellCnt.size()); put(o.ellNb, P.ellNb.data(), 4 * P.ellNb.size());
     put(o.ellFace, P.ellFace.data(), 4 * P.ellFace.size());
+    put(o.ovPtr, P.ovPtr.data(), 4 * P.ovPtr.size()); put(o.ovFace, P.ovFace.data(), 4 * P.ovFace.size());
+    put(o.ovNb, P.ovNb.data(), 4 * P.ovNb.size());
+    put(o.bPtr, P.bPtr.data(), 4 * P.bPtr.size()); put(o.bFace, P.bFace.data(), 4 * P.bFace.size());
 }
 
 // RsgMeshDev whose pointers address `base` (host staging or device copy).
@@ -244,6 +257,9 @@ inline RsgMeshDev mesh_view(const PrepMesh& P, const MeshOffsets& o, char* base)
     m.ellK = P.ellK;
     m.ellCnt = (const int*)(base + o.ellCnt); m.ellNb = (const int*)(base + o.ellNb);
     m.ellFace = (const int*)(base + o.ellFace);
+    m.ovPtr = (const int*)(base + o.ovPtr); m.ovFace = (const int*)(base + o.ovFace);
+    m.ovNb = (const int*)(base + o.ovNb);
+    m.bPtr = (const int*)(base + o.bPtr); m.bFace = (const int*)(base + o.bFace);
     return m;
 }
 

@@ -168,7 +168,8 @@ int rsg_poll(rsg_context* ctx, const int64_t* tickets, int32_t n, int32_t* state
 int rsg_wait(rsg_context* ctx, const int64_t* tickets, int32_t n, double timeout_s);
 int rsg_fetch(rsg_context* ctx, int64_t ticket, rsg_result* out);
 int rsg_release(rsg_context* ctx, int64_t ticket);
-/* Engine tuning: outer iterations per persistent launch and workgroup size. */
+/* Engine tuning: outer iterations per persistent launch and workgroup size
+ * (256: two members per CU, the default; 512: one member per CU). */
 int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_size);
 /* Engine counters: out[0] launches, out[1] members admitted, out[2] members retired,
  * out[3] kernel milliseconds (HIP events), out[4] max concurrent members. */
