@@ -139,6 +139,7 @@ struct MeshEntry {
 struct ProblemEntry {
     int meshId = 0;
     double nu = 0.0;
+    bool bcZeroZ = false;           // every boundary face prescribes zero z velocity
     rsg_settings settings{};
     size_t off = 0, bytes = 0;      // arena block: BC arrays + RsgBcDev + RsgSettingsDev
     RsgBcDev* bcDev = nullptr;
@@ -188,7 +189,7 @@ struct rsg_context {
     int nextMesh = 1, nextProblem = 1;
     int64_t nextTicket = 1;
     int itersPerLaunch = 200;
-    int blockSize = 256;
+    int blockSize = 0;       // 0: chosen per launch from the number of active members
     int maxSlots = 4096;
     std::thread engine;
     bool stop = false;
@@ -259,6 +260,9 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     for (int i = 0; i < 11; ++i) M.coeffs[i] = T.coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = T.init[i];
     M.initFromFields = T.warm;
+    M.planarZ = rsg::planar_z(pm, P.bcZeroZ, &P.settings, T.warm ? 0.0 : T.init[2],
+                              T.warm ? T.warmCells.data() : nullptr,
+                              T.warm && pm.nB > 0 ? T.warmBnd.data() : nullptr) ? 1 : 0;
     RsgScalars s;
     member_init_scalars(s, T.warm ? T.warmTurb : 0);
     hipError_t e = hipMemcpyAsync(M.sc, &s, sizeof s, hipMemcpyHostToDevice, ctx->stream);
@@ -360,7 +364,11 @@ void engine_loop(rsg_context* ctx) {
             T.M.iterBudget = ctx->itersPerLaunch;
             ctx->hMembers[i] = T.M;
         }
-        const int bs = ctx->blockSize;
+        // Auto: while more members are active than there are CUs, two
+        // 256-thread member workgroups share each CU; below that (the tail of
+        // an ensemble, a lightly loaded server) each member gets a 512-thread
+        // workgroup of its own, which lowers its iteration latency.
+        const int bs = ctx->blockSize ? ctx->blockSize : (na > ctx->cuCount ? 256 : 512);
         std::vector<int64_t> batch = ctx->active;
         lk.unlock();
         hipError_t e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * na,
@@ -523,6 +531,7 @@ int rsg_problem_create(rsg_context* ctx, int32_t mesh_id, const rsg_bc_desc* bc,
         return set_err(ctx, RSG_ERR_ARG, "bc descriptor: missing array");
     std::unique_ptr<ProblemEntry> P(new ProblemEntry());
     P->meshId = mesh_id; P->nu = nu; P->settings = *settings;
+    P->bcZeroZ = nB == 0 || rsg::bc_zero_z(bc, nB);
     rsg::BcOffsets bo = rsg::bc_offsets(std::max(nB, 1));
     const size_t bytes = bo.total + 512 + sizeof(RsgBcDev) + sizeof(RsgSettingsDev);
     std::vector<char> stage(bytes, 0);
@@ -691,8 +700,8 @@ int rsg_release(rsg_context* ctx, int64_t ticket) {
 
 int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_size) {
     if (!ctx || iters_per_launch <= 0) return RSG_ERR_ARG;
-    if (block_size != 256 && block_size != 512)
-        return set_err(ctx, RSG_ERR_ARG, "block size must be 256 or 512");
+    if (block_size != 0 && block_size != 256 && block_size != 512)
+        return set_err(ctx, RSG_ERR_ARG, "block size must be 0 (automatic), 256 or 512");
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->itersPerLaunch = iters_per_launch;
     ctx->blockSize = block_size;

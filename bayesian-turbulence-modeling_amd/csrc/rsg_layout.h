@@ -142,6 +142,15 @@ struct RsgAmgDev {
     double* x; double* b; double* r;   // level vectors at lvRow[l]
     double* cinv;          // [RSG_COARSE_DENSE^2] dense inverse of the coarsest level
     int* scr;              // serial-setup scratch [3 * rowCap + 2 * nnzCap]
+    // ELL images of the coarse levels (l >= 1) for the resident V-cycle:
+    // row i's k-th entry (CSR order) at lvE[l] + k * n_l + i (column -1 pads),
+    // and aggregate I's q-th fine row (restriction order) at lvAE[l] + q * n_l + I.
+    // lvK[l] / lvAK[l] = 0: that level has no image (capacity) and uses CSR.
+    int eCap, aCap;
+    int* lvK; int* lvE; int* lvAK; int* lvAE;   // [RSG_MAX_LEVELS+1]
+    int* eCol;             // [eCap]
+    double* eVal;          // [eCap]
+    int* aCol;             // [aCap]
 };
 
 struct RsgMemberDev {
@@ -152,6 +161,7 @@ struct RsgMemberDev {
     double coeffs[11];     // SST coefficients, InferenceParameters.hpp:12-14 order
     double init[6];        // uniform initial state Ux Uy Uz p k omega
     int initFromFields;    // state already uploaded (warm start)
+    int planarZ;           // Uz stays exactly zero: its momentum solve is skipped
     int iterBudget;        // outer iterations allowed in this launch
     // fields (cell) and boundary values
     double *Ux, *Uy, *Uz, *p, *k, *om, *nuT, *F1, *F2, *Pk, *CDkw, *Smag;
@@ -162,8 +172,6 @@ struct RsgMemberDev {
     double *gp[3];         // grad p (and grad p')
     // LDU system in use
     double *diag, *upper, *lower, *src;
-    double *fscr;          // [nIF] face scratch
-    double *ellA;          // [RSG_ELL_K * nC] off-diagonal of the LDU system in ELL order
     double *kv[8];         // Krylov vectors
     RsgAmgDev amg;
     double* hist;          // [6 * histCap] (optional)

@@ -159,12 +159,16 @@ struct Ctx {
 RSG_D double blend(double p1, double p2, double F1) { return F1 * p1 + (1.0 - F1) * p2; }
 
 // ------------------------------------------------------- cell face lists
-// Calls fn(code, other) for every face of cell c in the order of its
+// Calls fn(code, other, slot) for every face of cell c in the order of its
 // cell-face list -- internal faces by ascending face index, then boundary
 // faces ascending, the order in which the reference's face-loop scatter adds
-// into the cell -- with code = (f << 1) | isNeighbour and other the cell
-// across an internal face (-1 on a boundary face).  The ELL slots are read in
-// one round of independent loads; overflow and boundary faces follow.
+// into the cell -- with code = (f << 1) | isNeighbour, other the cell across
+// an internal face (-1 on a boundary face) and slot the face's ELL slot
+// (a compile-time constant after unrolling; -1 for overflow and boundary
+// faces).  The ELL slots are read in one round of independent loads.  A
+// caller that writes global memory per face defers the writes of ELL faces
+// to after the call (indexed by slot), so the compiler may issue every ELL
+// face's loads before any store.
 template <class Fn> RSG_D void cell_faces(const RsgMeshDev* m, int c, Fn&& fn) {
     const int nC = m->nC;
     const int cnt = m->ellCnt[c];
@@ -174,9 +178,28 @@ template <class Fn> RSG_D void cell_faces(const RsgMeshDev* m, int c, Fn&& fn) {
     const int ov0 = m->ovPtr[c], ov1 = m->ovPtr[c + 1], b0 = m->bPtr[c], b1 = m->bPtr[c + 1];
 #pragma unroll
     for (int k = 0; k < RSG_ELL_K; ++k)
-        if (k < cnt) fn(code[k], nb[k]);
-    for (int e = ov0; e < ov1; ++e) fn(m->ovFace[e], m->ovNb[e]);
-    for (int e = b0; e < b1; ++e) fn(m->bFace[e] << 1, -1);
+        if (k < cnt) fn(code[k], nb[k], k);
+    for (int e = ov0; e < ov1; ++e) fn(m->ovFace[e], m->ovNb[e], -1);
+    for (int e = b0; e < b1; ++e) fn(m->bFace[e] << 1, -1, -1);
+}
+
+// Owner-side LDU off-diagonal writes of one cell's ELL faces, deferred.
+struct PendingLdu {
+    int f[RSG_ELL_K];
+    double up[RSG_ELL_K], lo[RSG_ELL_K];
+};
+RSG_D void pending_clear(PendingLdu& P) {
+#pragma unroll
+    for (int k = 0; k < RSG_ELL_K; ++k) P.f[k] = -1;
+}
+RSG_D void pending_put(PendingLdu& P, RsgMemberDev* M, int slot, int f, double up, double lo) {
+    if (slot >= 0) { P.f[slot] = f; P.up[slot] = up; P.lo[slot] = lo; }
+    else { M->upper[f] = up; M->lower[f] = lo; }
+}
+RSG_D void pending_flush(const PendingLdu& P, RsgMemberDev* M) {
+#pragma unroll
+    for (int k = 0; k < RSG_ELL_K; ++k)
+        if (P.f[k] >= 0) { M->upper[P.f[k]] = P.up[k]; M->lower[P.f[k]] = P.lo[k]; }
 }
 
 // ------------------------------------------------------- Green-Gauss (gather)
@@ -188,7 +211,7 @@ RSG_D void gg_cell(const RsgMeshDev* m, int c, const double* const (&phi)[NF],
     double pc[NF];
 #pragma unroll
     for (int q = 0; q < NF; ++q) { g[q][0] = g[q][1] = g[q][2] = 0.0; pc[q] = phi[q][c]; }
-    cell_faces(m, c, [&](int code, int other) {
+    cell_faces(m, c, [&](int code, int other, int slot) {
         const int f = code >> 1;
         const double sx = m->sfx[f], sy = m->sfy[f], sz = m->sfz[f];
         if (f < nIF) {
@@ -471,8 +494,48 @@ RSG_HD int amg_aggregate(int n, const int* rp, const int* ci, const double* v, c
 }
 
 // Builds levels 1.. from level 0 (whose values must be current).
+// ELL image of coarse level lc (rows crp/cci/cv) and of its aggregates
+// (aP/aI over the finer level), appended at the running offsets.
+#ifdef RSG_EMULATE
+static int rsg_emu_no_images = 0;   // emulation build only: force the CSR V-cycle (tests)
+#endif
+RSG_HD void amg_level_images(RsgAmgDev& A, int lc, int nc, const int* crp, const int* cci, const double* cv,
+                             const int* aP, const int* aI, int& eNext, int& aNext) {
+    int K = 0, AK = 0;
+#ifdef RSG_EMULATE
+    if (rsg_emu_no_images) { A.lvK[lc] = 0; A.lvAK[lc] = 0; return; }
+#endif
+    for (int I = 0; I < nc; ++I) {
+        K = (crp[I + 1] - crp[I] > K) ? crp[I + 1] - crp[I] : K;
+        AK = (aP[I + 1] - aP[I] > AK) ? aP[I + 1] - aP[I] : AK;
+    }
+    A.lvK[lc] = 0; A.lvAK[lc] = 0;
+    if (eNext + (long)K * nc <= A.eCap) {
+        A.lvK[lc] = K; A.lvE[lc] = eNext;
+        for (int k = 0; k < K; ++k)
+            for (int I = 0; I < nc; ++I) {
+                const int e = crp[I] + k;
+                const bool in = e < crp[I + 1];
+                A.eCol[eNext + k * nc + I] = in ? cci[e] : -1;
+                A.eVal[eNext + k * nc + I] = in ? cv[e] : 0.0;
+            }
+        eNext += K * nc;
+    }
+    if (aNext + (long)AK * nc <= A.aCap) {
+        A.lvAK[lc] = AK; A.lvAE[lc] = aNext;
+        for (int q = 0; q < AK; ++q)
+            for (int I = 0; I < nc; ++I) {
+                const int e = aP[I] + q;
+                A.aCol[aNext + q * nc + I] = e < aP[I + 1] ? aI[e] : -1;
+            }
+        aNext += AK * nc;
+    }
+}
+
 RSG_D void amg_setup_serial(const RsgMeshDev* m, RsgAmgDev& A) {
     int L = 1;
+    int eNext = 0, aNext = 0;
+    for (int l = 0; l <= RSG_MAX_LEVELS; ++l) { A.lvK[l] = 0; A.lvAK[l] = 0; }
     for (int l = 0; l < RSG_MAX_LEVELS; ++l) {
         const int n = A.lvN[l];
         if (n <= RSG_MIN_COARSE) break;
@@ -554,6 +617,7 @@ RSG_D void amg_setup_serial(const RsgMeshDev* m, RsgAmgDev& A) {
                 cv[e] = s;
                 if (cci[e] == I) cdg[I] = s;
             }
+        amg_level_images(A, l + 1, nc, crp, cci, cv, aP, aI, eNext, aNext);
         L = l + 2;
     }
     A.lvN[RSG_MAX_LEVELS] = L;   // level count
@@ -582,12 +646,15 @@ RSG_D void amg_refresh_values(Ctx& C, bool coarse) {
         const double* fv = A.v + A.lvNz[l];
         double* cv = A.v + A.lvNz[l + 1];
         double* cdg = A.dg + A.lvRow[l + 1];
+        const int K = A.lvK[l + 1];
+        double* ev = A.eVal + A.lvE[l + 1];
         for (int I = t.tid; I < nc; I += t.bs) {
             double d = 0.0;
             for (int e = crp[I]; e < crp[I + 1]; ++e) {
                 double s = 0.0;
                 for (int q = gP[e]; q < gP[e + 1]; ++q) s += fv[gI[q]];
                 cv[e] = s;
+                if (K > 0) ev[(e - crp[I]) * nc + I] = s;
                 if (cci[e] == I) d = s;
             }
             cdg[I] = d;
@@ -629,21 +696,27 @@ RSG_D void amg_coarse_inverse(Ctx& C) {
         }
     }
 #else
-    if (t.tid < 64) {
-        const int j = t.tid;
+    // Lane j owns column j; wave w owns rows w, w + nw, ...  Within a row the
+    // factor G[i][k] is read before any lane of the (same) wave writes it; the
+    // pivot row is read by every wave before the barrier and rewritten after.
+    {
+        const int j = t.tid & 63, w = t.tid >> 6, nw = t.bs >> 6;
         for (int k = 0; k < n; ++k) {
             const double piv = G[k * DN + k];
-            if (!(fabs(piv) > 1e-300)) { ok = 0; break; }
+            if (!(fabs(piv) > 1e-300)) { ok = 0; break; }   // same value in every thread
             const double rk = (j < n) ? (j == k ? 1.0 : G[k * DN + j]) / piv : 0.0;
-            for (int i = 0; i < n; ++i) {
-                if (i == k) continue;
+            __syncthreads();
+            for (int i = w; i < n; i += nw) {
+                if (i == k) {
+                    if (j < n) G[k * DN + j] = rk;
+                    continue;
+                }
                 const double fct = G[i * DN + k];
                 __builtin_amdgcn_wave_barrier();
                 if (j < n) G[i * DN + j] = (j == k ? 0.0 : G[i * DN + j]) - fct * rk;
                 __builtin_amdgcn_wave_barrier();
             }
-            if (j < n) G[k * DN + j] = rk;
-            __builtin_amdgcn_wave_barrier();
+            __syncthreads();
         }
     }
 #endif
@@ -888,7 +961,9 @@ RSG_D void assemble_momentum(Ctx& C, int comp) {
         // exact; every branch produces increments and one add per accumulator
         // follows (no branch-dependent accumulator address).
         double dg = 0.0, src = 0.0, ts = 0.0;
-        cell_faces(m, c, [&](int code, int other) {
+        PendingLdu pend;
+        pending_clear(pend);
+        cell_faces(m, c, [&](int code, int other, int slot) {
             const int f = code >> 1;
             double ddg, dsrc = 0.0, dts;
             if (f < nIF) {
@@ -903,7 +978,7 @@ RSG_D void assemble_momentum(Ctx& C, int comp) {
                 const double gy = gc1[o] * w + gc1[n] * wm;
                 const double gz = gc2[o] * w + gc2[n] * wm;
                 const double fs = cf * (gx * m->nx[f] + gy * m->ny[f] + gz * m->nz[f]) * m->area[f];
-                if (!nbSide) { M->upper[f] = up; M->lower[f] = lo; }
+                if (!nbSide) pending_put(pend, M, slot, f, up, lo);
                 ddg = nbSide ? -up : -lo;
                 dts = nbSide ? -fs : fs;
             } else {
@@ -917,6 +992,7 @@ RSG_D void assemble_momentum(Ctx& C, int comp) {
             }
             dg += ddg; src += dsrc; ts += dts;
         });
+        pending_flush(pend, M);
         const double vol = m->vol[c];
         src -= gpc[c] * vol;
         if (fb != 0.0) src += fb * vol;
@@ -925,7 +1001,6 @@ RSG_D void assemble_momentum(Ctx& C, int comp) {
         dg /= a;
         M->diag[c] = dg;
         M->src[c] = src;
-        M->aPc[c] = dg;
         if (comp == 0) M->aPx[c] = dg;
     }
     RSG_SYNC();
@@ -940,7 +1015,9 @@ RSG_D void assemble_pcorr(Ctx& C) {
     const double* aP = M->aPx;
     FOR_CELLS(c) {
         double dg = 0.0, src = 0.0;
-        cell_faces(m, c, [&](int code, int other) {
+        PendingLdu pend;
+        pending_clear(pend);
+        cell_faces(m, c, [&](int code, int other, int slot) {
             const int f = code >> 1;
             double ddg, dsrc;
             if (f < nIF) {
@@ -963,7 +1040,7 @@ RSG_D void assemble_pcorr(Ctx& C) {
                     const double ge = gbx * m->ex[f] + gby * m->ey[f] + gbz * m->ez[f];
                     massFlux += -dP_f * Sf * ((M->p[n] - M->p[o]) / delta - ge);
                 }
-                if (!nbSide) { M->upper[f] = -coeff; M->lower[f] = -coeff; }
+                if (!nbSide) pending_put(pend, M, slot, f, -coeff, -coeff);
                 ddg = -(-coeff);
                 dsrc = nbSide ? massFlux : -massFlux;
             } else {
@@ -979,6 +1056,7 @@ RSG_D void assemble_pcorr(Ctx& C) {
             }
             dg += ddg; src += dsrc;   // +0 increments are exact (accumulators are never -0)
         });
+        pending_flush(pend, M);
         if (c == 0 && !m->hasOutlet) dg += fabs(dg) > 1e-30 ? dg : 1.0;
         M->diag[c] = dg;
         M->src[c] = src;
@@ -998,7 +1076,9 @@ RSG_D void assemble_turb(Ctx& C, bool isOmega) {
     const double a = isOmega ? C.s->aOm : C.s->aK;
     FOR_CELLS(c) {
         double dg = 0.0, src = 0.0;
-        cell_faces(m, c, [&](int code, int other) {
+        PendingLdu pend;
+        pending_clear(pend);
+        cell_faces(m, c, [&](int code, int other, int slot) {
             const int f = code >> 1;
             double ddg, dsrc = 0.0;
             if (f < nIF) {
@@ -1009,7 +1089,7 @@ RSG_D void assemble_turb(Ctx& C, bool isOmega) {
                 const double sg = blend(s1, s2, F1f);
                 double up, lo;
                 conv_diff_face(m, M, f, o, n, nu + sg * M->nuT[o], nu + sg * M->nuT[n], up, lo);
-                if (!nbSide) { M->upper[f] = up; M->lower[f] = lo; }
+                if (!nbSide) pending_put(pend, M, slot, f, up, lo);
                 ddg = nbSide ? -up : -lo;
             } else {
                 const int b = f - nIF;
@@ -1018,6 +1098,7 @@ RSG_D void assemble_turb(Ctx& C, bool isOmega) {
             }
             dg += ddg; src += dsrc;   // +0 increments are exact (accumulators are never -0)
         });
+        pending_flush(pend, M);
         const double vol = m->vol[c];
         if (!isOmega) {
             src += M->Pk[c] * vol;
@@ -1148,6 +1229,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     // 2. momentum components, each assembled from the latest U
     SolveRes rU0 = {0, 0.0, 0.0, 0}, rU1 = rU0, rU2 = rU0;   // named, not indexed: no private-memory array
     for (int comp = 0; comp < 3; ++comp) {
+        if (comp == 2 && M->planarZ) break;   // Uz stays 0: the z system's residual is exactly 0
         if (comp > 0) { grad_u(C, comp - 1, comp); RSG_SYNC(); }
         assemble_momentum(C, comp);
         RSG_TMARK(1);

@@ -34,6 +34,7 @@ struct PrepMesh {
     std::vector<int> own, nbr, cfPtr, cfIdx, bFlags, rp0, ci0, src0, ellCnt, ellNb, ellFace, ovPtr, ovFace, ovNb,
         bPtr, bFace;
     int ellK = 0;
+    int planar = 0;   // every face normal and area vector has a zero z component
     std::vector<double> w, dlt, area, nx, ny, nz, sfx, sfy, sfz, ex, ey, ez, vol, wd, pinY;
     std::vector<int> patchType, patchPtr, patchFaces;
 };
@@ -145,6 +146,9 @@ inline bool prep_mesh(const rsg_mesh_desc* d, PrepMesh& P, std::string& err) {
         P.bPtr[c + 1] = (int)P.bFace.size();
     }
     P.ellK = P.ovFace.empty() ? 1 : 0;
+    P.planar = 1;
+    for (int f = 0; f < nF && P.planar; ++f)
+        if (d->face_normal[3 * f + 2] != 0.0) P.planar = 0;
     if (P.ovFace.empty()) { P.ovFace.push_back(0); P.ovNb.push_back(0); }   // keep every array non-empty
     if (P.bFace.empty()) P.bFace.push_back(0);
     // level-0 CSR pattern, built as lduToCSR does (LinearSolver.hpp:344-404):
@@ -301,8 +305,6 @@ inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int hist
     for (double** a : bArrays) *a = cv.take<double>(nB > 0 ? nB : 1);
     M.upper = cv.take<double>(nIF > 0 ? nIF : 1);
     M.lower = cv.take<double>(nIF > 0 ? nIF : 1);
-    M.fscr = cv.take<double>(nIF > 0 ? nIF : 1);
-    M.ellA = cv.take<double>((size_t)RSG_ELL_K * nC);
     RsgAmgDev& A = M.amg;
     A.rowCap = 3 * nC + 64;
     A.nnzCap = 3 * nnz0 + 64;
@@ -320,6 +322,11 @@ inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int hist
     A.x = cv.take<double>(A.rowCap); A.b = cv.take<double>(A.rowCap); A.r = cv.take<double>(A.rowCap);
     A.cinv = cv.take<double>(RSG_COARSE_DENSE * RSG_COARSE_DENSE);
     A.scr = cv.take<int>(3 * (size_t)A.rowCap + 2 * (size_t)A.nnzCap);
+    A.eCap = A.nnzCap; A.aCap = A.rowCap;
+    A.lvK = cv.take<int>(LV); A.lvE = cv.take<int>(LV); A.lvAK = cv.take<int>(LV); A.lvAE = cv.take<int>(LV);
+    A.eCol = cv.take<int>(A.eCap);
+    A.eVal = cv.take<double>(A.eCap);
+    A.aCol = cv.take<int>(A.aCap);
     M.histCap = histCap;
     M.hist = histCap > 0 ? cv.take<double>(6 * (size_t)histCap) : nullptr;
     M.sc = cv.take<RsgScalars>(1);
@@ -360,6 +367,29 @@ inline bool check_settings(const rsg_settings* s, std::string& err) {
     if (!ok(s->pressure_solver) || !ok(s->momentum_solver) || !ok(s->turbulence_solver)) {
         err = "settings: unknown linear solver code"; return false;
     }
+    return true;
+}
+
+
+// Whether a member's Uz stays exactly zero, so that skipping the z-momentum
+// assembly and solve leaves every result bit-identical: a planar mesh (zero z
+// normals, hence zero z area vectors and z gradients), no z body force, zero
+// z velocity on every boundary face and in the initial state.  Under these
+// the reference's z system has source and residual exactly 0, its solve
+// returns at once with initialRes 0, and Uz, Ubz stay 0.
+inline bool bc_zero_z(const rsg_bc_desc* bc, int nB) {
+    for (int b = 0; b < nB; ++b)
+        if (bc->vel_value[3 * b + 2] != 0.0) return false;
+    return true;
+}
+
+inline bool planar_z(const PrepMesh& P, bool bcZeroZ, const rsg_settings* s,
+                     double initUz, const double* warmU, const double* warmUb) {
+    if (!P.planar || !bcZeroZ || s->body_force[2] != 0.0 || initUz != 0.0) return false;
+    if (warmU)
+        for (int c = 0; c < P.nC; ++c) if (warmU[3 * c + 2] != 0.0) return false;
+    if (warmUb)
+        for (int b = 0; b < P.nB; ++b) if (warmUb[3 * b + 2] != 0.0) return false;
     return true;
 }
 

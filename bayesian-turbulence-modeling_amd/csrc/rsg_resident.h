@@ -230,6 +230,90 @@ RSG_D void res_jacobiL(const Thr& t, int n, const int* rp, const int* ci, const 
     }
 }
 
+// The same sweep on a level's ELL image: entries in CSR order, so the sums
+// round exactly like res_jacobiL's; column -1 pads.
+RSG_D void res_jacobiE(const Thr& t, int n, int K, const int* ec, const double* ev, const double* dg,
+                       const double* b, const double* xin, double* xout, bool xzero) {
+    const double wJ = 2.0 / 3.0;
+    for (int i = t.tid; i < n; i += t.bs) {
+        double s = b[i];
+        if (!xzero) {
+#pragma unroll 4
+            for (int k = 0; k < K; ++k) {
+                const int col = ec[k * n + i];
+                const double a = ev[k * n + i];
+                if (col >= 0) s -= a * xin[col];
+            }
+        }
+        const double d = dg[i];
+        const double x0 = xzero ? 0.0 : xin[i];
+        xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+    }
+}
+
+// Coarse-level sweep / residual / restriction through the ELL images when present.
+RSG_D void res_sweepL(const Thr& t, const RsgAmgDev& A, int l, const double* b, const double* xin, double* xout,
+                      bool xzero) {
+    const int n = A.lvN[l];
+    const double* dg = A.dg + A.lvRow[l];
+    if (A.lvK[l] > 0)
+        res_jacobiE(t, n, A.lvK[l], A.eCol + A.lvE[l], A.eVal + A.lvE[l], dg, b, xin, xout, xzero);
+    else
+        res_jacobiL(t, n, A.rp + A.lvRp[l], A.ci + A.lvNz[l], A.v + A.lvNz[l], dg, b, xin, xout, xzero);
+}
+
+RSG_D void res_residualL(const Thr& t, const RsgAmgDev& A, int l, const double* b, const double* x, double* r) {
+    const int n = A.lvN[l];
+    if (A.lvK[l] > 0) {
+        const int K = A.lvK[l];
+        const int* ec = A.eCol + A.lvE[l];
+        const double* ev = A.eVal + A.lvE[l];
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = 0.0;
+#pragma unroll 4
+            for (int k = 0; k < K; ++k) {
+                const int col = ec[k * n + i];
+                const double a = ev[k * n + i];
+                if (col >= 0) s += a * x[col];
+            }
+            r[i] = b[i] - s;
+        }
+    } else {
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l]; const double* v = A.v + A.lvNz[l];
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = 0.0;
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+            r[i] = b[i] - s;
+        }
+    }
+}
+
+// b of level lc = sum over each aggregate's fine rows (restriction order) of r.
+RSG_D void res_restrict(const Thr& t, const RsgAmgDev& A, int lc, const double* r, double* bc) {
+    const int nc = A.lvN[lc];
+    if (A.lvAK[lc] > 0) {
+        const int AK = A.lvAK[lc];
+        const int* ac = A.aCol + A.lvAE[lc];
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+#pragma unroll 4
+            for (int q = 0; q < AK; ++q) {
+                const int i = ac[q * nc + I];
+                if (i >= 0) s += r[i];
+            }
+            bc[I] = s;
+        }
+    } else {
+        const int* aP = A.aggPtr + A.lvRp[lc];
+        const int* aI = A.aggIdx + A.lvRow[lc - 1];
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
+            bc[I] = s;
+        }
+    }
+}
+
 // z = V-cycle(rin): the cycle of amg_vcycle() with level 0 in registers.
 template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const double (&rin)[CPT],
                                          double (&z)[CPT]) {
@@ -288,43 +372,18 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const do
         if (c < nC) rs[c] = rin[j] - res_row(R, j, tmp[j], xs);
     }
     RSG_SYNC();
-    {
-        const int nc = A.lvN[1];
-        const int* aP = A.aggPtr + A.lvRp[1];
-        const int* aI = A.aggIdx + A.lvRow[0];
-        double* b1 = cb + (A.lvRow[1] - base);
-        for (int I = t.tid; I < nc; I += t.bs) {
-            double s = 0.0;
-            for (int q = aP[I]; q < aP[I + 1]; ++q) s += rs[aI[q]];
-            b1[I] = s;
-        }
-    }
+    res_restrict(t, A, 1, rs, cb + (A.lvRow[1] - base));
     // coarse levels down (row I of a level is owned by thread I % bs throughout)
     for (int l = 1; l + 1 < L; ++l) {
-        const int n = A.lvN[l];
-        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
-        const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
         const int off = A.lvRow[l] - base;
         double* x = cx + off; double* bb = cb + off; double* r = cr + off;
-        res_jacobiL(t, n, rp, ci, v, dg, bb, x, r, true);
+        res_sweepL(t, A, l, bb, x, r, true);
         RSG_SYNC();
-        res_jacobiL(t, n, rp, ci, v, dg, bb, r, x, false);
+        res_sweepL(t, A, l, bb, r, x, false);
         RSG_SYNC();
-        for (int i = t.tid; i < n; i += t.bs) {
-            double s = 0.0;
-            for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
-            r[i] = bb[i] - s;
-        }
+        res_residualL(t, A, l, bb, x, r);
         RSG_SYNC();
-        const int nc = A.lvN[l + 1];
-        const int* aP = A.aggPtr + A.lvRp[l + 1];
-        const int* aI = A.aggIdx + A.lvRow[l];
-        double* b1 = cb + (A.lvRow[l + 1] - base);
-        for (int I = t.tid; I < nc; I += t.bs) {
-            double s = 0.0;
-            for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
-            b1[I] = s;
-        }
+        res_restrict(t, A, l + 1, r, cb + (A.lvRow[l + 1] - base));
     }
     RSG_SYNC();
     {   // coarsest level
@@ -356,17 +415,15 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const do
     // coarse levels up
     for (int l = L - 2; l >= 1; --l) {
         const int n = A.lvN[l];
-        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
-        const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
         const int off = A.lvRow[l] - base;
         double* x = cx + off; double* bb = cb + off; double* r = cr + off;
         const int* map = A.map + A.lvRow[l];
         const double* xc = cx + (A.lvRow[l + 1] - base);
         for (int i = t.tid; i < n; i += t.bs) x[i] += xc[map[i]];
         RSG_SYNC();
-        res_jacobiL(t, n, rp, ci, v, dg, bb, x, r, false);
+        res_sweepL(t, A, l, bb, x, r, false);
         RSG_SYNC();
-        res_jacobiL(t, n, rp, ci, v, dg, bb, r, x, false);
+        res_sweepL(t, A, l, bb, r, x, false);
         RSG_SYNC();
     }
     // level 0 up: prolongation and two sweeps

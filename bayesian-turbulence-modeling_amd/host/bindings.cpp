@@ -266,11 +266,13 @@ PYBIND11_MODULE(rans_sst_py, m) {
              py::arg("k_init") = 1e-4, py::arg("omega_init") = 1.0, py::keep_alive<1, 2>())
         .def("evaluate", &ForwardModel::evaluate, py::call_guard<py::gil_scoped_release>())
         .def("evaluate_batch", &ForwardModel::evaluateBatch, py::call_guard<py::gil_scoped_release>())
-        .def("submit", [](ForwardModel& fm, const std::vector<double>& theta) {
+        .def("submit", [](ForwardModel& fm, const std::vector<double>& theta, bool useCache) {
                  if (!fm.paramSet().inBounds(theta)) throw std::invalid_argument("theta out of bounds");
                  py::gil_scoped_release nogil;
-                 return fm.submit(theta, true);
-             }, py::arg("theta"), "Enqueue a solve on the device engine; returns a ticket.")
+                 return fm.submit(theta, useCache);
+             }, py::arg("theta"), py::arg("use_cache") = true,
+             "Enqueue a solve on the device engine (warm-started from the nearest cached state unless "
+             "use_cache is False); returns a ticket.")
         .def("complete", &ForwardModel::complete, py::arg("ticket"), py::arg("theta"),
              py::arg("store_cache") = true, py::call_guard<py::gil_scoped_release>(),
              "Wait for a ticket and return its EvaluationResult.")
@@ -314,7 +316,7 @@ PYBIND11_MODULE(rans_sst_py, m) {
           "Solve one theta per ForwardModel concurrently on the device; results in input order.");
 
     m.def("engine_set_launch", [](int iters, int block) { Engine::get().setLaunch(iters, block); },
-          py::arg("iters_per_launch"), py::arg("block_size") = 256);
+          py::arg("iters_per_launch"), py::arg("block_size") = 0);
     m.def("ticket_states",
           [](const std::vector<int64_t>& tickets) {
               std::vector<int32_t> st(tickets.size(), 0);

@@ -140,7 +140,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--iters-per-launch", type=int, default=500)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=0, help="0: engine chooses per launch (256 or 512)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -21,6 +21,7 @@ PKG = os.path.join(ROOT, "bayesian-turbulence-modeling_amd")
 TESTS = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, TESTS)
 sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.join(PKG, "python"))
 
 import rsg_abi  # noqa: E402
 

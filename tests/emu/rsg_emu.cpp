@@ -15,6 +15,7 @@
 // paths can be held against the oracle.
 static int g_resident = 1;
 extern "C" void emu_set_resident(int on) { g_resident = on; }
+extern "C" void emu_set_coarse_images(int on) { rsg::rsg_emu_no_images = !on; }
 
 static rsg::Thr emu_thr() {
     static double red[256];
@@ -58,6 +59,8 @@ static int emu_run(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
     for (int i = 0; i < 11; ++i) M.coeffs[i] = coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = init[i];
     M.initFromFields = state != nullptr;
+    M.planarZ = rsg::planar_z(P, P.nB == 0 || rsg::bc_zero_z(bcd, P.nB), s, state ? 0.0 : init[2],
+                              state ? state[0] : nullptr, state && P.nB > 0 ? state[1] : nullptr) ? 1 : 0;
     M.iterBudget = iterBudget > 0 ? iterBudget : s->max_iterations;
     M.sc->status = RSGM_RUNNING;
     M.sc->amgBuiltIter = -1;

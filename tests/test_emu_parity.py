@@ -132,3 +132,20 @@ def test_emulated_warm_start_bit_exact(oracle, emu, meshes, case):
         for k_emu, k_or in (("U", "U"), ("p", "p"), ("k", "k"), ("omega", "om"), ("nuT", "nuT"),
                             ("U_bnd", "Ub"), ("p_bnd", "pb"), ("k_bnd", "kb"), ("om_bnd", "omb")):
             assert np.array_equal(a[k_emu], o[k_or]), (turb, k_emu)
+
+
+def test_emulated_vcycle_level_images_are_exact(emu, meshes):
+    """The resident V-cycle's coarse levels through their ELL images round
+    exactly like the CSR levels (same entries, same order): whole AMG
+    trajectories agree bit for bit."""
+    s = settings_for("c0", max_iterations=120, convergence_tol=1e-12, pressure_solver=0, amg_refresh=7,
+                     coarse_refresh=3)
+    outs = []
+    for images in (1, 0):
+        emu.emu_set_coarse_images(images)
+        try:
+            outs.append(emu_solve(emu, meshes["c0"], "c0", s)[1])
+        finally:
+            emu.emu_set_coarse_images(1)
+    for k in ("U", "p", "k", "omega", "nuT"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
