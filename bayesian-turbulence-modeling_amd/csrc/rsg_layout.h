@@ -16,14 +16,17 @@
 #define RSG_COARSE_DENSE 64   // coarsest levels up to this size are inverted densely
 #define RSG_ELL_K 4           // internal faces per cell in the cell-major (ELL) face slots
 // Resident solves: at most RSG_RES_MAXN cells per member (LDS exchange
-// buffers) and RSG_RES_MAXC coarse AMG rows (LDS level vectors).
+// buffers) and RSG_RES_MAXC coarse AMG rows (LDS coarse-level x).
 #ifdef RSG_EMULATE
 #define RSG_RES_MAXN 2048
 #define RSG_RES_MAXC 2048
 #else
 #define RSG_RES_MAXN 1536
-#define RSG_RES_MAXC 768
+#define RSG_RES_MAXC 1536
 #endif
+// LDS copies of the coarse-level ELL images in the one-member-per-CU kernel
+#define RSG_RES_CE 6144       // matrix entries (value + column)
+#define RSG_RES_CA 2048       // aggregate entries
 
 // boundary-face flag bits (from the face's patch type)
 #define RSG_BF_WALL 1
@@ -148,6 +151,7 @@ struct RsgAmgDev {
     // lvK[l] / lvAK[l] = 0: that level has no image (capacity) and uses CSR.
     int eCap, aCap;
     int* lvK; int* lvE; int* lvAK; int* lvAE;   // [RSG_MAX_LEVELS+1]
+    int* used;             // [2]: entries of eCol/eVal and of aCol in use
     int* eCol;             // [eCap]
     double* eVal;          // [eCap]
     int* aCol;             // [aCap]

@@ -78,8 +78,11 @@ struct Thr {
     // resident solves (null xb0: not available)
     double *xb0, *xb1;   // LDS exchange buffers [RSG_RES_MAXN] each
     int xsel;
-    double* cvec;        // LDS coarse-level vectors x | b | r, cvecCap rows each
+    double* cvec;        // LDS coarse-level x, cvecCap rows
     int cvecCap;
+    // LDS copies of the coarse ELL images (null: read them from global memory)
+    double* ceVal; int* ceCol; int ceCap;
+    int* caCol; int caCap;
 };
 
 #ifdef RSG_EMULATE
@@ -620,6 +623,7 @@ RSG_D void amg_setup_serial(const RsgMeshDev* m, RsgAmgDev& A) {
         amg_level_images(A, l + 1, nc, crp, cci, cv, aP, aI, eNext, aNext);
         L = l + 2;
     }
+    A.used[0] = eNext; A.used[1] = aNext;
     A.lvN[RSG_MAX_LEVELS] = L;   // level count
 }
 

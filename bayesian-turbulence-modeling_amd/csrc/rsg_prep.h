@@ -324,6 +324,7 @@ inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int hist
     A.scr = cv.take<int>(3 * (size_t)A.rowCap + 2 * (size_t)A.nnzCap);
     A.eCap = A.nnzCap; A.aCap = A.rowCap;
     A.lvK = cv.take<int>(LV); A.lvE = cv.take<int>(LV); A.lvAK = cv.take<int>(LV); A.lvAE = cv.take<int>(LV);
+    A.used = cv.take<int>(2);
     A.eCol = cv.take<int>(A.eCap);
     A.eVal = cv.take<double>(A.eCap);
     A.aCol = cv.take<int>(A.aCap);

@@ -230,6 +230,58 @@ RSG_D void res_jacobiL(const Thr& t, int n, const int* rp, const int* ci, const 
     }
 }
 
+// Where the V-cycle reads the coarse ELL images from.  When the kernel
+// provides LDS room (one member per CU), the images of the deepest levels
+// that fit are copied there at the start of each solve (levels are appended
+// in order, so they form one contiguous tail of the global arrays); the
+// other levels read global memory.
+#ifdef RSG_EMULATE
+static int rsg_emu_staged_first = 0;
+#endif
+struct CoarseImg {
+    const int* gCol; const double* gVal; const int* gAgg;   // global images
+    const int* sCol; const double* sVal; const int* sAgg;   // LDS copies
+    int eFirst, eBase, aFirst, aBase;                        // first staged level, its offset
+};
+
+RSG_D const int* img_col(const CoarseImg& G, const RsgAmgDev& A, int l) {
+    return l >= G.eFirst ? G.sCol + (A.lvE[l] - G.eBase) : G.gCol + A.lvE[l];
+}
+RSG_D const double* img_val(const CoarseImg& G, const RsgAmgDev& A, int l) {
+    return l >= G.eFirst ? G.sVal + (A.lvE[l] - G.eBase) : G.gVal + A.lvE[l];
+}
+RSG_D const int* img_agg(const CoarseImg& G, const RsgAmgDev& A, int l) {
+    return l >= G.aFirst ? G.sAgg + (A.lvAE[l] - G.aBase) : G.gAgg + A.lvAE[l];
+}
+
+RSG_D CoarseImg coarse_images(Thr& t, const RsgAmgDev& A) {
+    CoarseImg g;
+    g.gCol = A.eCol; g.gVal = A.eVal; g.gAgg = A.aCol;
+    g.sCol = t.ceCol; g.sVal = t.ceVal; g.sAgg = t.caCol;
+    g.eFirst = g.aFirst = RSG_MAX_LEVELS + 1;   // nothing staged
+    g.eBase = g.aBase = 0;
+    if (!t.ceVal) return g;
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    for (int l = L - 1; l >= 1 && A.lvK[l] > 0 && A.used[0] - A.lvE[l] <= t.ceCap; --l) g.eFirst = l;
+    for (int l = L - 1; l >= 1 && A.lvAK[l] > 0 && A.used[1] - A.lvAE[l] <= t.caCap; --l) g.aFirst = l;
+#ifdef RSG_EMULATE
+    rsg_emu_staged_first = g.eFirst;   // emulation build only: observed by the tests
+#endif
+    if (g.eFirst <= RSG_MAX_LEVELS) {
+        g.eBase = A.lvE[g.eFirst];
+        for (int i = t.tid; i < A.used[0] - g.eBase; i += t.bs) {
+            t.ceVal[i] = A.eVal[g.eBase + i];
+            t.ceCol[i] = A.eCol[g.eBase + i];
+        }
+    }
+    if (g.aFirst <= RSG_MAX_LEVELS) {
+        g.aBase = A.lvAE[g.aFirst];
+        for (int i = t.tid; i < A.used[1] - g.aBase; i += t.bs) t.caCol[i] = A.aCol[g.aBase + i];
+    }
+    RSG_SYNC();
+    return g;
+}
+
 // The same sweep on a level's ELL image: entries in CSR order, so the sums
 // round exactly like res_jacobiL's; column -1 pads.
 RSG_D void res_jacobiE(const Thr& t, int n, int K, const int* ec, const double* ev, const double* dg,
@@ -252,22 +304,23 @@ RSG_D void res_jacobiE(const Thr& t, int n, int K, const int* ec, const double* 
 }
 
 // Coarse-level sweep / residual / restriction through the ELL images when present.
-RSG_D void res_sweepL(const Thr& t, const RsgAmgDev& A, int l, const double* b, const double* xin, double* xout,
-                      bool xzero) {
+RSG_D void res_sweepL(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int l, const double* b,
+                      const double* xin, double* xout, bool xzero) {
     const int n = A.lvN[l];
     const double* dg = A.dg + A.lvRow[l];
     if (A.lvK[l] > 0)
-        res_jacobiE(t, n, A.lvK[l], A.eCol + A.lvE[l], A.eVal + A.lvE[l], dg, b, xin, xout, xzero);
+        res_jacobiE(t, n, A.lvK[l], img_col(G, A, l), img_val(G, A, l), dg, b, xin, xout, xzero);
     else
         res_jacobiL(t, n, A.rp + A.lvRp[l], A.ci + A.lvNz[l], A.v + A.lvNz[l], dg, b, xin, xout, xzero);
 }
 
-RSG_D void res_residualL(const Thr& t, const RsgAmgDev& A, int l, const double* b, const double* x, double* r) {
+RSG_D void res_residualL(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int l, const double* b,
+                         const double* x, double* r) {
     const int n = A.lvN[l];
     if (A.lvK[l] > 0) {
         const int K = A.lvK[l];
-        const int* ec = A.eCol + A.lvE[l];
-        const double* ev = A.eVal + A.lvE[l];
+        const int* ec = img_col(G, A, l);
+        const double* ev = img_val(G, A, l);
         for (int i = t.tid; i < n; i += t.bs) {
             double s = 0.0;
 #pragma unroll 4
@@ -289,11 +342,12 @@ RSG_D void res_residualL(const Thr& t, const RsgAmgDev& A, int l, const double* 
 }
 
 // b of level lc = sum over each aggregate's fine rows (restriction order) of r.
-RSG_D void res_restrict(const Thr& t, const RsgAmgDev& A, int lc, const double* r, double* bc) {
+RSG_D void res_restrict(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int lc, const double* r,
+                        double* bc) {
     const int nc = A.lvN[lc];
     if (A.lvAK[lc] > 0) {
         const int AK = A.lvAK[lc];
-        const int* ac = A.aCol + A.lvAE[lc];
+        const int* ac = img_agg(G, A, lc);
         for (int I = t.tid; I < nc; I += t.bs) {
             double s = 0.0;
 #pragma unroll 4
@@ -315,15 +369,23 @@ RSG_D void res_restrict(const Thr& t, const RsgAmgDev& A, int lc, const double* 
 }
 
 // z = V-cycle(rin): the cycle of amg_vcycle() with level 0 in registers.
-template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const double (&rin)[CPT],
-                                         double (&z)[CPT]) {
+template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const CoarseImg& G,
+                                         const double (&rin)[CPT], double (&z)[CPT]) {
     Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg; const int nC = C.nC;
     const int L = A.lvN[RSG_MAX_LEVELS];
     const double wJ = 2.0 / 3.0;
     const int base = A.lvRow[1];                  // first coarse row in the level arrays
-    double* cx = t.cvec;                          // coarse x / b / r, rows base.. in LDS
-    double* cb = t.cvec + t.cvecCap;
-    double* cr = t.cvec + 2 * t.cvecCap;
+    // Coarse x (rows base..) lives in the LDS coarse-vector area; coarse b in
+    // the global level arrays (each b row is written and read by the thread
+    // that owns it, except at the coarsest level, which reads it after a
+    // barrier); the two exchange buffers, named explicitly here, carry level
+    // 0's gathered vectors and, while the coarse levels run, the coarse r.
+    // Every write to a buffer follows a barrier that comes after its last reads.
+    double* cx = t.cvec;
+    double* cb = A.b + base;
+    double* B0 = t.xb0;
+    double* B1 = t.xb1;
+    double* cr = B1;
     double x0[CPT], tmp[CPT];
     if (L == 1) {
         // level 0 is the coarsest: dense inverse (or Jacobi sweeps) on the whole vector
@@ -357,33 +419,30 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const do
     // level 0 down: two sweeps from zero, residual, restriction
 #pragma unroll
     for (int j = 0; j < CPT; ++j) x0[j] = (fabs(R.dg[j]) > 1e-30) ? wJ * rin[j] / R.dg[j] : 0.0;
-    double* xs = res_buf(t);
-    res_put(t, nC, x0, xs);
+    res_put(t, nC, x0, B0);
     RSG_SYNC();
-    res_jacobi0(t, nC, R, rin, x0, xs, tmp);
-    xs = res_buf(t);
-    res_put(t, nC, tmp, xs);
+    res_jacobi0(t, nC, R, rin, x0, B0, tmp);
+    res_put(t, nC, tmp, B1);
     RSG_SYNC();
-    double* rs = res_buf(t);
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int c = t.tid + j * t.bs;
         x0[j] = tmp[j];
-        if (c < nC) rs[c] = rin[j] - res_row(R, j, tmp[j], xs);
+        if (c < nC) B0[c] = rin[j] - res_row(R, j, tmp[j], B1);
     }
     RSG_SYNC();
-    res_restrict(t, A, 1, rs, cb + (A.lvRow[1] - base));
+    res_restrict(t, A, G, 1, B0, cb + (A.lvRow[1] - base));
     // coarse levels down (row I of a level is owned by thread I % bs throughout)
     for (int l = 1; l + 1 < L; ++l) {
         const int off = A.lvRow[l] - base;
         double* x = cx + off; double* bb = cb + off; double* r = cr + off;
-        res_sweepL(t, A, l, bb, x, r, true);
+        res_sweepL(t, A, G, l, bb, x, r, true);
         RSG_SYNC();
-        res_sweepL(t, A, l, bb, r, x, false);
+        res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();
-        res_residualL(t, A, l, bb, x, r);
+        res_residualL(t, A, G, l, bb, x, r);
         RSG_SYNC();
-        res_restrict(t, A, l + 1, r, cb + (A.lvRow[l + 1] - base));
+        res_restrict(t, A, G, l + 1, r, cb + (A.lvRow[l + 1] - base));
     }
     RSG_SYNC();
     {   // coarsest level
@@ -421,9 +480,9 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const do
         const double* xc = cx + (A.lvRow[l + 1] - base);
         for (int i = t.tid; i < n; i += t.bs) x[i] += xc[map[i]];
         RSG_SYNC();
-        res_sweepL(t, A, l, bb, x, r, false);
+        res_sweepL(t, A, G, l, bb, x, r, false);
         RSG_SYNC();
-        res_sweepL(t, A, l, bb, r, x, false);
+        res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();
     }
     // level 0 up: prolongation and two sweeps
@@ -436,14 +495,12 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const do
             if (c < nC) x0[j] += xc[map[c]];
         }
     }
-    xs = res_buf(t);
-    res_put(t, nC, x0, xs);
+    res_put(t, nC, x0, B0);
     RSG_SYNC();
-    res_jacobi0(t, nC, R, rin, x0, xs, tmp);
-    xs = res_buf(t);
-    res_put(t, nC, tmp, xs);
+    res_jacobi0(t, nC, R, rin, x0, B0, tmp);
+    res_put(t, nC, tmp, B1);
     RSG_SYNC();
-    res_jacobi0(t, nC, R, rin, tmp, xs, z);
+    res_jacobi0(t, nC, R, rin, tmp, B1, z);
 }
 
 // PCG with Jacobi or V-cycle preconditioning (the recurrence of pcg() above).
@@ -473,8 +530,10 @@ template <int CPT> RSG_D SolveRes pcg_res(Ctx& C, double* xg, int maxIter, doubl
     res.initialRes = r0;
     if (r0 < 1e-30) { res.converged = 1; return res; }
     double rzv[1] = {0.0};
+    CoarseImg G;
+    if (useAmg) G = coarse_images(t, C.M->amg);
     if (useAmg) {
-        vcycle_res(C, R, r, z);
+        vcycle_res(C, R, G, r, z);
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
             const int c = t.tid + j * t.bs;
@@ -529,7 +588,7 @@ template <int CPT> RSG_D SolveRes pcg_res(Ctx& C, double* xg, int maxIter, doubl
         if (res.finalRes < tol) { res.converged = 1; break; }
         double a3[1] = {0.0};
         if (useAmg) {
-            vcycle_res(C, R, r, z);
+            vcycle_res(C, R, G, r, z);
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
                 const int c = t.tid + j * t.bs;
@@ -566,7 +625,7 @@ RSG_D bool res_amg_fits(const Ctx& C) {
     const int L = A.lvN[RSG_MAX_LEVELS];
     if (L <= 1) return true;
     const int rows = A.lvRow[L - 1] + A.lvN[L - 1] - A.lvRow[1];
-    return rows <= C.t.cvecCap;
+    return rows <= C.t.cvecCap && rows <= RSG_RES_MAXN;   // coarse x, b in cvec; coarse r in xb1
 }
 
 template <int CPT> RSG_D SolveRes lsolve_cpt(Ctx& C, int code, double* x, bool amg, int maxIter, double tol) {

@@ -75,7 +75,7 @@ struct SolverSettings {
     double vonKarman = 0.41, wallFnE = 9.0;
     int sstVariant = 0;
     // engine knobs (no reference counterpart)
-    int amgRefresh = 1000;
+    int amgRefresh = 5000;
     int coarseRefresh = 10;
     bool recordHistory = false;
 

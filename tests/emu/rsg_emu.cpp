@@ -14,6 +14,9 @@
 // exchange buffers, registers) are used unless switched off, so both solver
 // paths can be held against the oracle.
 static int g_resident = 1;
+static int g_stage = 1;      // LDS copies of the coarse AMG images
+extern "C" void emu_set_stage(int on) { g_stage = on; }
+extern "C" int emu_last_staged_first(void) { return rsg::rsg_emu_staged_first; }
 extern "C" void emu_set_resident(int on) { g_resident = on; }
 extern "C" void emu_set_coarse_images(int on) { rsg::rsg_emu_no_images = !on; }
 
@@ -21,11 +24,15 @@ static rsg::Thr emu_thr() {
     static double red[256];
     static double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
     static double xb[2 * RSG_RES_MAXN];
-    static double cv[3 * RSG_RES_MAXC];
+    static double cv[RSG_RES_MAXC];
     rsg::Thr t;
     t.tid = 0; t.bs = 1; t.red = red; t.lds = lds; t.phase = 0;
     t.xb0 = g_resident ? xb : nullptr; t.xb1 = xb + RSG_RES_MAXN; t.xsel = 0;
     t.cvec = cv; t.cvecCap = RSG_RES_MAXC;
+    static double ceVal[RSG_RES_CE];
+    static int ceCol[RSG_RES_CE], caCol[RSG_RES_CA];
+    t.ceVal = g_stage ? ceVal : nullptr; t.ceCol = ceCol; t.ceCap = RSG_RES_CE;
+    t.caCol = caCol; t.caCap = RSG_RES_CA;
     return t;
 }
 

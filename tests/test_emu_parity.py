@@ -141,11 +141,38 @@ def test_emulated_vcycle_level_images_are_exact(emu, meshes):
     s = settings_for("c0", max_iterations=120, convergence_tol=1e-12, pressure_solver=0, amg_refresh=7,
                      coarse_refresh=3)
     outs = []
-    for images in (1, 0):
+    for images, stage in ((1, 1), (1, 0), (0, 0)):
         emu.emu_set_coarse_images(images)
+        emu.emu_set_stage(stage)
         try:
             outs.append(emu_solve(emu, meshes["c0"], "c0", s)[1])
         finally:
             emu.emu_set_coarse_images(1)
+            emu.emu_set_stage(1)
+    for o in outs[1:]:
+        for k in ("U", "p", "k", "omega", "nuT"):
+            assert np.array_equal(outs[0][k], o[k]), k
+
+
+def test_emulated_partial_staging_is_exact(emu, rs, tmp_path):
+    """A 4 x 384 periodic channel (the largest ensemble member): its coarse
+    images exceed the LDS staging capacity, so only the deepest levels are
+    copied; the trajectory equals the unstaged one bit for bit."""
+    import ensembles
+    ny, r = ensembles.member_spec(2000.0)
+    y = rs.Mesh.geometric_channel_nodes(ny // 2, r, 1.0)
+    m = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))
+    path = str(tmp_path / "c2max.msh")
+    m.save_binary(path)
+    s = settings_for("c0", max_iterations=25, convergence_tol=1e-12, pressure_solver=0, amg_refresh=1000,
+                     coarse_refresh=3)
+    outs = []
+    for stage in (1, 0):
+        emu.emu_set_stage(stage)
+        try:
+            outs.append(emu_solve(emu, path, "c0", s)[1])
+        finally:
+            emu.emu_set_stage(1)
     for k in ("U", "p", "k", "omega", "nuT"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+    assert emu.emu_last_staged_first() > 1     # some, not all, coarse levels were staged
