@@ -32,6 +32,7 @@
 #include "rsg_layout.h"
 #include "rsg_prep.h"
 #include "rsg_member.h"
+#include "rsg_grid.h"
 
 // ------------------------------------------------------------------ kernels
 namespace {
@@ -854,6 +855,112 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     ctx->arena.release(off, total);
     if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
     for (int i = 0; i < 4; ++i) stats[i] = st[i];
+    return RSG_OK;
+}
+
+int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                     const double* diag, const double* upper, const double* lower, const double* b, double* x,
+                     int32_t solver, int32_t precision, int32_t max_iter, double tol, double* stats) {
+    using namespace rsg::grid;
+    if (!ctx || n <= 0 || nif < 0 || !own || (nif > 0 && !nbr) || !diag || (nif > 0 && (!upper || !lower)) ||
+        !b || !x || !stats || max_iter < 0)
+        return RSG_ERR_ARG;
+    if (solver != RSG_SOLVER_AMG && solver != RSG_SOLVER_PCG)
+        return set_err(ctx, RSG_ERR_ARG, "solver must be RSG_SOLVER_AMG or RSG_SOLVER_PCG");
+    if (precision != RSG_PREC_FP64 && precision != RSG_PREC_MIXED)
+        return set_err(ctx, RSG_ERR_ARG, "unknown precision code");
+    for (int f = 0; f < nif; ++f)
+        if (own[f] < 0 || own[f] >= n || nbr[f] < 0 || nbr[f] >= n) return set_err(ctx, RSG_ERR_ARG, "face index out of range");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<Level> H(1);
+    ldu_to_csr(n, nif, own, nbr, diag, upper, lower, H[0]);
+    if (solver == RSG_SOLVER_AMG) {
+        while ((int)H.size() < RSG_MAX_LEVELS && H.back().n > RSG_MIN_COARSE) {
+            Level c;
+            if (!coarsen(H.back(), c)) break;
+            H.push_back(std::move(c));
+        }
+    } else {
+        H.resize(1);
+    }
+    std::vector<double> G;
+    const bool dense = solver == RSG_SOLVER_AMG && H.back().n <= 64 && dense_inverse(H.back(), G);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "hipSetDevice");
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "stream");
+    hipError_t e = hipSuccess;
+    PcgResult res;
+    int refinements = 0;
+    {
+        DevAlloc A;
+        Scratch S;
+        S.part = A.take<double>(2 * (size_t)blocks(n) + 2);
+        S.dots = A.take<double>(8);
+        if (A.err == hipSuccess) A.err = hipHostMalloc((void**)&S.hdots, 8 * sizeof(double), hipHostMallocDefault);
+        Hierarchy<double> H64;
+        if (precision == RSG_PREC_FP64) {
+            upload<double>(H, G, dense, A, H64, s);
+        } else {
+            std::vector<Level> top(1, H[0]);   // FP64 level 0 for the defect
+            std::vector<double> none;
+            upload<double>(top, none, false, A, H64, s);
+        }
+        double* b64 = A.take<double>(n);
+        double* x64 = A.take<double>(n);
+        if (A.err == hipSuccess) A.err = hipMemcpyAsync(b64, b, 8 * (size_t)n, hipMemcpyHostToDevice, s);
+        if (A.err == hipSuccess) A.err = hipMemcpyAsync(x64, x, 8 * (size_t)n, hipMemcpyHostToDevice, s);
+        if (precision == RSG_PREC_FP64) {
+            double* r = A.take<double>(n); double* z = A.take<double>(n);
+            double* p = A.take<double>(n); double* q = A.take<double>(n);
+            if (A.err == hipSuccess)
+                A.err = pcg<double>(H64, b64, x64, r, z, p, q, max_iter, tol, solver == RSG_SOLVER_AMG, S, s, res);
+        } else {
+            // FP64 defect / refinement around FP32 inner PCG solves to 1e-5 relative
+            Hierarchy<float> H32;
+            upload<float>(H, G, dense, A, H32, s);
+            double* r64 = A.take<double>(n);
+            float* b32 = A.take<float>(n); float* e32 = A.take<float>(n);
+            float* r = A.take<float>(n); float* z = A.take<float>(n);
+            float* p = A.take<float>(n); float* q = A.take<float>(n);
+            const DevLevel<double>& A0 = H64.lv[0];
+            const dim3 gr(blocks(n)), bl(kBlock);
+            double r0 = -1.0;
+            int inner = 0;
+            for (int k = 0; k < 30 && A.err == hipSuccess; ++k) {
+                hipLaunchKernelGGL(k_residual<double>, gr, bl, 0, s, n, A0.rp, A0.ci, A0.v, (const double*)b64,
+                                   (const double*)x64, r64);
+                dot2<double>(n, r64, r64, (const double*)nullptr, (const double*)nullptr, S, S.dots + 2, s);
+                A.err = hipMemcpyAsync(S.hdots, S.dots, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+                if (A.err == hipSuccess) A.err = hipStreamSynchronize(s);
+                if (A.err != hipSuccess) break;
+                const double rn = std::sqrt(S.hdots[2]);
+                if (r0 < 0) { r0 = rn; res.initialRes = rn; }
+                res.finalRes = r0 > 0 ? rn / r0 : 0.0;
+                if (r0 < 1e-30 || res.finalRes < tol) { res.converged = 1; break; }
+                if (inner >= max_iter) break;
+                hipLaunchKernelGGL((k_convert<float, double>), gr, bl, 0, s, n, (const double*)r64, b32);
+                hipLaunchKernelGGL(k_fill<float>, gr, bl, 0, s, n, e32, 0.0f);
+                PcgResult ir;
+                A.err = pcg<float>(H32, b32, e32, r, z, p, q, max_iter - inner, 1e-5, solver == RSG_SOLVER_AMG, S, s, ir);
+                inner += ir.iterations;
+                hipLaunchKernelGGL(k_add<float>, gr, bl, 0, s, n, (const float*)e32, x64);
+                ++refinements;
+            }
+            res.iterations = inner;
+        }
+        if (A.err == hipSuccess) A.err = hipMemcpyAsync(x, x64, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
+        if (A.err == hipSuccess) A.err = hipStreamSynchronize(s);
+        e = A.err;
+        if (S.hdots) (void)hipHostFree(S.hdots);
+    }
+    (void)hipStreamDestroy(s);
+    const auto t2 = std::chrono::steady_clock::now();
+    if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
+    stats[0] = res.iterations; stats[1] = res.finalRes; stats[2] = res.initialRes; stats[3] = res.converged;
+    stats[4] = (double)H.size(); stats[5] = refinements;
+    stats[6] = std::chrono::duration<double>(t1 - t0).count();
+    stats[7] = std::chrono::duration<double>(t2 - t1).count();
     return RSG_OK;
 }
 

@@ -59,10 +59,10 @@ def algorithmic_bytes(n, nif, nb, outer, inner):
     return outer * per_outer + inner_bytes
 
 
-def run_batch(rs, specs, tol, fms_cache=None):
+def run_batch(rs, specs, tol, fms_cache=None, max_iterations=200000):
     import ensembles
     if fms_cache is None:
-        meshes, fms = ensembles.build_models(rs, specs, tol=tol)
+        meshes, fms = ensembles.build_models(rs, specs, tol=tol, max_iterations=max_iterations)
     else:
         meshes, fms = fms_cache
     for fm in fms:
@@ -92,8 +92,21 @@ def run_batch(rs, specs, tol, fms_cache=None):
     return (meshes, fms), res, wall
 
 
-def cpu_baseline(specs, tol, budget_s=30.0):
-    """Reference solver (oracle/_ref) on one core, one member to tol."""
+def _member_mesh_file(rs, sp, path):
+    y = rs.Mesh.geometric_channel_nodes(sp["ny"] // 2, sp["ratio"], 1.0)
+    mesh = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))
+    mesh.save_binary(path)
+    return 4 * sp["ny"]
+
+
+def cpu_baseline(specs, tol, results, sample_iters=1500):
+    """The reference solver compiled from its own sources (oracle/_ref) on one
+    host core.  Sample: `sample_iters` outer iterations of three members (the
+    smallest, the median and the largest mesh of the batch) give the CPU time
+    per outer iteration as a function of the cell count (linear fit); the CPU
+    time of the whole batch is the sum over members of their outer-iteration
+    counts (this run's, the same SIMPLE algorithm) times that cost, and the
+    reported value is the batch's converged solves over that time."""
     import ctypes as C
     lib_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
     kind = "reference"
@@ -103,32 +116,39 @@ def cpu_baseline(specs, tol, budget_s=30.0):
     if not os.path.exists(lib_path):
         return None
     import rans_sst_py as rs
-    sp = min(specs, key=lambda s: s["ny"])           # smallest member keeps the sample to ~10-30 s
-    y = rs.Mesh.geometric_channel_nodes(sp["ny"] // 2, sp["ratio"], 1.0)
-    mesh = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))
-    path = "/tmp/bench_cpu_member.msh"
-    mesh.save_binary(path)
-    coeffs = [0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]
-    for idx, v in zip([8, 9, 0, 1, 2], sp["theta"]):
-        coeffs[idx] = v
-    packed = [400000, tol, 1e6, 0.7, 0.3, 0.5, 0.5, 200, 1e-3, 5, 1, 500, 0, 1.0, 0, 0,
-              1e-10, 1e10, 1e-6, 0, 0.41, 0, 0, 2, 2]
-    D = lambda a: (C.c_double * len(a))(*a)
     lib = C.CDLL(lib_path)
-    info = (C.c_int * 5)()
-    t0 = time.perf_counter()
-    if kind == "reference":
-        lib.ref_simple_solve(path.encode(), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs), D(packed),
-                             D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
-    else:
-        lib.om_mesh_load.restype = C.c_void_p
-        m = lib.om_mesh_load(path.encode())
-        lib.om_simple_solve(C.c_void_p(m), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs), D(packed),
-                            D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": kind,
-            "sample": "one member (Re_tau=%.0f, 4x%d, %d cells) solved to %g in %d outer iterations, %.1f s"
-                      % (sp["re_tau"], sp["ny"], 4 * sp["ny"], tol, info[0], dt)}
+    D = lambda a: (C.c_double * len(a))(*a)
+    by_size = sorted(specs, key=lambda s: s["ny"])
+    picks = [by_size[0], by_size[len(by_size) // 2], by_size[-1]]
+    cells, ms = [], []
+    for i, sp in enumerate(picks):
+        path = "/tmp/bench_cpu_member_%d.msh" % i
+        n = _member_mesh_file(rs, sp, path)
+        coeffs = [0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]
+        for idx, v in zip([8, 9, 0, 1, 2], sp["theta"]):
+            coeffs[idx] = v
+        packed = [sample_iters, 1e-300, 1e6, 0.7, 0.3, 0.5, 0.5, 200, 1e-3, 5, 1, 500, 0, 1.0, 0, 0,
+                  1e-10, 1e10, 1e-6, 0, 0.41, 0, 0, 2, 2]
+        info = (C.c_int * 5)()
+        t0 = time.perf_counter()
+        if kind == "reference":
+            lib.ref_simple_solve(path.encode(), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs),
+                                 D(packed), D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+        else:
+            lib.om_mesh_load.restype = C.c_void_p
+            m = lib.om_mesh_load(path.encode())
+            lib.om_simple_solve(C.c_void_p(m), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs),
+                                D(packed), D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+        dt = time.perf_counter() - t0
+        cells.append(n)
+        ms.append(1e3 * dt / max(info[0], 1))
+    a, b = np.polyfit(np.array(cells, dtype=float), np.array(ms), 1)
+    cpu_s = sum(max(r["outer"], 1) * (a * r["cells"] + b) for r in results) / 1e3
+    conv = sum(r["converged"] for r in results)
+    return {"value": conv / cpu_s, "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": "%d outer iterations of 3 members (%s cells): %s ms/iteration on one core; batch CPU "
+                      "time = sum of this run's outer-iteration counts x that cost = %.0f s"
+                      % (sample_iters, "/".join(str(c) for c in cells), "/".join("%.2f" % v for v in ms), cpu_s)}
 
 
 def main():
@@ -142,6 +162,8 @@ def main():
     ap.add_argument("--iters-per-launch", type=int, default=500)
     ap.add_argument("--block", type=int, default=0, help="0: engine chooses per launch (256 or 512)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-iterations", type=int, default=200000,
+                    help="outer-iteration cap per member (profiling runs shorten it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this build has a single MI355X "
@@ -156,13 +178,13 @@ def main():
     rs.evaluate_many(prime, [specs[0]["theta"]])
     cache = None
     for _ in range(args.warmup):
-        cache, _, wall = run_batch(rs, specs, args.tol, cache)
+        cache, _, wall = run_batch(rs, specs, args.tol, cache, args.max_iterations)
         print("bench: warmup step %.2f s" % wall, file=sys.stderr, flush=True)
     stats0 = rs.engine_stats()
     walls, lat, conv, outer = [], [], 0, []
     res = None
     for _ in range(args.steps):
-        cache, res, wall = run_batch(rs, specs, args.tol, cache)
+        cache, res, wall = run_batch(rs, specs, args.tol, cache, args.max_iterations)
         walls.append(wall)
         print("bench: step %d %.2f s" % (len(walls), wall), file=sys.stderr, flush=True)
         for r in res:
@@ -211,7 +233,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_simple_chunk", "launches": launches,
                      "avg_launch_ms": kernel_ms / max(launches, 1)},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(specs, args.tol),
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(
+            specs, args.tol, [dict(outer=r.simple_iters, cells=m.n_cells(),
+                                   converged=str(r.status).split(".")[-1] == "Converged")
+                              for r, m in zip(res, meshes)]),
     }
     print(json.dumps(out), flush=True)
 

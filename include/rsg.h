@@ -183,6 +183,20 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
                   double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
                   double* stats);
 
+/* One large LDU linear solve spread over the whole GPU (the stand-alone
+ * pressure-type system): PCG from x (in/out, host buffers) preconditioned by
+ * the AMG V-cycle (solver RSG_SOLVER_AMG, hierarchy built on the host with the
+ * reference's aggregation) or Jacobi (RSG_SOLVER_PCG); tol is relative to
+ * ||b - A x0||.  precision RSG_PREC_FP64: FP64 throughout; RSG_PREC_MIXED: FP64
+ * defect and update around FP32 inner PCG solves (relative 1e-5 each).
+ * Replaces AMGSolver::solve / PCGSolver::solve (core/include/LinearSolver.hpp:
+ * 602-653, 147-186) on systems too large for one workgroup.
+ * stats = {iterations (inner, summed), finalRes, initialRes, converged, levels,
+ *          refinement sweeps, setup seconds, solve seconds}. */
+int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                     const double* diag, const double* upper, const double* lower, const double* b, double* x,
+                     int32_t solver, int32_t precision, int32_t max_iter, double tol, double* stats);
+
 /* Greedy aggregation of one CSR matrix (host, bit-exact to the reference).
  * Returns the number of aggregates, or a negative status. */
 int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
