@@ -872,17 +872,8 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
     for (int f = 0; f < nif; ++f)
         if (own[f] < 0 || own[f] >= n || nbr[f] < 0 || nbr[f] >= n) return set_err(ctx, RSG_ERR_ARG, "face index out of range");
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<Level> H(1);
-    ldu_to_csr(n, nif, own, nbr, diag, upper, lower, H[0]);
-    if (solver == RSG_SOLVER_AMG) {
-        while ((int)H.size() < RSG_MAX_LEVELS && H.back().n > RSG_MIN_COARSE) {
-            Level c;
-            if (!coarsen(H.back(), c)) break;
-            H.push_back(std::move(c));
-        }
-    } else {
-        H.resize(1);
-    }
+    std::vector<Level> H;
+    build_hierarchy(n, nif, own, nbr, diag, upper, lower, solver == RSG_SOLVER_AMG ? RSG_MAX_LEVELS : 0, H);
     std::vector<double> G;
     const bool dense = solver == RSG_SOLVER_AMG && H.back().n <= 64 && dense_inverse(H.back(), G);
     const auto t1 = std::chrono::steady_clock::now();
@@ -962,6 +953,28 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
     stats[6] = std::chrono::duration<double>(t1 - t0).count();
     stats[7] = std::chrono::duration<double>(t2 - t1).count();
     return RSG_OK;
+}
+
+int rsg_system_hierarchy(int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr, const double* diag,
+                         const double* upper, const double* lower, int32_t* level_sizes, int32_t* level_nnz,
+                         int32_t max_levels, int32_t* maps, int64_t maps_cap) {
+    using namespace rsg::grid;
+    if (n <= 0 || nif < 0 || !own || (nif > 0 && !nbr) || !diag || (nif > 0 && (!upper || !lower)) || max_levels < 0)
+        return RSG_ERR_ARG;
+    for (int f = 0; f < nif; ++f)
+        if (own[f] < 0 || own[f] >= n || nbr[f] < 0 || nbr[f] >= n) return RSG_ERR_ARG;
+    std::vector<Level> H;
+    build_hierarchy(n, nif, own, nbr, diag, upper, lower, RSG_MAX_LEVELS, H);
+    int64_t off = 0;
+    for (size_t l = 0; l < H.size(); ++l) {
+        if ((int)l < max_levels) {
+            if (level_sizes) level_sizes[l] = H[l].n;
+            if (level_nnz) level_nnz[l] = H[l].nnz;
+        }
+        if (maps)
+            for (size_t i = 0; i < H[l].map.size() && off < maps_cap; ++i) maps[off++] = H[l].map[i];
+    }
+    return (int)H.size();
 }
 
 int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* values,

@@ -270,6 +270,21 @@ inline bool coarsen(Level& f, Level& c) {
     return true;
 }
 
+// AMGSolver::setup (LinearSolver.hpp:516-550): level 0 from the LDU arrays,
+// then coarsening while the level has more than RSG_MIN_COARSE rows, at most
+// `maxCoarsenings` times, stopping early when aggregation no longer reduces.
+inline void build_hierarchy(int n, int nif, const int* own, const int* nbr, const double* diag,
+                            const double* upper, const double* lower, int maxCoarsenings, std::vector<Level>& H) {
+    H.assign(1, Level());
+    ldu_to_csr(n, nif, own, nbr, diag, upper, lower, H[0]);
+    for (int l = 0; l < maxCoarsenings; ++l) {
+        if (H.back().n <= RSG_MIN_COARSE) break;
+        Level c;
+        if (!coarsen(H.back(), c)) break;
+        H.push_back(std::move(c));
+    }
+}
+
 // Gauss-Jordan inverse of the coarsest level (no pivoting: SPD M-matrix image),
 // the same elimination as the member solver's amg_coarse_inverse.
 inline bool dense_inverse(const Level& L, std::vector<double>& G) {

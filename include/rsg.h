@@ -17,6 +17,9 @@
  *   rsg_fetch              FlowFields + ConvergenceHistory returned by solve
  *                          (core/include/FlowFields.hpp:9-49)
  *   rsg_ldu_solve          ILinearSolver::solve (core/include/LinearSolver.hpp:115-120)
+ *   rsg_system_solve       AMGSolver::solve / PCGSolver::solve on one large system
+ *                          (core/include/LinearSolver.hpp:602-653, :147-186)
+ *   rsg_system_hierarchy   AMGSolver::setup (core/include/LinearSolver.hpp:516-550)
  *   rsg_amg_aggregate      buildAggregates (core/include/LinearSolver.hpp:407-443)
  */
 #ifndef RSG_H
@@ -196,6 +199,15 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
 int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
                      const double* diag, const double* upper, const double* lower, const double* b, double* x,
                      int32_t solver, int32_t precision, int32_t max_iter, double tol, double* stats);
+
+/* Host-only: the AMG hierarchy rsg_system_solve builds (AMGSolver::setup,
+ * core/include/LinearSolver.hpp:516-550).  level_sizes/level_nnz get up to
+ * max_levels entries; maps gets the concatenated fine->aggregate maps of every
+ * level but the coarsest (up to maps_cap entries).  Returns the level count,
+ * or RSG_ERR_ARG. */
+int rsg_system_hierarchy(int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr, const double* diag,
+                         const double* upper, const double* lower, int32_t* level_sizes, int32_t* level_nnz,
+                         int32_t max_levels, int32_t* maps, int64_t maps_cap);
 
 /* Greedy aggregation of one CSR matrix (host, bit-exact to the reference).
  * Returns the number of aggregates, or a negative status. */

@@ -1,0 +1,123 @@
+"""GPU parity of the large-system solve (rsg_system_solve through the C-ABI):
+the backward-facing-step pressure-type system of c1_system (mixed
+triangle/quad cells, an SPD M-matrix) solved by PCG with the AMG V-cycle or
+Jacobi, FP64 or FP32-inner with FP64 refinement.
+
+Bars: the returned x meets the requested relative residual when the residual
+is recomputed on the host in FP64; x matches a direct solve (scipy) to the
+bound the tolerance and conditioning allow; the Jacobi-PCG recurrence takes
+the oracle's iteration count; the mixed-precision solution is within 1e-6
+rel-L2 of the FP64 one (the accuracy bar of the solve metric).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import c1_system
+import rsg_abi as A
+from test_gpu_parity import chk, rel
+
+pytestmark = pytest.mark.gpu
+
+AMG, PCG = 0, 1
+FP64, MIXED = 0, 1
+SMALL = dict(nx_up=15, nx_dn=45, ny_up=20, ny_low=4, wall_layers=3)
+
+
+def solve(lib, ctx, s, solver, prec, tol, x0=None, max_iter=5000):
+    x = np.zeros(s["n"]) if x0 is None else x0.copy()
+    st = np.zeros(8)
+    print("system solve n=%d solver=%d precision=%d tol=%g" % (s["n"], solver, prec, tol), flush=True)
+    chk(lib, ctx, lib.rsg_system_solve(ctx, s["n"], s["nif"], A.iptr(s["own"]), A.iptr(s["nbr"]),
+                                       A.dptr(s["diag"]), A.dptr(s["upper"]), A.dptr(s["lower"]),
+                                       A.dptr(s["b"]), A.dptr(x), solver, prec, max_iter, C.c_double(tol),
+                                       A.dptr(st)))
+    return x, st
+
+
+def true_rel_res(s, x, x0=None):
+    M = c1_system.csr_matrix(s)
+    r0 = s["b"] - (M @ x0 if x0 is not None else 0.0)
+    return np.linalg.norm(s["b"] - M @ x) / np.linalg.norm(r0)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return c1_system.make_system(**SMALL)
+
+
+@pytest.fixture(scope="module")
+def small_direct(small):
+    import scipy.sparse.linalg as spl
+    return spl.spsolve(c1_system.csr_matrix(small).tocsc(), small["b"])
+
+
+@pytest.mark.parametrize("solver,prec", [(AMG, FP64), (PCG, FP64), (AMG, MIXED), (PCG, MIXED)])
+def test_small_system(rsg_lib, gpu_ctx, small, small_direct, solver, prec):
+    x, st = solve(rsg_lib, gpu_ctx, small, solver, prec, 1e-10)
+    assert st[3] == 1, st
+    assert st[1] < 1e-10
+    assert st[2] == pytest.approx(np.linalg.norm(small["b"]), rel=1e-12)
+    assert true_rel_res(small, x) < 1.5e-10
+    assert rel(x, small_direct) < 1e-7
+    assert st[4] == (1 if solver == PCG else st[4]) and st[4] >= 1
+    if prec == MIXED:
+        assert st[5] >= 2   # at least one refinement beyond the first inner solve
+
+
+def test_small_jacobi_pcg_iterations_match_oracle(rsg_lib, gpu_ctx, oracle, small):
+    x, st = solve(rsg_lib, gpu_ctx, small, PCG, FP64, 1e-10)
+    xo = np.zeros(small["n"]); so = np.zeros(4)
+    s = small
+    oracle.om_ldu_solve(1, s["n"], s["nif"], A.iptr(s["own"]), A.iptr(s["nbr"]), A.dptr(s["diag"]),
+                        A.dptr(s["upper"]), A.dptr(s["lower"]), A.dptr(s["b"]), A.dptr(xo), 5000,
+                        C.c_double(1e-10), A.dptr(so))
+    assert so[3] == 1 and abs(st[0] - so[0]) <= 2
+    assert rel(x, xo) < 1e-7
+
+
+def test_small_hierarchy_levels(rsg_lib, gpu_ctx, small):
+    sizes = np.zeros(32, np.int32)
+    nl = rsg_lib.rsg_system_hierarchy(small["n"], small["nif"], A.iptr(small["own"]), A.iptr(small["nbr"]),
+                                      A.dptr(small["diag"]), A.dptr(small["upper"]), A.dptr(small["lower"]),
+                                      A.iptr(sizes), None, 32, None, C.c_int64(0))
+    _, st = solve(rsg_lib, gpu_ctx, small, AMG, FP64, 1e-8)
+    assert st[4] == nl
+
+
+def test_small_warm_start(rsg_lib, gpu_ctx, small, small_direct):
+    x0 = small_direct + 1e-3 * np.random.default_rng(1).normal(size=small["n"])
+    x, st = solve(rsg_lib, gpu_ctx, small, AMG, FP64, 1e-8, x0=x0)
+    assert st[3] == 1
+    assert true_rel_res(small, x, x0) < 1.5e-8
+
+
+def test_zero_rhs_is_converged_without_iterations(rsg_lib, gpu_ctx, small):
+    z = dict(small); z["b"] = np.zeros(small["n"])
+    x, st = solve(rsg_lib, gpu_ctx, z, AMG, FP64, 1e-10)
+    assert st[3] == 1 and st[0] == 0 and np.all(x == 0.0)
+
+
+def test_rejects_bad_arguments(rsg_lib, gpu_ctx, small):
+    x = np.zeros(small["n"]); st = np.zeros(8)
+    s = small
+    args = (s["n"], s["nif"], A.iptr(s["own"]), A.iptr(s["nbr"]), A.dptr(s["diag"]), A.dptr(s["upper"]),
+            A.dptr(s["lower"]), A.dptr(s["b"]), A.dptr(x))
+    assert rsg_lib.rsg_system_solve(gpu_ctx, *args, 3, FP64, 10, C.c_double(1e-8), A.dptr(st)) != 0
+    assert rsg_lib.rsg_system_solve(gpu_ctx, *args, AMG, 7, 10, C.c_double(1e-8), A.dptr(st)) != 0
+    bad = s["nbr"].copy(); bad[0] = s["n"] + 5
+    assert rsg_lib.rsg_system_solve(gpu_ctx, s["n"], s["nif"], A.iptr(s["own"]), A.iptr(bad), A.dptr(s["diag"]),
+                                    A.dptr(s["upper"]), A.dptr(s["lower"]), A.dptr(s["b"]), A.dptr(x), AMG,
+                                    FP64, 10, C.c_double(1e-8), A.dptr(st)) != 0
+
+
+def test_full_size_fp64_and_mixed(rsg_lib, gpu_ctx):
+    s = c1_system.make_system()
+    x64, st64 = solve(rsg_lib, gpu_ctx, s, AMG, FP64, 1e-10)
+    assert st64[3] == 1 and true_rel_res(s, x64) < 1.5e-10
+    xm, stm = solve(rsg_lib, gpu_ctx, s, AMG, MIXED, 1e-10)
+    assert stm[3] == 1 and true_rel_res(s, xm) < 1.5e-10
+    assert rel(xm, x64) < 1e-6
+    print("full size: fp64 %d it %.3f s, mixed %d it %d sweeps %.3f s, rel diff %.2e"
+          % (st64[0], st64[7], stm[0], stm[5], stm[7], rel(xm, x64)), flush=True)
