@@ -746,30 +746,33 @@ RSG_D void csr_jacobi(Thr& t, int n, const int* rp, const int* ci, const double*
     }
 }
 
+// Residual row i of a CSR level: b_i - sum of the row's entries times x, in
+// CSR order.
+RSG_D double csr_row_residual(const int* rp, const int* ci, const double* v, const double* b, const double* x,
+                              int i) {
+    double s = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+    return b[i] - s;
+}
+
+// Level 0's right-hand side is rin itself.  Going down, each coarse row sums
+// the residuals of its aggregate's rows straight from x (a residual pass, a
+// barrier and the restriction, rounded the same, without the barrier); going
+// up, the first post-smoothing sweep reads every x value as the prolonged
+// x[j] + xc[map[j]], so the prolongation pass and its barrier go away.
 RSG_D void amg_vcycle(Ctx& C, const double* rin, double* z) {
     Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg;
     const int L = A.lvN[RSG_MAX_LEVELS];
-    // level 0 right-hand side
-    {
-        double* b0 = A.b;
-        const int n0 = A.lvN[0];
-        for (int i = t.tid; i < n0; i += t.bs) b0[i] = rin[i];
-    }
-    RSG_SYNC();
+    const double wJ = 2.0 / 3.0;
     for (int l = 0; l + 1 < L; ++l) {
         const int n = A.lvN[l];
         const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
         const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
-        double* x = A.x + A.lvRow[l]; double* b = A.b + A.lvRow[l]; double* r = A.r + A.lvRow[l];
+        double* x = A.x + A.lvRow[l]; double* r = A.r + A.lvRow[l];
+        const double* b = (l == 0) ? rin : A.b + A.lvRow[l];
         csr_jacobi(t, n, rp, ci, v, dg, b, x, r, true);
         RSG_SYNC();
         csr_jacobi(t, n, rp, ci, v, dg, b, r, x, false);
-        RSG_SYNC();
-        for (int i = t.tid; i < n; i += t.bs) {
-            double s = 0.0;
-            for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
-            r[i] = b[i] - s;
-        }
         RSG_SYNC();
         const int nc = A.lvN[l + 1];
         const int* aP = A.aggPtr + A.lvRp[l + 1];
@@ -777,14 +780,15 @@ RSG_D void amg_vcycle(Ctx& C, const double* rin, double* z) {
         double* cb = A.b + A.lvRow[l + 1];
         for (int I = t.tid; I < nc; I += t.bs) {
             double s = 0.0;
-            for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += csr_row_residual(rp, ci, v, b, x, aI[q]);
             cb[I] = s;
         }
-        RSG_SYNC();
     }
+    RSG_SYNC();
     {   // coarsest level
         const int l = L - 1, n = A.lvN[l];
-        double* x = A.x + A.lvRow[l]; const double* b = A.b + A.lvRow[l];
+        double* x = (L == 1) ? z : A.x + A.lvRow[l];
+        const double* b = (l == 0) ? rin : A.b + A.lvRow[l];
         if (M->sc->coarseDense) {
             const int DN = RSG_COARSE_DENSE;
             for (int i = t.tid; i < n; i += t.bs) {
@@ -811,20 +815,20 @@ RSG_D void amg_vcycle(Ctx& C, const double* rin, double* z) {
         const int n = A.lvN[l];
         const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
         const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
-        double* x = A.x + A.lvRow[l]; double* b = A.b + A.lvRow[l]; double* r = A.r + A.lvRow[l];
+        double* x = A.x + A.lvRow[l]; double* r = A.r + A.lvRow[l];
+        const double* b = (l == 0) ? rin : A.b + A.lvRow[l];
         const int* map = A.map + A.lvRow[l];
         const double* cx = A.x + A.lvRow[l + 1];
-        for (int i = t.tid; i < n; i += t.bs) x[i] += cx[map[i]];
-        RSG_SYNC();
-        csr_jacobi(t, n, rp, ci, v, dg, b, x, r, false);
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = b[i];
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * (x[ci[e]] + cx[map[ci[e]]]);
+            const double d = dg[i];
+            const double x0 = x[i] + cx[map[i]];
+            r[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+        }
         RSG_SYNC();
         double* out = (l == 0) ? z : x;
         csr_jacobi(t, n, rp, ci, v, dg, b, r, out, false);
-        RSG_SYNC();
-    }
-    if (L == 1) {
-        const int n = A.lvN[0];
-        for (int i = t.tid; i < n; i += t.bs) z[i] = A.x[i];
         RSG_SYNC();
     }
 }

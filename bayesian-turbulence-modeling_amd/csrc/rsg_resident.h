@@ -303,7 +303,7 @@ RSG_D void res_jacobiE(const Thr& t, int n, int K, const int* ec, const double* 
     }
 }
 
-// Coarse-level sweep / residual / restriction through the ELL images when present.
+// Coarse-level sweep and restriction through the ELL images when present.
 RSG_D void res_sweepL(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int l, const double* b,
                       const double* xin, double* xout, bool xzero) {
     const int n = A.lvN[l];
@@ -312,33 +312,6 @@ RSG_D void res_sweepL(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int 
         res_jacobiE(t, n, A.lvK[l], img_col(G, A, l), img_val(G, A, l), dg, b, xin, xout, xzero);
     else
         res_jacobiL(t, n, A.rp + A.lvRp[l], A.ci + A.lvNz[l], A.v + A.lvNz[l], dg, b, xin, xout, xzero);
-}
-
-RSG_D void res_residualL(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int l, const double* b,
-                         const double* x, double* r) {
-    const int n = A.lvN[l];
-    if (A.lvK[l] > 0) {
-        const int K = A.lvK[l];
-        const int* ec = img_col(G, A, l);
-        const double* ev = img_val(G, A, l);
-        for (int i = t.tid; i < n; i += t.bs) {
-            double s = 0.0;
-#pragma unroll 4
-            for (int k = 0; k < K; ++k) {
-                const int col = ec[k * n + i];
-                const double a = ev[k * n + i];
-                if (col >= 0) s += a * x[col];
-            }
-            r[i] = b[i] - s;
-        }
-    } else {
-        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l]; const double* v = A.v + A.lvNz[l];
-        for (int i = t.tid; i < n; i += t.bs) {
-            double s = 0.0;
-            for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
-            r[i] = b[i] - s;
-        }
-    }
 }
 
 // b of level lc = sum over each aggregate's fine rows (restriction order) of r.
@@ -364,6 +337,93 @@ RSG_D void res_restrict(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, in
             double s = 0.0;
             for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
             bc[I] = s;
+        }
+    }
+}
+
+// Residual row i of level l at x: b_i - sum of the row entries times x in
+// CSR order (the ELL image holds the entries in that order).
+RSG_D double res_row_residual(const RsgAmgDev& A, const CoarseImg& G, int l, int i, const double* b,
+                              const double* x) {
+    double s = 0.0;
+    if (A.lvK[l] > 0) {
+        const int n = A.lvN[l], K = A.lvK[l];
+        const int* ec = img_col(G, A, l);
+        const double* ev = img_val(G, A, l);
+#pragma unroll 4
+        for (int k = 0; k < K; ++k) {
+            const int col = ec[k * n + i];
+            const double a = ev[k * n + i];
+            if (col >= 0) s += a * x[col];
+        }
+    } else {
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l]; const double* v = A.v + A.lvNz[l];
+        for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+    }
+    return b[i] - s;
+}
+
+// b of level l+1 straight from x of level l: each coarse row sums the
+// residuals of its aggregate's fine rows (restriction order) -- a residual
+// pass, a barrier and res_restrict with the same roundings, one barrier fewer.
+RSG_D void res_restrict_residual(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int l, const double* b,
+                                 const double* x, double* bc) {
+    const int lc = l + 1, nc = A.lvN[lc];
+    if (A.lvAK[lc] > 0) {
+        const int AK = A.lvAK[lc];
+        const int* ac = img_agg(G, A, lc);
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+            for (int q = 0; q < AK; ++q) {
+                const int i = ac[q * nc + I];
+                if (i >= 0) s += res_row_residual(A, G, l, i, b, x);
+            }
+            bc[I] = s;
+        }
+    } else {
+        const int* aP = A.aggPtr + A.lvRp[lc];
+        const int* aI = A.aggIdx + A.lvRow[l];
+        for (int I = t.tid; I < nc; I += t.bs) {
+            double s = 0.0;
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += res_row_residual(A, G, l, aI[q], b, x);
+            bc[I] = s;
+        }
+    }
+}
+
+// First post-smoothing sweep of level l with the prolongation folded in:
+// every x value is read as x[j] + xc[map[j]] (the prolonged value, rounded as
+// the in-place update x[j] += xc[map[j]] rounds it), so the separate
+// prolongation pass and its barrier go away.  xout = xp + wJ (b - A xp) / d.
+RSG_D void res_sweep_prolong(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, int l, const double* b,
+                             const double* x, const double* xc, const int* map, double* xout) {
+    const double wJ = 2.0 / 3.0;
+    const int n = A.lvN[l];
+    const double* dg = A.dg + A.lvRow[l];
+    if (A.lvK[l] > 0) {
+        const int K = A.lvK[l];
+        const int* ec = img_col(G, A, l);
+        const double* ev = img_val(G, A, l);
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = b[i];
+#pragma unroll 4
+            for (int k = 0; k < K; ++k) {
+                const int col = ec[k * n + i];
+                const double a = ev[k * n + i];
+                if (col >= 0) s -= a * (x[col] + xc[map[col]]);
+            }
+            const double d = dg[i];
+            const double x0 = x[i] + xc[map[i]];
+            xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+        }
+    } else {
+        const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l]; const double* v = A.v + A.lvNz[l];
+        for (int i = t.tid; i < n; i += t.bs) {
+            double s = b[i];
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * (x[ci[e]] + xc[map[ci[e]]]);
+            const double d = dg[i];
+            const double x0 = x[i] + xc[map[i]];
+            xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
         }
     }
 }
@@ -440,9 +500,7 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         RSG_SYNC();
         res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();
-        res_residualL(t, A, G, l, bb, x, r);
-        RSG_SYNC();
-        res_restrict(t, A, G, l + 1, r, cb + (A.lvRow[l + 1] - base));
+        res_restrict_residual(t, A, G, l, bb, x, cb + (A.lvRow[l + 1] - base));
     }
     RSG_SYNC();
     {   // coarsest level
@@ -478,9 +536,8 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         double* x = cx + off; double* bb = cb + off; double* r = cr + off;
         const int* map = A.map + A.lvRow[l];
         const double* xc = cx + (A.lvRow[l + 1] - base);
-        for (int i = t.tid; i < n; i += t.bs) x[i] += xc[map[i]];
-        RSG_SYNC();
-        res_sweepL(t, A, G, l, bb, x, r, false);
+        (void)n;
+        res_sweep_prolong(t, A, G, l, bb, x, xc, map, r);
         RSG_SYNC();
         res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();

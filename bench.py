@@ -10,16 +10,26 @@ workgroup per member, persistent launches, continuous batching).
 Prints ONE JSON line (rank 0).  value = converged solves / wall second over the
 timed steps; p50/p99 = per-member latency (submit -> result) inside a step.
 roofline = algorithmic bytes of the SIMPLE chunk kernel / its HIP-event time,
-against the 8 TB/s HBM peak.  cpu_baseline = the reference solver compiled
-from its own sources (oracle/_ref) on one host core, one member solved to the
-same tolerance.
+against the 8 TB/s HBM peak; traffic = the kernel's measured HBM bytes per
+launch (two rocprofv3 counter passes after the timed region, scaled by a copy
+kernel).  cpu_baseline = the reference solver compiled from its own sources
+(oracle/_ref) on one host core: a bounded sample of three members gives the
+cost per outer iteration, applied to this run's iteration counts.
+
+--workload c1 instead times the stand-alone 261,600-cell backward-facing-step
+pressure system (rsg_system_solve, AMG-PCG to 1e-10, FP64 and FP32-inner with
+FP64 refinement) next to the reference's CPU PCG on the same system.
 """
 import argparse
+import glob
 import json
 import math
 import os
+import shutil
+import sqlite3
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -151,6 +161,155 @@ def cpu_baseline(specs, tol, results, sample_iters=1500):
                       % (sample_iters, "/".join(str(c) for c in cells), "/".join("%.2f" % v for v in ms), cpu_s)}
 
 
+COPY_BYTES = 256 << 20
+
+
+def copy_calibration():
+    """A plain copy kernel of known size (256 MiB read, 256 MiB written), run
+    under the same counters as the solver to scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print("copy: %d bytes read, %d bytes written" % (COPY_BYTES, COPY_BYTES), flush=True)
+
+
+def _pmc_run(counter, argv, tag, tmp):
+    """One rocprofv3 --pmc pass over `argv` (a python program); returns
+    ([(kernel name, counter KiB)], stdout)."""
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    out = os.path.join(tmp, tag)
+    cmd = ["timeout", "-s", "KILL", "240", rocprof, "--pmc", counter, "-d", out, "-o", "run", "--",
+           sys.executable, "-u"] + argv
+    r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s pass exited with %d: %s" % (counter, r.returncode, r.stderr[-400:]))
+    dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise RuntimeError("%s pass wrote no database" % counter)
+    rows = []
+    for db in dbs:
+        con = sqlite3.connect(db)
+        rows += list(con.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                                 (counter,)))
+        con.close()
+    return rows, r.stdout
+
+
+def measure_traffic(args):
+    """HBM traffic of the chunk kernel relative to its algorithmic bytes.
+
+    Two counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over
+    the same ensemble capped at 1000 outer iterations, each scaled by what the
+    counter reads for a plain copy kernel of known size; the ratio of the
+    scaled bytes to the algorithmic bytes of the same pass."""
+    tmp = tempfile.mkdtemp(prefix="bench_pmc_", dir="/tmp")
+    try:
+        me = os.path.abspath(__file__)
+        cal = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows, _ = _pmc_run(counter, [me, "--copy-calibration"], "cal_" + counter, tmp)
+            kib = [v for k, v in rows if "MulFunctor" in k]
+            if not kib:
+                raise RuntimeError("copy kernel not found in the %s calibration" % counter)
+            cal[counter] = COPY_BYTES / (1024.0 * kib[0])
+        work = [me, "--members", str(args.members), "--seed", str(args.seed), "--tol", str(args.tol),
+                "--iters-per-launch", str(args.iters_per_launch), "--block", str(args.block),
+                "--max-iterations", "1000", "--no-cpu", "--no-traffic"]
+        hbm, alg, launches = {}, {}, {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows, stdout = _pmc_run(counter, work, counter, tmp)
+            kib = [v for k, v in rows if "k_simple_chunk" in k]
+            line = json.loads(stdout.strip().splitlines()[-1])
+            hbm[counter] = 1024.0 * sum(kib) * cal[counter]
+            alg[counter] = line["roofline"]["alg_bytes_total"]
+            launches[counter] = len(kib)
+        ratio = hbm["FETCH_SIZE"] / alg["FETCH_SIZE"] + hbm["WRITE_SIZE"] / alg["WRITE_SIZE"]
+        return {"hbm_over_algorithmic": ratio,
+                "fetch_scale": cal["FETCH_SIZE"], "write_scale": cal["WRITE_SIZE"],
+                "fetch_bytes": hbm["FETCH_SIZE"], "write_bytes": hbm["WRITE_SIZE"],
+                "alg_bytes": alg["FETCH_SIZE"], "launches": launches["FETCH_SIZE"],
+                "sample": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over the same %d-member ensemble "
+                          "capped at 1000 outer iterations; counters scaled by a 256 MiB copy kernel"
+                          % args.members}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def bench_c1(args):
+    """The large single system: solves/s of AMG-PCG to 1e-10 (host hierarchy
+    setup + device solve + host<->device copies, wall clock), FP64 and mixed."""
+    import ctypes as C
+    import c1_system
+    lib = C.CDLL(os.path.join(PKG, "librsg.so"))
+    lib.rsg_last_error.restype = C.c_char_p
+    ctx = C.c_void_p()
+    if lib.rsg_context_create(0, C.c_uint64(8 << 30), C.byref(ctx)) != 0:
+        raise RuntimeError("rsg_context_create failed: no usable HIP device")
+    s = c1_system.make_system()
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    M = c1_system.csr_matrix(s)
+
+    def solve(prec):
+        x = np.zeros(s["n"]); st = np.zeros(8)
+        t0 = time.perf_counter()
+        rc = lib.rsg_system_solve(ctx, s["n"], s["nif"], ip(s["own"]), ip(s["nbr"]), dp(s["diag"]), dp(s["upper"]),
+                                  dp(s["lower"]), dp(s["b"]), dp(x), 0, prec, 20000, C.c_double(1e-10), dp(st))
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError("rsg_system_solve: %s" % lib.rsg_last_error(ctx))
+        return x, st, dt
+
+    out = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        for _ in range(args.warmup):
+            solve(prec)
+        walls, st = [], None
+        for _ in range(args.steps):
+            x, st, dt = solve(prec)
+            walls.append(dt)
+            print("bench: c1 %s %.3f s (%d iterations)" % (name, dt, st[0]), file=sys.stderr, flush=True)
+        out[name] = dict(x=x, st=st, walls=walls,
+                         true_res=float(np.linalg.norm(s["b"] - M @ x) / np.linalg.norm(s["b"])))
+    lib.rsg_context_destroy(ctx)
+    f, m = out["fp64"], out["mixed"]
+    rel = float(np.linalg.norm(m["x"] - f["x"]) / np.linalg.norm(f["x"]))
+    cpu = None
+    lib_ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    kind = "reference"
+    if not os.path.exists(lib_ref):
+        lib_ref, kind = os.path.join(ROOT, "oracle", "_build", "liboracle.so"), "port"
+    if not args.no_cpu and os.path.exists(lib_ref):
+        L = C.CDLL(lib_ref)
+        fn = L.ref_ldu_solve if kind == "reference" else L.om_ldu_solve
+        x = np.zeros(s["n"]); st = np.zeros(4)
+        t0 = time.perf_counter()
+        fn(1, s["n"], s["nif"], ip(s["own"]), ip(s["nbr"]), dp(s["diag"]), dp(s["upper"]), dp(s["lower"]),
+           dp(s["b"]), dp(x), 20000, C.c_double(1e-10), dp(st))
+        dt = time.perf_counter() - t0
+        cpu = {"value": (1.0 / dt) if st[3] == 1 else 0.0, "unit": "solves/s", "cores": 1, "kind": kind,
+               "sample": "one full solve: the reference PCGSolver (Jacobi) to 1e-10, %d iterations, %.1f s; its "
+                         "AMGSolver does not reach 1e-10 on this system (forward Gauss-Seidel V-cycle)"
+                         % (st[0], dt)}
+    tf = float(np.median(f["walls"]))
+    print(json.dumps({
+        "metric": METRIC, "value": 1.0 / tf, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * tf, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C1 stand-alone pressure system: BFS expansion 1.2, %d cells (%d tri + %d quad), "
+                               "log-normal nu_t sigma 1, AMG-PCG to 1e-10 relative residual"
+                               % (s["n"], s["n_tri"], s["n_quad"]), "cells": s["n"], "tol": 1e-10},
+        "fp64": {"iterations": int(f["st"][0]), "levels": int(f["st"][4]), "setup_s": f["st"][6],
+                 "device_solve_s": f["st"][7], "wall_s": tf, "true_rel_res": f["true_res"]},
+        "mixed": {"iterations": int(m["st"][0]), "refinements": int(m["st"][5]), "setup_s": m["st"][6],
+                  "device_solve_s": m["st"][7], "wall_s": float(np.median(m["walls"])),
+                  "true_rel_res": m["true_res"], "rel_l2_vs_fp64": rel},
+        "roofline": None, "cpu_baseline": cpu}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,13 +321,24 @@ def main():
     ap.add_argument("--iters-per-launch", type=int, default=500)
     ap.add_argument("--block", type=int, default=0, help="0: engine chooses per launch (256 or 512)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
+    ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--workload", choices=("c2", "c1"), default="c2",
+                    help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system")
     ap.add_argument("--max-iterations", type=int, default=200000,
                     help="outer-iteration cap per member (profiling runs shorten it)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this build has a single MI355X "
-              "(ensemble members shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): ensemble members would shard across GPUs with no "
+              "exchange between them, and runs on several GPUs are not part of this benchmark", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c1":
+        bench_c1(args)
+        return
     import rans_sst_py as rs
     import ensembles
     specs = ensembles.channel_ensemble(args.members, args.seed)
@@ -205,6 +375,15 @@ def main():
         bytes_alg += algorithmic_bytes(n, nif, mesh.n_faces() - nif, r.simple_iters, inner)
     bytes_alg *= args.steps
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    alg_per_launch = bytes_alg / max(launches, 1)
+    traffic, traffic_info = None, None
+    if not args.no_traffic:
+        try:
+            traffic_info = measure_traffic(args)
+            traffic = traffic_info["hbm_over_algorithmic"] * alg_per_launch
+        except Exception as e:   # counters unavailable: report null with the reason
+            traffic_info = {"error": str(e)[:300]}
+        print("bench: traffic passes done", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": conv / total,
@@ -230,9 +409,11 @@ def main():
         "outer_iters_median": float(np.median(outer)),
         "max_rel_l2_vs_fp64": 0.0,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_simple_chunk", "launches": launches,
-                     "avg_launch_ms": kernel_ms / max(launches, 1)},
+                     "avg_launch_ms": kernel_ms / max(launches, 1),
+                     "alg_bytes_per_launch": alg_per_launch, "alg_bytes_total": bytes_alg,
+                     "traffic_measurement": traffic_info},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(
             specs, args.tol, [dict(outer=r.simple_iters, cells=m.n_cells(),
                                    converged=str(r.status).split(".")[-1] == "Converged")
