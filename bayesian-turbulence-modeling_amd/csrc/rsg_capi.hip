@@ -271,6 +271,8 @@ struct rsg_context {
     int64_t nextTicket = 1;
     int itersPerLaunch = 200;
     int blockSize = 0;       // 0: chosen per launch from the number of active members
+    int sliceUs = 0;         // > 0: each launch runs every member for this long (no iteration cap)
+    int clockKHz = 0;        // device wall clock (wall_clock64) frequency
     int maxSlots = 4096;
     std::thread engine;
     bool stop = false;
@@ -442,7 +444,13 @@ void engine_loop(rsg_context* ctx) {
         ctx->hStatus.resize(na);
         for (int i = 0; i < na; ++i) {
             Ticket& T = *ctx->tickets[ctx->active[i]];
-            T.M.iterBudget = ctx->itersPerLaunch;
+            if (ctx->sliceUs > 0 && ctx->clockKHz > 0) {
+                T.M.iterBudget = 1 << 30;
+                T.M.sliceTicks = (long long)ctx->sliceUs * ctx->clockKHz / 1000;
+            } else {
+                T.M.iterBudget = ctx->itersPerLaunch;
+                T.M.sliceTicks = 0;
+            }
             ctx->hMembers[i] = T.M;
         }
         // Auto: while more members are active than there are CUs, two
@@ -518,6 +526,7 @@ int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** 
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return RSG_ERR_DEVICE;
     c->cuCount = prop.multiProcessorCount;
     c->warpSize = prop.warpSize;
+    if (hipDeviceGetAttribute(&c->clockKHz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->clockKHz = 0;
     if (hbm_budget_bytes == 0) hbm_budget_bytes = 8ull << 30;
     if (hipMalloc(&c->arena.base, hbm_budget_bytes) != hipSuccess) return RSG_ERR_NOMEM;
     c->arena.size = hbm_budget_bytes;
@@ -789,6 +798,15 @@ int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_siz
     return RSG_OK;
 }
 
+int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds) {
+    if (!ctx || microseconds < 0) return RSG_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (microseconds > 0 && ctx->clockKHz <= 0)
+        return set_err(ctx, RSG_ERR_UNSUPPORTED, "device wall clock rate unknown: time slices unavailable");
+    ctx->sliceUs = microseconds;
+    return RSG_OK;
+}
+
 int rsg_stats(rsg_context* ctx, double* out) {
     if (!ctx || !out) return RSG_ERR_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -907,7 +925,9 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
             if (A.err == hipSuccess)
                 A.err = pcg<double>(H64, b64, x64, r, z, p, q, max_iter, tol, solver == RSG_SOLVER_AMG, S, s, res);
         } else {
-            // FP64 defect / refinement around FP32 inner PCG solves to 1e-5 relative
+            // FP64 defect / refinement around FP32 inner PCG solves; each inner
+            // solve reduces its defect by what is still missing (with a 0.3
+            // margin), but never asks FP32 for more than 1e-5
             Hierarchy<float> H32;
             upload<float>(H, G, dense, A, H32, s);
             double* r64 = A.take<double>(n);
@@ -933,7 +953,9 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
                 hipLaunchKernelGGL((k_convert<float, double>), gr, bl, 0, s, n, (const double*)r64, b32);
                 hipLaunchKernelGGL(k_fill<float>, gr, bl, 0, s, n, e32, 0.0f);
                 PcgResult ir;
-                A.err = pcg<float>(H32, b32, e32, r, z, p, q, max_iter - inner, 1e-5, solver == RSG_SOLVER_AMG, S, s, ir);
+                const double innerTol = std::max(1e-5, 0.3 * tol / res.finalRes);
+                A.err = pcg<float>(H32, b32, e32, r, z, p, q, max_iter - inner, innerTol, solver == RSG_SOLVER_AMG,
+                                   S, s, ir);
                 inner += ir.iterations;
                 hipLaunchKernelGGL(k_add<float>, gr, bl, 0, s, n, (const float*)e32, x64);
                 ++refinements;

@@ -69,24 +69,6 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int* __restrict_
     xo[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int* __restrict__ aP, const int* __restrict__ aI,
-                                                      const T* __restrict__ r, T* __restrict__ bc) {
-    const int I = blockIdx.x * kBlock + threadIdx.x;
-    if (I >= nc) return;
-    T s = 0;
-    for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
-    bc[I] = s;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_prolong(int n, const int* __restrict__ map, const T* __restrict__ xc,
-                                                     T* __restrict__ x) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    x[i] += xc[map[i]];
-}
-
 // x = C^{-1} b on the coarsest level (n <= 64), one block
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense(int n, const T* __restrict__ cinv, const T* __restrict__ b,
@@ -313,6 +295,52 @@ template <typename T> struct DevLevel {
     T *x = nullptr, *b = nullptr, *r = nullptr;
 };
 
+// ------------------------------------------------------- fused kernels
+// b_c[I] = sum over the aggregate's fine rows i of (b_f[i] - A_f[i,:] x_f)
+// (a residual pass and the restriction in one launch, residual rows summed
+// in aggregate order); when x1 is given, also the coarse level's first sweep
+// from zero, x1[I] = wJ b_c[I] / d_c[I].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict_residual(int nc, const int* __restrict__ aP,
+                                                               const int* __restrict__ aI, const int* __restrict__ rp,
+                                                               const int* __restrict__ ci, const T* __restrict__ v,
+                                                               const T* __restrict__ bf, const T* __restrict__ xf,
+                                                               T* __restrict__ bc, const T* __restrict__ dgc,
+                                                               T* __restrict__ x1) {
+    const int I = blockIdx.x * kBlock + threadIdx.x;
+    if (I >= nc) return;
+    T s = 0;
+    for (int q = aP[I]; q < aP[I + 1]; ++q) {
+        const int i = aI[q];
+        T a = 0;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) a += v[e] * xf[ci[e]];
+        s += bf[i] - a;
+    }
+    bc[I] = s;
+    if (x1) {
+        const T d = dgc[I];
+        x1[I] = (fabs((double)d) > 1e-30) ? T(2) / T(3) * s / d : T(0);
+    }
+}
+
+// First post-smoothing sweep with the prolongation folded in: every x value
+// is read as the prolonged x[j] + xc[map[j]]; xo = xp + wJ (b - A xp) / d.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi_prolong(int n, const int* __restrict__ rp,
+                                                            const int* __restrict__ ci, const T* __restrict__ v,
+                                                            const T* __restrict__ dg, const T* __restrict__ b,
+                                                            const T* __restrict__ x, const T* __restrict__ xc,
+                                                            const int* __restrict__ map, T* __restrict__ xo) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const T wJ = T(2) / T(3);
+    T s = b[i];
+    for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * (x[ci[e]] + xc[map[ci[e]]]);
+    const T x0 = x[i] + xc[map[i]];
+    const T d = dg[i];
+    xo[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
+}
+
 struct DevAlloc {
     std::vector<void*> ptrs;
     hipError_t err = hipSuccess;
@@ -367,33 +395,46 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
 
 inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
-// z = V-cycle(r) on hierarchy D (level 0 vectors: rin, z; level matrices in D)
+// z = V-cycle(r) on hierarchy D (level 0 vectors: rin, z; level matrices in D):
+// one grid-wide kernel per phase.  Down: sweep from zero, sweep, then the
+// residual folded into the restriction, which also makes the next level's
+// first sweep; up: prolongation folded into the first post-sweep, sweep.
+// (Single-workgroup kernels for the small levels were measured slower: one CU
+// cannot hide the dependent-load latency of a few-thousand-row level.)
 template <typename T>
 void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
     const int L = (int)D.lv.size();
+    const dim3 bl(kBlock);
     if (L == 1) {
         const DevLevel<T>& d = D.lv[0];
-        if (D.dense) { hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, d.n, D.cinv, rin, z); return; }
-        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), dim3(kBlock), 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin,
+        if (D.dense) { hipLaunchKernelGGL(k_dense<T>, dim3(1), bl, 0, s, d.n, D.cinv, rin, z); return; }
+        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin,
                            (const T*)nullptr, z);
         return;
     }
+    {
+        const DevLevel<T>& d = D.lv[0];
+        const dim3 g(blocks(d.n));
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin, (const T*)nullptr, d.r);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin, (const T*)d.r, d.x);
+    }
     for (int l = 0; l + 1 < L; ++l) {
         const DevLevel<T>& d = D.lv[l];
-        const T* b = (l == 0) ? rin : d.b;
-        const dim3 g(blocks(d.n)), bl(kBlock);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)nullptr, d.r);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.r, d.x);
-        hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, b, (const T*)d.x, d.r);
         const DevLevel<T>& c = D.lv[l + 1];
-        hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(c.n)), bl, 0, s, c.n, c.aP, c.aI, (const T*)d.r, c.b);
+        const T* b = (l == 0) ? rin : d.b;
+        const bool cSmooth = (l + 1) < L - 1;   // next level is smoothed (not the coarsest)
+        hipLaunchKernelGGL(k_restrict_residual<T>, dim3(blocks(c.n)), bl, 0, s, c.n, c.aP, c.aI, d.rp, d.ci, d.v, b,
+                           (const T*)d.x, c.b, (const T*)c.dg, cSmooth ? c.r : (T*)nullptr);
+        if (!cSmooth) break;
+        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(c.n)), bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b,
+                           (const T*)c.r, c.x);
     }
     {
         const DevLevel<T>& c = D.lv[L - 1];
         if (D.dense) {
-            hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, c.n, D.cinv, (const T*)c.b, c.x);
+            hipLaunchKernelGGL(k_dense<T>, dim3(1), bl, 0, s, c.n, D.cinv, (const T*)c.b, c.x);
         } else {
-            const dim3 g(blocks(c.n)), bl(kBlock);
+            const dim3 g(blocks(c.n));
             hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)nullptr, c.x);
             for (int q = 0; q < 25; ++q) {
                 hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.x, c.r);
@@ -404,9 +445,9 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
     for (int l = L - 2; l >= 0; --l) {
         const DevLevel<T>& d = D.lv[l];
         const T* b = (l == 0) ? rin : d.b;
-        const dim3 g(blocks(d.n)), bl(kBlock);
-        hipLaunchKernelGGL(k_prolong<T>, g, bl, 0, s, d.n, d.map, (const T*)D.lv[l + 1].x, d.x);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.x, d.r);
+        const dim3 g(blocks(d.n));
+        hipLaunchKernelGGL(k_jacobi_prolong<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.x,
+                           (const T*)D.lv[l + 1].x, d.map, d.r);
         hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.r, l == 0 ? z : d.x);
     }
 }

@@ -167,6 +167,7 @@ struct RsgMemberDev {
     int initFromFields;    // state already uploaded (warm start)
     int planarZ;           // Uz stays exactly zero: its momentum solve is skipped
     int iterBudget;        // outer iterations allowed in this launch
+    long long sliceTicks;  // wall-clock ticks this launch may run the member (0: no time limit)
     // fields (cell) and boundary values
     double *Ux, *Uy, *Uz, *p, *k, *om, *nuT, *F1, *F2, *Pk, *CDkw, *Smag;
     double *Ubx, *Uby, *Ubz, *pb, *kb, *omb;

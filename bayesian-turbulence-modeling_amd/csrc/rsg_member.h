@@ -1392,7 +1392,9 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     RSG_TMARK(13);
 }
 
-// Runs up to M->iterBudget outer iterations of one member.
+// Runs up to M->iterBudget outer iterations of one member, and no longer than
+// M->sliceTicks of wall clock when that is set (the iteration in progress
+// finishes; a member's results do not depend on where launches split it).
 template <int CMAX>
 RSG_D void run_member(Thr t, RsgMemberDev* M) {
     Ctx C;
@@ -1434,7 +1436,23 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
         for (int idx = C.t.tid; idx < nL * RSG_COARSE_DENSE; idx += C.t.bs) C.t.lds[idx] = A.cinv[idx];
         RSG_SYNC();
     }
-    for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) simple_iteration<CMAX>(C, S);
+#ifndef RSG_EMULATE
+    const long long tStart = wall_clock64();
+#endif
+    for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) {
+        simple_iteration<CMAX>(C, S);
+#ifndef RSG_EMULATE
+        // time slice: thread 0 reads the clock and every thread follows its
+        // decision, so the workgroup leaves the loop together
+        if (M->sliceTicks > 0) {
+            if (C.t.tid == 0) C.t.red[0] = (wall_clock64() - tStart >= M->sliceTicks) ? 1.0 : 0.0;
+            RSG_SYNC();
+            const bool stop = C.t.red[0] != 0.0;
+            RSG_SYNC();
+            if (stop) break;
+        }
+#endif
+    }
     if (C.t.tid == 0) {
         sc->iter = S.iter; sc->status = S.status; sc->turbEstablished = S.turbEstablished;
         sc->finalIter = S.finalIter; sc->amgBuiltIter = S.amgBuiltIter; sc->coarseIter = S.coarseIter;

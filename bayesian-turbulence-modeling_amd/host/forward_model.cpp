@@ -44,6 +44,12 @@ void Engine::setLaunch(int itersPerLaunch, int blockSize) {
     if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
 }
 
+void Engine::setTimeSlice(int microseconds) {
+    rsg_context* c = ctx();
+    int rc = rsg_set_time_slice(c, microseconds);
+    if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
+}
+
 std::vector<double> Engine::stats() {
     std::vector<double> s(5, 0.0);
     rsg_context* c = ctx();

@@ -37,6 +37,7 @@ public:
     rsg_context* ctx();           // throws if the GPU engine cannot start
     void shutdown();
     void setLaunch(int itersPerLaunch, int blockSize);
+    void setTimeSlice(int microseconds);
     std::vector<double> stats();
 private:
     std::mutex mu_;

@@ -10,7 +10,7 @@ Meshes (a request picks one uniformly):
     reference's Couette forward model builds it
     (python/UQ/datasets/couette_forward.py:49-75).
 
-Usage: python serve_c3.py [--rates 50,100] [--requests N] [--max-iterations M]
+Usage: python serve_c3.py [--rates 50,100] [--requests N] [--max-iterations M] [--slice-ms T]
 Prints one JSON line per rate: throughput (converged solves/s over the span
 from first arrival to last result), p50/p99 latency (arrival -> result).
 """
@@ -68,8 +68,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iterations", type=int, default=200000)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--slice-ms", type=float, default=100.0,
+                    help="engine time slice per launch (0: fixed iteration budget)")
     args = ap.parse_args()
     import rans_sst_py as rs
+    rs.engine_set_time_slice(int(args.slice_ms * 1000))
     keep, models, names = build_models(rs, args.tol, args.max_iterations)
     for rate in [float(r) for r in args.rates.split(",")]:
         trace = serving.arrival_trace(rate, args.requests, len(models), n_chains=args.chains, seed=args.seed)

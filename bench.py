@@ -5,7 +5,8 @@ channel solves (4 x ny quads, ny 96-384 from Re_tau log-uniform in
 [180, 2000], betaStar/a1/sigma_k1/sigma_w1/beta1 within +-20 % of their
 defaults), each driven by SIMPLE + SST k-omega to the reference's convergence
 criterion at tol 1e-8, all members resident on the device at once (one
-workgroup per member, persistent launches, continuous batching).
+workgroup per member, persistent launches time-sliced at 200 ms, continuous
+batching).
 
 Prints ONE JSON line (rank 0).  value = converged solves / wall second over the
 timed steps; p50/p99 = per-member latency (submit -> result) inside a step.
@@ -217,6 +218,7 @@ def measure_traffic(args):
             cal[counter] = COPY_BYTES / (1024.0 * kib[0])
         work = [me, "--members", str(args.members), "--seed", str(args.seed), "--tol", str(args.tol),
                 "--iters-per-launch", str(args.iters_per_launch), "--block", str(args.block),
+                "--slice-ms", str(args.slice_ms),
                 "--max-iterations", "1000", "--no-cpu", "--no-traffic"]
         hbm, alg, launches = {}, {}, {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -320,6 +322,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--iters-per-launch", type=int, default=500)
     ap.add_argument("--block", type=int, default=0, help="0: engine chooses per launch (256 or 512)")
+    ap.add_argument("--slice-ms", type=float, default=200.0,
+                    help="time slice per launch in ms (0: fixed --iters-per-launch budget)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
@@ -343,6 +347,7 @@ def main():
     import ensembles
     specs = ensembles.channel_ensemble(args.members, args.seed)
     rs.engine_set_launch(args.iters_per_launch, args.block)
+    rs.engine_set_time_slice(int(args.slice_ms * 1000))
     # engine start-up and code-object load, outside the timed region
     _, prime = ensembles.build_models(rs, specs[:1], tol=args.tol, max_iterations=2)
     rs.evaluate_many(prime, [specs[0]["theta"]])
@@ -401,7 +406,8 @@ def main():
                                "betaStar/a1/sigma_k1/sigma_w1/beta1 +-20%%, SIMPLE+SST to tol %g, FP64"
                                % (args.members, args.tol),
                    "members": args.members, "tol": args.tol, "seed": args.seed,
-                   "iters_per_launch": args.iters_per_launch, "block": args.block},
+                   "iters_per_launch": args.iters_per_launch, "block": args.block,
+                   "slice_ms": args.slice_ms},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "converged": int(conv),

@@ -175,6 +175,12 @@ int rsg_release(rsg_context* ctx, int64_t ticket);
  * (256: two members per CU; 512: one member per CU, lower latency; 0, the
  * default: 256 while more members are active than the device has CUs, else 512). */
 int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_size);
+/* Time-sliced launches: with microseconds > 0 every launch runs each active
+ * member for that long of device wall clock (the outer iteration in progress
+ * completes), instead of a fixed iters_per_launch, so small members are not
+ * held to the pace of the largest one in the launch.  0 (the default) returns
+ * to the fixed iteration budget.  Results do not depend on the setting. */
+int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds);
 /* Engine counters: out[0] launches, out[1] members admitted, out[2] members retired,
  * out[3] kernel milliseconds (HIP events), out[4] max concurrent members. */
 int rsg_stats(rsg_context* ctx, double* out);
@@ -191,7 +197,8 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
  * the AMG V-cycle (solver RSG_SOLVER_AMG, hierarchy built on the host with the
  * reference's aggregation) or Jacobi (RSG_SOLVER_PCG); tol is relative to
  * ||b - A x0||.  precision RSG_PREC_FP64: FP64 throughout; RSG_PREC_MIXED: FP64
- * defect and update around FP32 inner PCG solves (relative 1e-5 each).
+ * defect and update around FP32 inner PCG solves, each asked for the
+ * reduction still missing (0.3 margin), at most 1e-5 relative.
  * Replaces AMGSolver::solve / PCGSolver::solve (core/include/LinearSolver.hpp:
  * 602-653, 147-186) on systems too large for one workgroup.
  * stats = {iterations (inner, summed), finalRes, initialRes, converged, levels,

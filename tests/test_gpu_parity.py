@@ -190,6 +190,31 @@ def test_member_identical_alone_and_in_mix(rsg_lib, gpu_ctx, meshes):
     assert alone.final_iter == mix.final_iter
 
 
+def test_time_sliced_launches_do_not_change_results(rsg_lib, gpu_ctx, meshes):
+    """rsg_set_time_slice: launches split a member at wall-clock boundaries
+    (here ~2 ms, a few iterations each) instead of every 300 iterations; the
+    member's fields and iteration count are bit-identical."""
+    s = settings_for("c0", max_iterations=1200, convergence_tol=1e-12, pressure_solver=0, amg_refresh=1000)
+    P = Problem(rsg_lib, gpu_ctx, meshes["c0"], "c0", s)
+    try:
+        rsg_lib.rsg_set_launch(gpu_ctx, 300, 0)
+        st0 = np.zeros(5); rsg_lib.rsg_stats(gpu_ctx, A.dptr(st0))
+        fixed, a0 = P.solve()
+        st1 = np.zeros(5); rsg_lib.rsg_stats(gpu_ctx, A.dptr(st1))
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_time_slice(gpu_ctx, 2000))
+        sliced, a1 = P.solve()
+        st2 = np.zeros(5); rsg_lib.rsg_stats(gpu_ctx, A.dptr(st2))
+    finally:
+        rsg_lib.rsg_set_time_slice(gpu_ctx, 0)
+        P.close()
+    assert st1[0] - st0[0] == 4                    # 1200 iterations in launches of 300
+    assert st2[0] - st1[0] > 4                     # more, shorter launches
+    assert fixed.final_iter == sliced.final_iter == 1200
+    for k in ("U", "p", "k", "omega", "nuT"):
+        assert np.array_equal(a0[k], a1[k]), k
+    assert rsg_lib.rsg_set_time_slice(gpu_ctx, -1) != 0
+
+
 @pytest.mark.parametrize("case", ["c0", "bfs"])
 def test_warm_start_matches_oracle(rsg_lib, gpu_ctx, oracle, meshes, case):
     """rsg_init kind 1 (ForwardModel's warm start from a cached state, with its
