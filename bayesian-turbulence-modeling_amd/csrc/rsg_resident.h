@@ -492,6 +492,7 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
     }
     RSG_SYNC();
     res_restrict(t, A, G, 1, B0, cb + (A.lvRow[1] - base));
+    RSG_TMARK(19);
     // coarse levels down (row I of a level is owned by thread I % bs throughout)
     for (int l = 1; l + 1 < L; ++l) {
         const int off = A.lvRow[l] - base;
@@ -503,6 +504,7 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         res_restrict_residual(t, A, G, l, bb, x, cb + (A.lvRow[l + 1] - base));
     }
     RSG_SYNC();
+    RSG_TMARK(20);
     {   // coarsest level
         const int l = L - 1, n = A.lvN[l];
         const int off = A.lvRow[l] - base;
@@ -529,6 +531,7 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         }
         RSG_SYNC();
     }
+    RSG_TMARK(21);
     // coarse levels up
     for (int l = L - 2; l >= 1; --l) {
         const int n = A.lvN[l];
@@ -542,6 +545,7 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();
     }
+    RSG_TMARK(22);
     // level 0 up: prolongation and two sweeps
     {
         const int* map = A.map + A.lvRow[0];
@@ -558,6 +562,7 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
     res_put(t, nC, tmp, B1);
     RSG_SYNC();
     res_jacobi0(t, nC, R, rin, tmp, B1, z);
+    RSG_TMARK(23);
 }
 
 // PCG with Jacobi or V-cycle preconditioning (the recurrence of pcg() above).
@@ -625,6 +630,7 @@ template <int CPT> RSG_D SolveRes pcg_res(Ctx& C, double* xg, int maxIter, doubl
             }
         }
         bsum(t, a1);
+        RSG_TMARK(16);
         const double pAp = a1[0];
         if (fabs(pAp) < 1e-30) break;
         const double alpha = rz / pAp;
@@ -639,6 +645,7 @@ template <int CPT> RSG_D SolveRes pcg_res(Ctx& C, double* xg, int maxIter, doubl
         }
         updated = true;
         bsum(t, a2);
+        RSG_TMARK(17);
         const double rn = sqrt(a2[0]);
         res.iterations = it + 1;
         res.finalRes = rn / r0;
@@ -661,6 +668,7 @@ template <int CPT> RSG_D SolveRes pcg_res(Ctx& C, double* xg, int maxIter, doubl
             }
         }
         bsum(t, a3);
+        RSG_TMARK(18);
         const double rzNew = a3[0];
         const double beta = rzNew / (rz + 1e-30);
 #pragma unroll
