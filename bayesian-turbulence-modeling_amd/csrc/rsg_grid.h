@@ -15,6 +15,11 @@
 //   PCGSolver / AMG     :147-186, :602-653  PCG recurrence, tol relative to r0
 // The V-cycle smooths with damped Jacobi (omega 2/3, V(2,2)) and solves the
 // coarsest level (<= 64 rows) with a dense inverse, as the member V-cycle does.
+// Kernels stay one phase each, CSR, one row per thread: on a 261,600-row
+// system with 17 levels, folding the residual into the restriction and the
+// prolongation into the next sweep, ELL images, and a single-workgroup kernel
+// for the small levels were each measured slower (longer dependent-load
+// chains per thread, padded rows, one CU's latency) than these plain launches.
 //
 // The hierarchy is built on the host (the greedy aggregation is sequential);
 // every iteration runs on the device.  Included by rsg_capi.hip.
@@ -28,81 +33,63 @@ namespace grid {
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ kernels
-// A level's matrix as the kernels see it: CSR (columns ascending), plus an
-// ELL image (column-major [K][n], entries in CSR order, column -1 pads) when
-// the longest row is short.  Through the image a row's columns and values are
-// loaded without first waiting for its row pointer, which takes one
-// dependent round trip off every sweep; the sums run in the same order.
-template <typename T> struct Mat {
-    int n = 0, K = 0;
-    const int* rp = nullptr; const int* ci = nullptr; const T* v = nullptr;
-    const int* ec = nullptr; const T* ev = nullptr;
-};
-
-// sum over row i of a_ij * xval(j), in CSR order
-template <typename T, typename F>
-__device__ inline T row_sum(const Mat<T>& A, int i, F xval) {
-    T s = 0;
-    if (A.K > 0) {
-#pragma unroll 4
-        for (int k = 0; k < A.K; ++k) {
-            const int c = A.ec[(size_t)k * A.n + i];
-            const T a = A.ev[(size_t)k * A.n + i];
-            if (c >= 0) s += a * xval(c);
-        }
-    } else {
-        for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) s += A.v[e] * xval(A.ci[e]);
-    }
-    return s;
-}
-
-// s - sum over row i of a_ij * xval(j), subtracting entry by entry in CSR order
-template <typename T, typename F>
-__device__ inline T row_sub(const Mat<T>& A, int i, T s, F xval) {
-    if (A.K > 0) {
-#pragma unroll 4
-        for (int k = 0; k < A.K; ++k) {
-            const int c = A.ec[(size_t)k * A.n + i];
-            const T a = A.ev[(size_t)k * A.n + i];
-            if (c >= 0) s -= a * xval(c);
-        }
-    } else {
-        for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) s -= A.v[e] * xval(A.ci[e]);
-    }
-    return s;
-}
-
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv(Mat<T> A, const T* __restrict__ x, T* __restrict__ y) {
+__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const T* __restrict__ v, const T* __restrict__ x,
+                                                  T* __restrict__ y) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= A.n) return;
-    y[i] = row_sum(A, i, [&](int c) { return x[c]; });
+    if (i >= n) return;
+    T s = 0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+    y[i] = s;
 }
 
 // r = b - A x
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_residual(Mat<T> A, const T* __restrict__ b, const T* __restrict__ x,
-                                                      T* __restrict__ r) {
+__global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      const T* __restrict__ v, const T* __restrict__ b,
+                                                      const T* __restrict__ x, T* __restrict__ r) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= A.n) return;
-    r[i] = b[i] - row_sum(A, i, [&](int c) { return x[c]; });
+    if (i >= n) return;
+    T s = 0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+    r[i] = b[i] - s;
 }
 
 // xo = xi + w (b - A xi) / d  (xi == null: from zero)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_jacobi(Mat<T> A, const T* __restrict__ dg, const T* __restrict__ b,
-                                                    const T* xi, T* xo) {
+__global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                    const T* __restrict__ v, const T* __restrict__ dg,
+                                                    const T* __restrict__ b, const T* xi, T* xo) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= A.n) return;
+    if (i >= n) return;
     const T wJ = T(2) / T(3);
     T s = b[i];
     T x0 = 0;
     if (xi) {
-        s = row_sub(A, i, s, [&](int c) { return xi[c]; });
+        for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xi[ci[e]];
         x0 = xi[i];
     }
     const T d = dg[i];
     xo[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int* __restrict__ aP, const int* __restrict__ aI,
+                                                      const T* __restrict__ r, T* __restrict__ bc) {
+    const int I = blockIdx.x * kBlock + threadIdx.x;
+    if (I >= nc) return;
+    T s = 0;
+    for (int q = aP[I]; q < aP[I + 1]; ++q) s += r[aI[q]];
+    bc[I] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong(int n, const int* __restrict__ map, const T* __restrict__ xc,
+                                                     T* __restrict__ x) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    x[i] += xc[map[i]];
 }
 
 // x = C^{-1} b on the coarsest level (n <= 64), one block
@@ -324,60 +311,12 @@ inline bool dense_inverse(const Level& L, std::vector<double>& G) {
 }
 
 // Device image of one level in precision T.
-constexpr int kEllMaxK = 32;   // ELL image when the longest row has at most this many entries
-
 template <typename T> struct DevLevel {
-    int n = 0, nnz = 0, K = 0;
-    int *rp = nullptr, *ci = nullptr, *map = nullptr, *aP = nullptr, *aI = nullptr, *ec = nullptr;
-    T *v = nullptr, *dg = nullptr, *ev = nullptr;
+    int n = 0, nnz = 0;
+    int *rp = nullptr, *ci = nullptr, *map = nullptr, *aP = nullptr, *aI = nullptr;
+    T *v = nullptr, *dg = nullptr;
     T *x = nullptr, *b = nullptr, *r = nullptr;
-    Mat<T> mat() const {
-        Mat<T> A;
-        A.n = n; A.K = K; A.rp = rp; A.ci = ci; A.v = v; A.ec = ec; A.ev = ev;
-        return A;
-    }
 };
-
-// ------------------------------------------------------- fused kernels
-// b_c[I] = sum over the aggregate's fine rows i of (b_f[i] - A_f[i,:] x_f)
-// (a residual pass and the restriction in one launch, residual rows summed
-// in aggregate order); when x1 is given, also the coarse level's first sweep
-// from zero, x1[I] = wJ b_c[I] / d_c[I].
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict_residual(int nc, const int* __restrict__ aP,
-                                                               const int* __restrict__ aI, Mat<T> F,
-                                                               const T* __restrict__ bf, const T* __restrict__ xf,
-                                                               T* __restrict__ bc, const T* __restrict__ dgc,
-                                                               T* __restrict__ x1) {
-    const int I = blockIdx.x * kBlock + threadIdx.x;
-    if (I >= nc) return;
-    T s = 0;
-    for (int q = aP[I]; q < aP[I + 1]; ++q) {
-        const int i = aI[q];
-        s += bf[i] - row_sum(F, i, [&](int c) { return xf[c]; });
-    }
-    bc[I] = s;
-    if (x1) {
-        const T d = dgc[I];
-        x1[I] = (fabs((double)d) > 1e-30) ? T(2) / T(3) * s / d : T(0);
-    }
-}
-
-// First post-smoothing sweep with the prolongation folded in: every x value
-// is read as the prolonged x[j] + xc[map[j]]; xo = xp + wJ (b - A xp) / d.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_jacobi_prolong(Mat<T> A, const T* __restrict__ dg,
-                                                            const T* __restrict__ b, const T* __restrict__ x,
-                                                            const T* __restrict__ xc, const int* __restrict__ map,
-                                                            T* __restrict__ xo) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= A.n) return;
-    const T wJ = T(2) / T(3);
-    const T s = row_sub(A, i, b[i], [&](int c) { return x[c] + xc[map[c]]; });
-    const T x0 = x[i] + xc[map[i]];
-    const T d = dg[i];
-    xo[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
-}
 
 struct DevAlloc {
     std::vector<void*> ptrs;
@@ -424,22 +363,6 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
             d.aP = A.take<int>(L.n + 1); d.aI = A.take<int>(L.aI.size());
             put(d.aP, L.aP.data(), 4 * L.aP.size()); put(d.aI, L.aI.data(), 4 * L.aI.size());
         }
-        int K = 0;
-        for (int i = 0; i < L.n; ++i) K = std::max(K, L.rp[i + 1] - L.rp[i]);
-        if (K <= kEllMaxK) {
-            std::vector<int> ec((size_t)K * L.n, -1);
-            std::vector<double> ev((size_t)K * L.n, 0.0);
-            for (int i = 0; i < L.n; ++i)
-                for (int e = L.rp[i]; e < L.rp[i + 1]; ++e) {
-                    const size_t slot = (size_t)(e - L.rp[i]) * L.n + i;
-                    ec[slot] = L.ci[e]; ev[slot] = L.v[e];
-                }
-            d.K = K;
-            d.ec = A.take<int>(ec.size()); d.ev = A.take<T>(ev.size());
-            put(d.ec, ec.data(), 4 * ec.size());
-            putT(d.ev, ev);
-            (void)hipStreamSynchronize(s);   // ec and ev are freed at the end of this scope
-        }
     }
     D.dense = dense;
     if (dense) { D.cinv = A.take<T>(64 * 64); putT(D.cinv, G); }
@@ -449,62 +372,47 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
 
 inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
-// z = V-cycle(r) on hierarchy D (level 0 vectors: rin, z; level matrices in D):
-// one grid-wide kernel per phase.  Down: sweep from zero, sweep, then the
-// residual folded into the restriction, which also makes the next level's
-// first sweep; up: prolongation folded into the first post-sweep, sweep.
-// (Single-workgroup kernels for the small levels were measured slower: one CU
-// cannot hide the dependent-load latency of a few-thousand-row level.)
+// z = V-cycle(r) on hierarchy D (level 0 vectors: rin, z; level matrices in D)
 template <typename T>
 void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
     const int L = (int)D.lv.size();
-    const dim3 bl(kBlock);
     if (L == 1) {
         const DevLevel<T>& d = D.lv[0];
-        if (D.dense) { hipLaunchKernelGGL(k_dense<T>, dim3(1), bl, 0, s, d.n, D.cinv, rin, z); return; }
-        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), bl, 0, s, d.mat(), (const T*)d.dg, rin,
+        if (D.dense) { hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, d.n, D.cinv, rin, z); return; }
+        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), dim3(kBlock), 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin,
                            (const T*)nullptr, z);
         return;
     }
-    {
-        const DevLevel<T>& d = D.lv[0];
-        const dim3 g(blocks(d.n));
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.mat(), (const T*)d.dg, rin, (const T*)nullptr, d.r);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.mat(), (const T*)d.dg, rin, (const T*)d.r, d.x);
-    }
     for (int l = 0; l + 1 < L; ++l) {
         const DevLevel<T>& d = D.lv[l];
-        const DevLevel<T>& c = D.lv[l + 1];
         const T* b = (l == 0) ? rin : d.b;
-        const bool cSmooth = (l + 1) < L - 1;   // next level is smoothed (not the coarsest)
-        hipLaunchKernelGGL(k_restrict_residual<T>, dim3(blocks(c.n)), bl, 0, s, c.n, (const int*)c.aP,
-                           (const int*)c.aI, d.mat(), b, (const T*)d.x, c.b, (const T*)c.dg,
-                           cSmooth ? c.r : (T*)nullptr);
-        if (!cSmooth) break;
-        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(c.n)), bl, 0, s, c.mat(), (const T*)c.dg, (const T*)c.b,
-                           (const T*)c.r, c.x);
+        const dim3 g(blocks(d.n)), bl(kBlock);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)nullptr, d.r);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.r, d.x);
+        hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, b, (const T*)d.x, d.r);
+        const DevLevel<T>& c = D.lv[l + 1];
+        hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(c.n)), bl, 0, s, c.n, c.aP, c.aI, (const T*)d.r, c.b);
     }
     {
         const DevLevel<T>& c = D.lv[L - 1];
         if (D.dense) {
-            hipLaunchKernelGGL(k_dense<T>, dim3(1), bl, 0, s, c.n, D.cinv, (const T*)c.b, c.x);
+            hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, c.n, D.cinv, (const T*)c.b, c.x);
         } else {
-            const dim3 g(blocks(c.n));
-            const Mat<T> M = c.mat();
-            hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, M, (const T*)c.dg, (const T*)c.b, (const T*)nullptr, c.x);
+            const dim3 g(blocks(c.n)), bl(kBlock);
+            hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)nullptr, c.x);
             for (int q = 0; q < 25; ++q) {
-                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, M, (const T*)c.dg, (const T*)c.b, (const T*)c.x, c.r);
-                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, M, (const T*)c.dg, (const T*)c.b, (const T*)c.r, c.x);
+                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.x, c.r);
+                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.r, c.x);
             }
         }
     }
     for (int l = L - 2; l >= 0; --l) {
         const DevLevel<T>& d = D.lv[l];
         const T* b = (l == 0) ? rin : d.b;
-        const dim3 g(blocks(d.n));
-        hipLaunchKernelGGL(k_jacobi_prolong<T>, g, bl, 0, s, d.mat(), (const T*)d.dg, b, (const T*)d.x,
-                           (const T*)D.lv[l + 1].x, (const int*)d.map, d.r);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.mat(), (const T*)d.dg, b, (const T*)d.r, l == 0 ? z : d.x);
+        const dim3 g(blocks(d.n)), bl(kBlock);
+        hipLaunchKernelGGL(k_prolong<T>, g, bl, 0, s, d.n, d.map, (const T*)D.lv[l + 1].x, d.x);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.x, d.r);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.r, l == 0 ? z : d.x);
     }
 }
 
@@ -533,7 +441,7 @@ hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, 
     const int n = A0.n;
     const dim3 g(blocks(n)), bl(kBlock);
     double* dv = S.dots;          // [0] rz, [1] pAp, [2] rr, [3] rzNew
-    hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, A0.mat(), b, (const T*)x, r);
+    hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, n, A0.rp, A0.ci, A0.v, b, (const T*)x, r);
     dot2<T>(n, r, r, (const T*)nullptr, (const T*)nullptr, S, dv + 2, s);
     hipError_t e = hipMemcpyAsync(S.hdots, dv, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -546,7 +454,7 @@ hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, 
     hipLaunchKernelGGL((k_convert<T, T>), g, bl, 0, s, n, (const T*)z, p);
     dot2<T>(n, r, z, (const T*)nullptr, (const T*)nullptr, S, dv + 0, s);          // rz
     for (int it = 0; it < maxIter; ++it) {
-        hipLaunchKernelGGL(k_spmv<T>, g, bl, 0, s, A0.mat(), (const T*)p, q);
+        hipLaunchKernelGGL(k_spmv<T>, g, bl, 0, s, n, A0.rp, A0.ci, A0.v, (const T*)p, q);
         dot2<T>(n, p, q, (const T*)nullptr, (const T*)nullptr, S, dv + 1, s);      // pAp
         hipLaunchKernelGGL(k_pcg_update<T>, g, bl, 0, s, n, (const double*)dv, x, r, (const T*)p, (const T*)q);
         dot2<T>(n, r, r, (const T*)nullptr, (const T*)nullptr, S, dv + 2, s);      // rr

@@ -240,6 +240,57 @@ def measure_traffic(args):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def bench_c0(args):
+    """The single-solve latency case: the periodic Re_tau 550 channel (4 x 128,
+    geometric 1.06, default SST coefficients) solved cold to tol 1e-8 through
+    the rans_sst_py drop-in, one solve at a time."""
+    import ctypes as C
+    import rans_sst_py as rs
+    import ensembles
+    rs.engine_set_launch(args.iters_per_launch, args.block)
+    rs.engine_set_time_slice(int(args.slice_ms * 1000))
+    spec = dict(ny=128, ratio=1.06, nu=1.0 / 550.0, theta=list(ensembles.THETA_DEFAULTS))
+    meshes, fms = ensembles.build_models(rs, [spec], tol=1e-8, max_iterations=200000)
+    fm = fms[0]
+    lat, iters, conv = [], 0, 0
+    for k in range(args.warmup + args.steps):
+        fm.clear_cache()
+        t0 = time.perf_counter()
+        r = fm.evaluate(spec["theta"])
+        dt = time.perf_counter() - t0
+        print("bench: c0 solve %.2f s (%d iterations)" % (dt, r.simple_iters), file=sys.stderr, flush=True)
+        if k >= args.warmup:
+            lat.append(dt); iters = r.simple_iters
+            conv += str(r.status).split(".")[-1] == "Converged"
+    cpu = None
+    lib_ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if not args.no_cpu and os.path.exists(lib_ref):
+        L = C.CDLL(lib_ref)
+        D = lambda a: (C.c_double * len(a))(*a)
+        path = "/tmp/bench_c0.msh"
+        meshes[0].save_binary(path)
+        coeffs = [0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]
+        packed = [200000, 1e-8, 1e6, 0.7, 0.3, 0.5, 0.5, 200, 1e-3, 5, 1, 500, 0, 1.0, 0, 0,
+                  1e-10, 1e10, 1e-6, 0, 0.41, 0, 0, 2, 2]
+        info = (C.c_int * 5)()
+        t0 = time.perf_counter()
+        L.ref_simple_solve(path.encode(), 0, D([1.0, 1e-4, 1.0]), C.c_double(spec["nu"]), D(coeffs), D(packed),
+                           D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+        dt = time.perf_counter() - t0
+        cpu = {"value": (1.0 / dt) if info[1] else 0.0, "unit": "solves/s", "cores": 1, "kind": "reference",
+               "sample": "one full C0 solve by the reference solver: %d iterations, converged %d, %.1f s"
+                         % (info[0], info[1], dt)}
+    t = float(np.median(lat))
+    print(json.dumps({
+        "metric": METRIC, "value": conv / sum(lat), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C0 periodic channel 4x128, geometric 1.06, Re_tau 550, SIMPLE+SST to tol 1e-8, "
+                               "one solve at a time", "cells": 512, "tol": 1e-8},
+        "p50_latency_s": t, "outer_iterations": int(iters), "converged": conv,
+        "roofline": None, "cpu_baseline": cpu}), flush=True)
+
+
 def bench_c1(args):
     """The large single system: solves/s of AMG-PCG to 1e-10 (host hierarchy
     setup + device solve + host<->device copies, wall clock), FP64 and mixed."""
@@ -328,8 +379,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
     ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=("c2", "c1"), default="c2",
-                    help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system")
+    ap.add_argument("--workload", choices=("c2", "c1", "c0"), default="c2",
+                    help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
+                         "c0: single-solve latency of the Re_tau 550 channel")
     ap.add_argument("--max-iterations", type=int, default=200000,
                     help="outer-iteration cap per member (profiling runs shorten it)")
     args = ap.parse_args()
@@ -342,6 +394,9 @@ def main():
         sys.exit(2)
     if args.workload == "c1":
         bench_c1(args)
+        return
+    if args.workload == "c0":
+        bench_c0(args)
         return
     import rans_sst_py as rs
     import ensembles
