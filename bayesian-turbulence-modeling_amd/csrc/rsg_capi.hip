@@ -50,12 +50,12 @@ struct LdsArea {
     double xb[2 * RSG_RES_MAXN];
     double cv[RSG_RES_MAXC];
 };
-// One member per CU: room for LDS copies of the coarse AMG levels (~141 KB).
+// One member per CU: room for LDS copies of the coarse AMG levels (~150 KB).
 struct LdsAreaWide {
     LdsArea a;
     double ceVal[RSG_RES_CE];
-    int ceCol[RSG_RES_CE];
-    int caCol[RSG_RES_CA];
+    short ceCol[RSG_RES_CE];
+    short caCol[RSG_RES_CA];
 };
 
 __device__ rsg::Thr make_thr(LdsArea& L, int bs) {

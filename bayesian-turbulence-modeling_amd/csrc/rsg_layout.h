@@ -25,8 +25,8 @@
 #define RSG_RES_MAXC 1536
 #endif
 // LDS copies of the coarse-level ELL images in the one-member-per-CU kernel
-#define RSG_RES_CE 6144       // matrix entries (value + column)
-#define RSG_RES_CA 2048       // aggregate entries
+#define RSG_RES_CE 7680       // matrix entries (FP64 value + 16-bit column)
+#define RSG_RES_CA 3072       // aggregate entries (16-bit)
 
 // boundary-face flag bits (from the face's patch type)
 #define RSG_BF_WALL 1

@@ -37,11 +37,13 @@
 #include <math.h>
 #define RSG_D static inline
 #define RSG_HD static inline
+#define RSG_DM inline
 #define RSG_SYNC() ((void)0)
 #else
 #include <hip/hip_runtime.h>
 #define RSG_D __device__ static __attribute__((always_inline)) inline
 #define RSG_HD __host__ __device__ static inline
+#define RSG_DM __device__ __attribute__((always_inline)) inline   // member functions
 #define RSG_SYNC() __syncthreads()
 #endif
 
@@ -81,8 +83,8 @@ struct Thr {
     double* cvec;        // LDS coarse-level x, cvecCap rows
     int cvecCap;
     // LDS copies of the coarse ELL images (null: read them from global memory)
-    double* ceVal; int* ceCol; int ceCap;
-    int* caCol; int caCap;
+    double* ceVal; short* ceCol; int ceCap;
+    short* caCol; int caCap;
 };
 
 #ifdef RSG_EMULATE

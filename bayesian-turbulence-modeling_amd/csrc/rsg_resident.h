@@ -238,20 +238,36 @@ RSG_D void res_jacobiL(const Thr& t, int n, const int* rp, const int* ci, const 
 #ifdef RSG_EMULATE
 static int rsg_emu_staged_first = 0;
 #endif
+// LDS copies hold column and row indices as 16-bit integers (indices within a
+// level are below RSG_RES_MAXN, and the -1 pad stays -1), so the deepest
+// levels of the largest resident members fit entirely.
+static_assert(RSG_RES_MAXN < 32768 && RSG_RES_MAXC < 32768, "16-bit LDS image indices");
 struct CoarseImg {
-    const int* gCol; const double* gVal; const int* gAgg;   // global images
-    const int* sCol; const double* sVal; const int* sAgg;   // LDS copies
-    int eFirst, eBase, aFirst, aBase;                        // first staged level, its offset
+    const int* gCol; const double* gVal; const int* gAgg;       // global images
+    const short* sCol; const double* sVal; const short* sAgg;   // LDS copies
+    int eFirst, eBase, aFirst, aBase;                            // first staged level, its offset
 };
 
-RSG_D const int* img_col(const CoarseImg& G, const RsgAmgDev& A, int l) {
-    return l >= G.eFirst ? G.sCol + (A.lvE[l] - G.eBase) : G.gCol + A.lvE[l];
+// An index array of one level, in global memory (int) or LDS (short).
+struct IdxRef {
+    const int* g; const short* s;
+    RSG_DM int operator[](int i) const { return s ? (int)s[i] : g[i]; }
+};
+
+RSG_D IdxRef img_col(const CoarseImg& G, const RsgAmgDev& A, int l) {
+    IdxRef r;
+    if (l >= G.eFirst) { r.g = nullptr; r.s = G.sCol + (A.lvE[l] - G.eBase); }
+    else { r.g = G.gCol + A.lvE[l]; r.s = nullptr; }
+    return r;
 }
 RSG_D const double* img_val(const CoarseImg& G, const RsgAmgDev& A, int l) {
     return l >= G.eFirst ? G.sVal + (A.lvE[l] - G.eBase) : G.gVal + A.lvE[l];
 }
-RSG_D const int* img_agg(const CoarseImg& G, const RsgAmgDev& A, int l) {
-    return l >= G.aFirst ? G.sAgg + (A.lvAE[l] - G.aBase) : G.gAgg + A.lvAE[l];
+RSG_D IdxRef img_agg(const CoarseImg& G, const RsgAmgDev& A, int l) {
+    IdxRef r;
+    if (l >= G.aFirst) { r.g = nullptr; r.s = G.sAgg + (A.lvAE[l] - G.aBase); }
+    else { r.g = G.gAgg + A.lvAE[l]; r.s = nullptr; }
+    return r;
 }
 
 RSG_D CoarseImg coarse_images(Thr& t, const RsgAmgDev& A) {
@@ -271,12 +287,12 @@ RSG_D CoarseImg coarse_images(Thr& t, const RsgAmgDev& A) {
         g.eBase = A.lvE[g.eFirst];
         for (int i = t.tid; i < A.used[0] - g.eBase; i += t.bs) {
             t.ceVal[i] = A.eVal[g.eBase + i];
-            t.ceCol[i] = A.eCol[g.eBase + i];
+            t.ceCol[i] = (short)A.eCol[g.eBase + i];
         }
     }
     if (g.aFirst <= RSG_MAX_LEVELS) {
         g.aBase = A.lvAE[g.aFirst];
-        for (int i = t.tid; i < A.used[1] - g.aBase; i += t.bs) t.caCol[i] = A.aCol[g.aBase + i];
+        for (int i = t.tid; i < A.used[1] - g.aBase; i += t.bs) t.caCol[i] = (short)A.aCol[g.aBase + i];
     }
     RSG_SYNC();
     return g;
@@ -284,7 +300,7 @@ RSG_D CoarseImg coarse_images(Thr& t, const RsgAmgDev& A) {
 
 // The same sweep on a level's ELL image: entries in CSR order, so the sums
 // round exactly like res_jacobiL's; column -1 pads.
-RSG_D void res_jacobiE(const Thr& t, int n, int K, const int* ec, const double* ev, const double* dg,
+RSG_D void res_jacobiE(const Thr& t, int n, int K, IdxRef ec, const double* ev, const double* dg,
                        const double* b, const double* xin, double* xout, bool xzero) {
     const double wJ = 2.0 / 3.0;
     for (int i = t.tid; i < n; i += t.bs) {
@@ -320,7 +336,7 @@ RSG_D void res_restrict(const Thr& t, const RsgAmgDev& A, const CoarseImg& G, in
     const int nc = A.lvN[lc];
     if (A.lvAK[lc] > 0) {
         const int AK = A.lvAK[lc];
-        const int* ac = img_agg(G, A, lc);
+        const IdxRef ac = img_agg(G, A, lc);
         for (int I = t.tid; I < nc; I += t.bs) {
             double s = 0.0;
 #pragma unroll 4
@@ -348,7 +364,7 @@ RSG_D double res_row_residual(const RsgAmgDev& A, const CoarseImg& G, int l, int
     double s = 0.0;
     if (A.lvK[l] > 0) {
         const int n = A.lvN[l], K = A.lvK[l];
-        const int* ec = img_col(G, A, l);
+        const IdxRef ec = img_col(G, A, l);
         const double* ev = img_val(G, A, l);
 #pragma unroll 4
         for (int k = 0; k < K; ++k) {
@@ -371,7 +387,7 @@ RSG_D void res_restrict_residual(const Thr& t, const RsgAmgDev& A, const CoarseI
     const int lc = l + 1, nc = A.lvN[lc];
     if (A.lvAK[lc] > 0) {
         const int AK = A.lvAK[lc];
-        const int* ac = img_agg(G, A, lc);
+        const IdxRef ac = img_agg(G, A, lc);
         for (int I = t.tid; I < nc; I += t.bs) {
             double s = 0.0;
             for (int q = 0; q < AK; ++q) {
@@ -402,7 +418,7 @@ RSG_D void res_sweep_prolong(const Thr& t, const RsgAmgDev& A, const CoarseImg& 
     const double* dg = A.dg + A.lvRow[l];
     if (A.lvK[l] > 0) {
         const int K = A.lvK[l];
-        const int* ec = img_col(G, A, l);
+        const IdxRef ec = img_col(G, A, l);
         const double* ev = img_val(G, A, l);
         for (int i = t.tid; i < n; i += t.bs) {
             double s = b[i];
@@ -499,9 +515,12 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         double* x = cx + off; double* bb = cb + off; double* r = cr + off;
         res_sweepL(t, A, G, l, bb, x, r, true);
         RSG_SYNC();
+        RSG_TMARK(l == 1 ? 24 : 29);
         res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();
+        RSG_TMARK(l == 1 ? 25 : 30);
         res_restrict_residual(t, A, G, l, bb, x, cb + (A.lvRow[l + 1] - base));
+        RSG_TMARK(l == 1 ? 26 : 31);
     }
     RSG_SYNC();
     RSG_TMARK(20);
@@ -542,8 +561,10 @@ template <int CPT> RSG_D void vcycle_res(Ctx& C, const ResRows<CPT>& R, const Co
         (void)n;
         res_sweep_prolong(t, A, G, l, bb, x, xc, map, r);
         RSG_SYNC();
+        RSG_TMARK(l == 1 ? 27 : 14);
         res_sweepL(t, A, G, l, bb, r, x, false);
         RSG_SYNC();
+        RSG_TMARK(l == 1 ? 28 : 15);
     }
     RSG_TMARK(22);
     // level 0 up: prolongation and two sweeps

@@ -262,6 +262,19 @@ def bench_c0(args):
         if k >= args.warmup:
             lat.append(dt); iters = r.simple_iters
             conv += str(r.status).split(".")[-1] == "Converged"
+    batch = None
+    if args.members > 1:
+        # the same mesh batched: `members` solves with the ensemble's theta
+        # spread (five coefficients +-20 %), all submitted at once
+        specs = ensembles.channel_ensemble(args.members, args.seed)
+        bspecs = [dict(spec, theta=sp["theta"]) for sp in specs]
+        (_, bfms), res, wall = run_batch(rs, bspecs, 1e-8, ensembles.build_models(rs, bspecs, tol=1e-8))
+        bconv = sum(str(r.status).split(".")[-1] == "Converged" for r in res)
+        batch = {"members": args.members, "wall_s": wall, "converged": int(bconv),
+                 "solves_per_s": bconv / wall,
+                 "outer_iters_median": float(np.median([r.simple_iters for r in res])),
+                 "p50_latency_s": float(np.percentile([r.latency for r in res], 50)),
+                 "p99_latency_s": float(np.percentile([r.latency for r in res], 99))}
     cpu = None
     lib_ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
     if not args.no_cpu and os.path.exists(lib_ref):
@@ -287,7 +300,7 @@ def bench_c0(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "C0 periodic channel 4x128, geometric 1.06, Re_tau 550, SIMPLE+SST to tol 1e-8, "
                                "one solve at a time", "cells": 512, "tol": 1e-8},
-        "p50_latency_s": t, "outer_iterations": int(iters), "converged": conv,
+        "p50_latency_s": t, "outer_iterations": int(iters), "converged": conv, "batched_same_mesh": batch,
         "roofline": None, "cpu_baseline": cpu}), flush=True)
 
 

@@ -30,7 +30,7 @@ static rsg::Thr emu_thr() {
     t.xb0 = g_resident ? xb : nullptr; t.xb1 = xb + RSG_RES_MAXN; t.xsel = 0;
     t.cvec = cv; t.cvecCap = RSG_RES_MAXC;
     static double ceVal[RSG_RES_CE];
-    static int ceCol[RSG_RES_CE], caCol[RSG_RES_CA];
+    static short ceCol[RSG_RES_CE], caCol[RSG_RES_CA];
     t.ceVal = g_stage ? ceVal : nullptr; t.ceCol = ceCol; t.ceCap = RSG_RES_CE;
     t.caCol = caCol; t.caCap = RSG_RES_CA;
     return t;
