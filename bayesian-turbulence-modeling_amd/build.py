@@ -45,7 +45,8 @@ def build_module():
     suffix = sysconfig.get_config_var("EXT_SUFFIX")
     out = os.path.join(HERE, "rans_sst_py" + suffix)
     host = os.path.join(HERE, "host")
-    srcs = [os.path.join(host, f) for f in ("bindings.cpp", "forward_model.cpp", "model.cpp", "mesh.cpp")]
+    srcs = [os.path.join(host, f) for f in ("bindings.cpp", "forward_model.cpp", "sensitivity.cpp", "model.cpp",
+                                            "mesh.cpp")]
     py_inc = sysconfig.get_paths()["include"]
     cmd = ["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-shared", "-fPIC",
            "-I", pybind11.get_include(), "-I", py_inc, "-I", INCLUDE] + srcs + \

@@ -9,6 +9,7 @@
 
 #include "forward_model.hpp"
 #include "mesh.hpp"
+#include "sensitivity.hpp"
 #include "model.hpp"
 
 namespace py = pybind11;
@@ -298,6 +299,54 @@ PYBIND11_MODULE(rans_sst_py, m) {
         })
         .def("cache_size", [](ForwardModel& fm) { return fm.cache().size(); })
         .def("clear_cache", [](ForwardModel& fm) { fm.cache().clear(); });
+
+    // TangentGradientResult / ParameterSensitivity (python/Bindings.cpp:521-576): the
+    // forward-path members -- solve_state, observe, log_lik and the warm-FD gradient.
+    py::class_<TangentGradientResult>(m, "TangentGradientResult")
+        .def_readonly("d_obs_d_theta", &TangentGradientResult::dObsDTheta)
+        .def_readonly("log_lik_gradient", &TangentGradientResult::logLikGradient)
+        .def_readonly("krylov_iters", &TangentGradientResult::krylovIters)
+        .def_readonly("krylov_rel_res", &TangentGradientResult::krylovRelRes)
+        .def_readonly("krylov_converged", &TangentGradientResult::krylovConverged)
+        .def_readonly("n_residual_evals", &TangentGradientResult::nResidualEvals)
+        .def_readonly("n_colors", &TangentGradientResult::nColors);
+
+    py::class_<ParameterSensitivity>(m, "ParameterSensitivity")
+        .def(py::init<const Mesh&, const ObservationOperator&, const FlowBoundaryConditions&, double,
+                      const SolverSettings&, const Vec3&, double, double, double>(),
+             py::arg("mesh"), py::arg("obs_op"), py::arg("bcs"), py::arg("nu"),
+             py::arg("settings") = SolverSettings{}, py::arg("u_init") = Vec3{1, 0, 0}, py::arg("p_init") = 0.0,
+             py::arg("k_init") = 1e-4, py::arg("omega_init") = 1.0, py::keep_alive<1, 2>())
+        .def("solve_state", &ParameterSensitivity::solveState, py::arg("theta"),
+             py::call_guard<py::gil_scoped_release>())
+        .def("has_state", &ParameterSensitivity::hasState)
+        .def("n_state", &ParameterSensitivity::nState)
+        .def("n_cells", &ParameterSensitivity::nCells)
+        .def("n_obs", &ParameterSensitivity::nObs)
+        .def("observe", &ParameterSensitivity::observe, py::arg("theta"))
+        .def("log_lik", &ParameterSensitivity::logLik, py::arg("theta"))
+        .def("eta_jacobian_warm_fd", &ParameterSensitivity::etaJacobianWarmFD, py::arg("theta"),
+             py::arg("h_rel") = 5e-4, py::arg("h_floor") = 1e-7, py::arg("warm_max_iter") = 0,
+             py::arg("warm_tol") = 0.0, py::call_guard<py::gil_scoped_release>())
+        .def("state_fields", [](const ParameterSensitivity& ps) {
+            py::dict d;
+            const FlowFields& f = ps.state();
+            d["U"] = py::array_t<double>(f.U.size(), f.U.data());
+            d["p"] = py::array_t<double>(f.p.size(), f.p.data());
+            d["k"] = py::array_t<double>(f.k.size(), f.k.data());
+            d["omega"] = py::array_t<double>(f.omega.size(), f.omega.data());
+            d["nuT"] = py::array_t<double>(f.nuT.size(), f.nuT.data());
+            d["F1"] = py::array_t<double>(f.F1.size(), f.F1.data());
+            d["F2"] = py::array_t<double>(f.F2.size(), f.F2.data());
+            d["Pk"] = py::array_t<double>(f.Pk.size(), f.Pk.data());
+            d["CDkw"] = py::array_t<double>(f.CDkw.size(), f.CDkw.data());
+            d["U_bnd"] = py::array_t<double>(f.Ub.size(), f.Ub.data());
+            d["p_bnd"] = py::array_t<double>(f.pb.size(), f.pb.data());
+            d["k_bnd"] = py::array_t<double>(f.kb.size(), f.kb.data());
+            d["omega_bnd"] = py::array_t<double>(f.omb.size(), f.omb.data());
+            d["turb_established"] = f.turbEstablished;
+            return d;
+        }, "The stored fixed-point state (host copies).");
 
     m.def("odd_even_energy_ratio",
           [](const Mesh& mesh, const std::vector<double>& values) {
