@@ -7,10 +7,13 @@
 #include "mesh.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cstdint>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <limits>
+#include <sstream>
 #include <stdexcept>
 
 namespace rsh {
@@ -558,10 +561,226 @@ void Mesh::saveBinary(const std::string& path) const {
     if (!o) throw std::runtime_error("write failed: " + path);
 }
 
+// ------------------------------------------------------------ OpenFOAM polyMesh
+namespace {
+
+// An ASCII OpenFOAM file as tokens: comments removed, the FoamFile header
+// dictionary skipped, '(' and ')' as tokens of their own.
+struct FoamTokens {
+    std::vector<std::string> t;
+    size_t i = 0;
+    explicit FoamTokens(const std::string& path) {
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error("Cannot open " + path);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        const std::string s = ss.str();
+        std::string cur;
+        auto flush = [&]() { if (!cur.empty()) { t.push_back(cur); cur.clear(); } };
+        for (size_t k = 0; k < s.size(); ++k) {
+            const char c = s[k];
+            if (c == '/' && k + 1 < s.size() && s[k + 1] == '/') {
+                flush();
+                while (k < s.size() && s[k] != '\n') ++k;
+            } else if (c == '/' && k + 1 < s.size() && s[k + 1] == '*') {
+                flush();
+                k += 2;
+                while (k + 1 < s.size() && !(s[k] == '*' && s[k + 1] == '/')) ++k;
+                ++k;
+            } else if (c == '(' || c == ')' || c == '{' || c == '}' || c == ';') {
+                flush();
+                t.push_back(std::string(1, c));
+            } else if (std::isspace(static_cast<unsigned char>(c))) {
+                flush();
+            } else {
+                cur.push_back(c);
+            }
+        }
+        flush();
+        // skip "FoamFile { ... }"
+        if (i < t.size() && t[i] == "FoamFile") {
+            while (i < t.size() && t[i] != "}") ++i;
+            ++i;
+        }
+    }
+    bool done() const { return i >= t.size(); }
+    const std::string& next() {
+        if (i >= t.size()) throw std::runtime_error("unexpected end of OpenFOAM file");
+        return t[i++];
+    }
+    void expect(const char* s) {
+        if (next() != s) throw std::runtime_error(std::string("malformed OpenFOAM file: expected ") + s);
+    }
+    int nextInt() { return std::stoi(next()); }
+    double nextDouble() { return std::stod(next()); }
+    // the list size that opens the data block
+    int count() {
+        while (!done()) {
+            const std::string& s = t[i];
+            if (!s.empty() && std::isdigit(static_cast<unsigned char>(s[0]))) return nextInt();
+            ++i;
+        }
+        throw std::runtime_error("Could not read count");
+    }
+};
+
+}  // namespace
+
+// Mesh::loadOpenFOAM (Mesh.cpp:88-265): points, faces, owner, neighbour and
+// boundary of an ASCII polyMesh; the cell count from the largest cell index;
+// each cell's faces in face order (owner side, then neighbour side); face
+// centre = node average, area vector = sum of the node-fan triangles about
+// that centre; then cell centres and volumes, face deltas and weights.
+Mesh Mesh::loadOpenFOAM(const std::string& polyMeshDir) {
+    namespace fs = std::filesystem;
+    const fs::path dir(polyMeshDir);
+    Mesh m;
+    {
+        FoamTokens T((dir / "points").string());
+        const int n = T.count();
+        T.expect("(");
+        m.nodes_.resize(n);
+        for (int k = 0; k < n; ++k) {
+            T.expect("(");
+            const double x = T.nextDouble(), y = T.nextDouble(), z = T.nextDouble();
+            m.nodes_[k] = Vec3(x, y, z);
+            T.expect(")");
+        }
+    }
+    std::vector<std::vector<int>> faceNodes;
+    {
+        FoamTokens T((dir / "faces").string());
+        const int n = T.count();
+        T.expect("(");
+        faceNodes.resize(n);
+        for (int k = 0; k < n; ++k) {
+            const int nv = T.nextInt();
+            T.expect("(");
+            faceNodes[k].resize(nv);
+            for (int& v : faceNodes[k]) {
+                v = T.nextInt();
+                if (v < 0 || v >= (int)m.nodes_.size()) throw std::runtime_error("face node index out of range");
+            }
+            T.expect(")");
+        }
+        m.faces_.resize(n);
+    }
+    {
+        FoamTokens T((dir / "owner").string());
+        const int n = T.count();
+        if (n > (int)m.faces_.size()) throw std::runtime_error("owner list longer than the face list");
+        T.expect("(");
+        for (int k = 0; k < n; ++k) m.faces_[k].owner = T.nextInt();
+    }
+    {
+        FoamTokens T((dir / "neighbour").string());
+        const int n = T.count();
+        if (n > (int)m.faces_.size()) throw std::runtime_error("neighbour list longer than the face list");
+        m.nInternal_ = n;
+        T.expect("(");
+        for (int k = 0; k < n; ++k) m.faces_[k].neighbor = T.nextInt();
+    }
+    int nCells = 0;
+    for (const Face& f : m.faces_) {
+        if (f.owner < 0) throw std::runtime_error("face without an owner cell");
+        nCells = std::max(nCells, f.owner + 1);
+        if (f.neighbor >= 0) nCells = std::max(nCells, f.neighbor + 1);
+    }
+    m.cells_.resize(nCells);
+    for (int fi = 0; fi < (int)m.faces_.size(); ++fi) {
+        m.cells_[m.faces_[fi].owner].faces.push_back(fi);
+        if (m.faces_[fi].neighbor >= 0) m.cells_[m.faces_[fi].neighbor].faces.push_back(fi);
+    }
+    if (fs::exists(dir / "boundary")) {
+        FoamTokens T((dir / "boundary").string());
+        const int np = T.count();
+        T.expect("(");
+        m.patches_.resize(np);
+        for (int p = 0; p < np; ++p) {
+            Patch& P = m.patches_[p];
+            P.name = T.next();
+            T.expect("{");
+            int nF = 0, start = 0;
+            int depth = 1;
+            while (depth > 0) {
+                const std::string key = T.next();
+                if (key == "{") { ++depth; continue; }
+                if (key == "}") { --depth; continue; }
+                if (key == ";" || depth > 1) continue;
+                if (key == "type") P.type = T.next();
+                else if (key == "nFaces") nF = T.nextInt();
+                else if (key == "startFace") start = T.nextInt();
+            }
+            if (nF < 0 || start < 0 || start + nF > (int)m.faces_.size())
+                throw std::runtime_error("patch face range out of bounds: " + P.name);
+            P.faces.resize(nF);
+            for (int k = 0; k < nF; ++k) {
+                P.faces[k] = start + k;
+                m.faces_[start + k].patchID = p;
+            }
+        }
+    }
+    for (int fi = 0; fi < (int)m.faces_.size(); ++fi) {
+        const std::vector<int>& fn = faceNodes[fi];
+        const int nv = (int)fn.size();
+        Vec3 ctr;
+        for (int j = 0; j < nv; ++j) ctr = ctr + m.nodes_[fn[j]];
+        ctr = ctr / static_cast<double>(nv);
+        Vec3 sa;
+        for (int j = 0; j < nv; ++j) {
+            const Vec3 a = m.nodes_[fn[j]] - ctr, b = m.nodes_[fn[(j + 1) % nv]] - ctr;
+            const Vec3 c(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+            sa = sa + c * 0.5;
+        }
+        Face& f = m.faces_[fi];
+        f.center = ctr;
+        f.area = sa.norm();
+        f.normal = (f.area > 1e-300) ? sa / f.area : Vec3();
+    }
+    m.computeCellCentersAndVolumes();
+    m.computeFaceGeometry();
+    m.computeInterpolationWeights();
+    return m;
+}
+
+// Mesh.cpp:268-297: centre = mean of the face centres; volume from the
+// divergence theorem about that centre, |sum_f ±S_f . (c_f - c_C)| / 3.
+void Mesh::computeCellCentersAndVolumes() {
+    for (Cell& c : cells_) {
+        Vec3 ctr;
+        for (int fi : c.faces) ctr = ctr + faces_[fi].center;
+        c.center = ctr / static_cast<double>(c.faces.size());
+    }
+    for (int ci = 0; ci < (int)cells_.size(); ++ci) {
+        Cell& c = cells_[ci];
+        double vol = 0.0;
+        for (int fi : c.faces) {
+            const Face& f = faces_[fi];
+            const Vec3 Sf = f.normal * f.area;
+            const double sign = (f.owner == ci) ? 1.0 : -1.0;
+            vol += sign * Sf.dot(f.center - c.center);
+        }
+        c.volume = std::abs(vol) / 3.0;
+        if (c.volume < 1e-30) c.volume = 1e-20;
+    }
+}
+
+// Mesh::loadFromFile (Mesh.cpp:21-47): a directory holding points and faces,
+// or a case directory with constant/polyMesh, is read as OpenFOAM; a file
+// ending in .msh as MSH1; a .foam file or a path without extension as a
+// polyMesh directory.
 Mesh Mesh::loadFromFile(const std::string& path) {
-    const std::string ext = path.size() >= 4 ? path.substr(path.size() - 4) : "";
-    if (ext != ".msh")
-        throw std::runtime_error("Unknown mesh format (only MSH1 '.msh' files are supported): " + path);
+    namespace fs = std::filesystem;
+    if (fs::is_directory(path)) {
+        const fs::path p(path);
+        if (fs::exists(p / "points") && fs::exists(p / "faces")) return loadOpenFOAM(path);
+        const fs::path poly = p / "constant" / "polyMesh";
+        if (fs::exists(poly / "points")) return loadOpenFOAM(poly.string());
+        throw std::runtime_error("Cannot find polyMesh in: " + path);
+    }
+    const std::string ext = fs::path(path).extension().string();
+    if (ext == ".foam" || ext.empty()) return loadOpenFOAM(path);
+    if (ext != ".msh") throw std::runtime_error("Unknown mesh format: " + ext);
     std::ifstream in(path, std::ios::binary);
     if (!in) throw std::runtime_error("Cannot open binary mesh: " + path);
     auto ri = [&]() { int v = 0; in.read(reinterpret_cast<char*>(&v), 4); return v; };

@@ -71,7 +71,8 @@ public:
     // streamwise faces dropped (they carry zero net flux in x-invariant flow).
     static Mesh makeChannelColumn1D(const std::vector<double>& yNodes, double dx);
 
-    static Mesh loadFromFile(const std::string& path);
+    static Mesh loadFromFile(const std::string& path);      // MSH1 file or OpenFOAM polyMesh
+    static Mesh loadOpenFOAM(const std::string& polyMeshDir);
     void saveBinary(const std::string& path) const;
 
     void computeWallDistance();
@@ -114,6 +115,7 @@ private:
 
     void computeFaceGeometry();
     void computeInterpolationWeights();
+    void computeCellCentersAndVolumes();
     void bump();
     static Mesh periodicFromNodes(const std::vector<double>& xNodes,
                                   const std::vector<std::vector<double>>& Y);   // Y[i][j]

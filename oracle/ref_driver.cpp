@@ -375,6 +375,16 @@ int ref_residual(const char* path, int bcKind, const double* bcVals, double nu,
 // 5 makeCurvedChannelPeriodic2D(x[nx+1], yb[nx+1], yTop, ny, Re, y+).
 // ip / dp carry the integer / real arguments in signature order (kind 5:
 // ip = {nx, ny}, dp = {yTop, Re, y+, x..., yb...}).
+// Mesh::loadFromFile (MSH1 or an OpenFOAM polyMesh directory) written back
+// as MSH1: the reference reader's topology and geometry, byte for byte.
+int ref_convert_mesh(const char* in, const char* out) {
+    try {
+        Mesh m = Mesh::loadFromFile(in);
+        m.saveBinary(out);
+        return 0;
+    } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
+}
+
 int ref_make_mesh(int kind, const int* ip, const double* dp, const char* path) {
     try {
         Mesh m;
