@@ -270,7 +270,7 @@ struct rsg_context {
     int nextMesh = 1, nextProblem = 1;
     int64_t nextTicket = 1;
     int itersPerLaunch = 200;
-    int blockSize = 0;       // 0: chosen per launch from the number of active members
+    int blockSize = 0;       // 0: automatic (512)
     int sliceUs = 0;         // > 0: each launch runs every member for this long (no iteration cap)
     int clockKHz = 0;        // device wall clock (wall_clock64) frequency
     int maxSlots = 4096;
@@ -453,11 +453,12 @@ void engine_loop(rsg_context* ctx) {
             }
             ctx->hMembers[i] = T.M;
         }
-        // Auto: while more members are active than there are CUs, two
-        // 256-thread member workgroups share each CU; below that (the tail of
-        // an ensemble, a lightly loaded server) each member gets a 512-thread
-        // workgroup of its own, which lowers its iteration latency.
-        const int bs = ctx->blockSize ? ctx->blockSize : (na > ctx->cuCount ? 256 : 512);
+        // Automatic: 512-thread member workgroups, one per CU (more active
+        // members than CUs run in successive rounds within the launch).  Two
+        // 256-thread workgroups per CU were measured to deliver fewer member
+        // iterations per second than this at every ensemble size (512
+        // members of 1536 cells: 171k against 256k per second).
+        const int bs = ctx->blockSize ? ctx->blockSize : 512;
         std::vector<int64_t> batch = ctx->active;
         lk.unlock();
         hipError_t e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * na,
