@@ -273,6 +273,8 @@ struct rsg_context {
     int blockSize = 0;       // 0: automatic (512)
     int sliceUs = 0;         // > 0: each launch runs every member for this long (no iteration cap)
     int clockKHz = 0;        // device wall clock (wall_clock64) frequency
+    int resident256 = 0, resident512 = 0;   // member workgroups the device holds at once, per shape
+    size_t rot = 0;          // first active member of the next launch (rotating window)
     int maxSlots = 4096;
     std::thread engine;
     bool stop = false;
@@ -440,10 +442,25 @@ void engine_loop(rsg_context* ctx) {
             hipMalloc(&ctx->dStatus, sizeof(RsgScalars) * cap);
             ctx->dCap = cap;
         }
-        ctx->hMembers.resize(na);
-        ctx->hStatus.resize(na);
-        for (int i = 0; i < na; ++i) {
-            Ticket& T = *ctx->tickets[ctx->active[i]];
+        // Automatic: 512-thread member workgroups, one per CU.  Two 256-thread
+        // workgroups per CU were measured to deliver fewer member iterations
+        // per second at every ensemble size (512 members of 1536 cells: 171k
+        // against 256k per second).
+        const int bs = ctx->blockSize ? ctx->blockSize : 512;
+        // One launch holds at most as many members as the device runs at
+        // once: a launch of more would run them in rounds, and its last
+        // round would leave most CUs idle.  The window rotates over the
+        // active members so that every one of them keeps advancing.
+        const int resident = bs == 256 ? ctx->resident256 : ctx->resident512;
+        const int nl = resident > 0 ? std::min(na, resident) : na;
+        std::vector<int64_t> batch(nl);
+        const size_t r0 = ctx->rot % (size_t)na;
+        for (int i = 0; i < nl; ++i) batch[i] = ctx->active[(r0 + i) % na];
+        ctx->rot = r0 + nl;
+        ctx->hMembers.resize(nl);
+        ctx->hStatus.resize(nl);
+        for (int i = 0; i < nl; ++i) {
+            Ticket& T = *ctx->tickets[batch[i]];
             if (ctx->sliceUs > 0 && ctx->clockKHz > 0) {
                 T.M.iterBudget = 1 << 30;
                 T.M.sliceTicks = (long long)ctx->sliceUs * ctx->clockKHz / 1000;
@@ -453,28 +470,21 @@ void engine_loop(rsg_context* ctx) {
             }
             ctx->hMembers[i] = T.M;
         }
-        // Automatic: 512-thread member workgroups, one per CU (more active
-        // members than CUs run in successive rounds within the launch).  Two
-        // 256-thread workgroups per CU were measured to deliver fewer member
-        // iterations per second than this at every ensemble size (512
-        // members of 1536 cells: 171k against 256k per second).
-        const int bs = ctx->blockSize ? ctx->blockSize : 512;
-        std::vector<int64_t> batch = ctx->active;
         lk.unlock();
-        hipError_t e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * na,
+        hipError_t e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * nl,
                                       hipMemcpyHostToDevice, ctx->stream);
         hipEventRecord(ctx->ev0, ctx->stream);
-        hipLaunchKernelGGL(chunk_kernel(bs), dim3(na), dim3(bs), 0, ctx->stream, ctx->dMembers, ctx->dStatus);
+        hipLaunchKernelGGL(chunk_kernel(bs), dim3(nl), dim3(bs), 0, ctx->stream, ctx->dMembers, ctx->dStatus);
         hipEventRecord(ctx->ev1, ctx->stream);
         if (e == hipSuccess) e = hipGetLastError();
         if (e == hipSuccess)
-            e = hipMemcpyAsync(ctx->hStatus.data(), ctx->dStatus, sizeof(RsgScalars) * na, hipMemcpyDeviceToHost, ctx->stream);
+            e = hipMemcpyAsync(ctx->hStatus.data(), ctx->dStatus, sizeof(RsgScalars) * nl, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         float ms = 0.f;
         if (e == hipSuccess) hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
         // retire finished members
         std::vector<int64_t> done;
-        for (int i = 0; i < na && e == hipSuccess; ++i) {
+        for (int i = 0; i < nl && e == hipSuccess; ++i) {
             Ticket& T = *ctx->tickets[batch[i]];
             T.sc = ctx->hStatus[i];
             if (T.sc.status != RSGM_RUNNING) {
@@ -528,6 +538,14 @@ int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** 
     c->cuCount = prop.multiProcessorCount;
     c->warpSize = prop.warpSize;
     if (hipDeviceGetAttribute(&c->clockKHz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->clockKHz = 0;
+    {
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chunk_kernel(256), 256, 0) == hipSuccess && per > 0)
+            c->resident256 = per * c->cuCount;
+        per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chunk_kernel(512), 512, 0) == hipSuccess && per > 0)
+            c->resident512 = per * c->cuCount;
+    }
     if (hbm_budget_bytes == 0) hbm_budget_bytes = 8ull << 30;
     if (hipMalloc(&c->arena.base, hbm_budget_bytes) != hipSuccess) return RSG_ERR_NOMEM;
     c->arena.size = hbm_budget_bytes;
