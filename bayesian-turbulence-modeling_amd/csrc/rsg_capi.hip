@@ -275,6 +275,7 @@ struct rsg_context {
     int clockKHz = 0;        // device wall clock (wall_clock64) frequency
     int resident256 = 0, resident512 = 0;   // member workgroups the device holds at once, per shape
     size_t rot = 0;          // first active member of the next launch (rotating window)
+    int schedule = RSG_SCHEDULE_ROUND_ROBIN;
     int maxSlots = 4096;
     std::thread engine;
     bool stop = false;
@@ -454,9 +455,22 @@ void engine_loop(rsg_context* ctx) {
         const int resident = bs == 256 ? ctx->resident256 : ctx->resident512;
         const int nl = resident > 0 ? std::min(na, resident) : na;
         std::vector<int64_t> batch(nl);
-        const size_t r0 = ctx->rot % (size_t)na;
-        for (int i = 0; i < nl; ++i) batch[i] = ctx->active[(r0 + i) % na];
-        ctx->rot = r0 + nl;
+        if (ctx->schedule == RSG_SCHEDULE_LARGEST_FIRST && nl < na) {
+            // longest-processing-time first: the members on the largest
+            // meshes (the longest solves of an ensemble) hold the device,
+            // the others fill the places they free (ties: admission order)
+            std::vector<int64_t> order = ctx->active;
+            auto cells = [&](int64_t id) {
+                return ctx->meshes[ctx->problems[ctx->tickets[id]->problemId]->meshId]->prep.nC;
+            };
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int64_t a, int64_t b) { return cells(a) > cells(b); });
+            std::copy(order.begin(), order.begin() + nl, batch.begin());
+        } else {
+            const size_t r0 = ctx->rot % (size_t)na;
+            for (int i = 0; i < nl; ++i) batch[i] = ctx->active[(r0 + i) % na];
+            ctx->rot = r0 + nl;
+        }
         ctx->hMembers.resize(nl);
         ctx->hStatus.resize(nl);
         for (int i = 0; i < nl; ++i) {
@@ -814,6 +828,15 @@ int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_siz
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->itersPerLaunch = iters_per_launch;
     ctx->blockSize = block_size;
+    return RSG_OK;
+}
+
+int rsg_set_schedule(rsg_context* ctx, int32_t policy) {
+    if (!ctx) return RSG_ERR_ARG;
+    if (policy != RSG_SCHEDULE_ROUND_ROBIN && policy != RSG_SCHEDULE_LARGEST_FIRST)
+        return set_err(ctx, RSG_ERR_ARG, "unknown schedule policy");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->schedule = policy;
     return RSG_OK;
 }
 

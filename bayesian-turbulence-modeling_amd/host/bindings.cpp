@@ -367,6 +367,11 @@ PYBIND11_MODULE(rans_sst_py, m) {
     m.def("engine_set_launch", [](int iters, int block) { Engine::get().setLaunch(iters, block); },
           py::arg("iters_per_launch"), py::arg("block_size") = 0);
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
+    m.def("engine_set_schedule", [](const std::string& p) {
+              if (p == "round_robin") Engine::get().setSchedule(RSG_SCHEDULE_ROUND_ROBIN);
+              else if (p == "largest_first") Engine::get().setSchedule(RSG_SCHEDULE_LARGEST_FIRST);
+              else throw std::invalid_argument("schedule must be 'round_robin' or 'largest_first'");
+          }, py::arg("policy"));
     m.def("ticket_states",
           [](const std::vector<int64_t>& tickets) {
               std::vector<int32_t> st(tickets.size(), 0);

@@ -44,6 +44,12 @@ void Engine::setLaunch(int itersPerLaunch, int blockSize) {
     if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
 }
 
+void Engine::setSchedule(int policy) {
+    rsg_context* c = ctx();
+    int rc = rsg_set_schedule(c, policy);
+    if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
+}
+
 void Engine::setTimeSlice(int microseconds) {
     rsg_context* c = ctx();
     int rc = rsg_set_time_slice(c, microseconds);

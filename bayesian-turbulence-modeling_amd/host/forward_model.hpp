@@ -38,6 +38,7 @@ public:
     void shutdown();
     void setLaunch(int itersPerLaunch, int blockSize);
     void setTimeSlice(int microseconds);
+    void setSchedule(int policy);
     std::vector<double> stats();
 private:
     std::mutex mu_;

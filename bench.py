@@ -218,7 +218,7 @@ def measure_traffic(args):
             cal[counter] = COPY_BYTES / (1024.0 * kib[0])
         work = [me, "--members", str(args.members), "--seed", str(args.seed), "--tol", str(args.tol),
                 "--iters-per-launch", str(args.iters_per_launch), "--block", str(args.block),
-                "--slice-ms", str(args.slice_ms),
+                "--slice-ms", str(args.slice_ms), "--schedule", args.schedule,
                 "--max-iterations", "1000", "--no-cpu", "--no-traffic"]
         hbm, alg, launches = {}, {}, {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -388,6 +388,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="0: engine chooses per launch (256 or 512)")
     ap.add_argument("--slice-ms", type=float, default=200.0,
                     help="time slice per launch in ms (0: fixed --iters-per-launch budget)")
+    ap.add_argument("--schedule", choices=("round_robin", "largest_first"), default="largest_first",
+                    help="which members a launch runs when more are active than the device holds")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
@@ -416,6 +418,7 @@ def main():
     specs = ensembles.channel_ensemble(args.members, args.seed)
     rs.engine_set_launch(args.iters_per_launch, args.block)
     rs.engine_set_time_slice(int(args.slice_ms * 1000))
+    rs.engine_set_schedule(args.schedule)
     # engine start-up and code-object load, outside the timed region
     _, prime = ensembles.build_models(rs, specs[:1], tol=args.tol, max_iterations=2)
     rs.evaluate_many(prime, [specs[0]["theta"]])
@@ -475,7 +478,7 @@ def main():
                                % (args.members, args.tol),
                    "members": args.members, "tol": args.tol, "seed": args.seed,
                    "iters_per_launch": args.iters_per_launch, "block": args.block,
-                   "slice_ms": args.slice_ms},
+                   "slice_ms": args.slice_ms, "schedule": args.schedule},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "converged": int(conv),

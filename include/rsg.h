@@ -17,6 +17,9 @@
  *   rsg_fetch              FlowFields + ConvergenceHistory returned by solve
  *                          (core/include/FlowFields.hpp:9-49)
  *   rsg_ldu_solve          ILinearSolver::solve (core/include/LinearSolver.hpp:115-120)
+ *   rsg_set_launch / rsg_set_time_slice / rsg_set_schedule
+ *                          engine tuning (no reference counterpart: the reference
+ *                          solves one member at a time)
  *   rsg_system_solve       AMGSolver::solve / PCGSolver::solve on one large system
  *                          (core/include/LinearSolver.hpp:602-653, :147-186)
  *   rsg_system_hierarchy   AMGSolver::setup (core/include/LinearSolver.hpp:516-550)
@@ -181,6 +184,15 @@ int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_siz
  * held to the pace of the largest one in the launch.  0 (the default) returns
  * to the fixed iteration budget.  Results do not depend on the setting. */
 int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds);
+/* Which active members a launch runs when more are active than the device
+ * holds at once: RSG_SCHEDULE_ROUND_ROBIN (default) rotates a window over the
+ * active members in admission order; RSG_SCHEDULE_LARGEST_FIRST runs the
+ * members on the largest meshes first (longest-processing-time order, which
+ * shortens an ensemble's makespan; smaller members wait).  Results do not
+ * depend on the policy. */
+#define RSG_SCHEDULE_ROUND_ROBIN 0
+#define RSG_SCHEDULE_LARGEST_FIRST 1
+int rsg_set_schedule(rsg_context* ctx, int32_t policy);
 /* Engine counters: out[0] launches, out[1] members admitted, out[2] members retired,
  * out[3] kernel milliseconds (HIP events), out[4] max concurrent members. */
 int rsg_stats(rsg_context* ctx, double* out);

@@ -215,6 +215,33 @@ def test_time_sliced_launches_do_not_change_results(rsg_lib, gpu_ctx, meshes):
     assert rsg_lib.rsg_set_time_slice(gpu_ctx, -1) != 0
 
 
+@pytest.mark.parametrize("policy", [0, 1])
+def test_more_members_than_resident_slots(rsg_lib, gpu_ctx, meshes, policy):
+    """300 members on two meshes, more than the device holds at once: each
+    launch takes a rotating window (policy 0) or the largest meshes first
+    (policy 1); every member's fields equal its solve alone."""
+    s_small = settings_for("channel_bounded", max_iterations=60, pressure_solver=0)
+    s_big = settings_for("c0", max_iterations=80, convergence_tol=1e-12, pressure_solver=0, amg_refresh=1000)
+    Q = Problem(rsg_lib, gpu_ctx, meshes["channel_bounded"], "channel_bounded", s_small)
+    P = Problem(rsg_lib, gpu_ctx, meshes["c0"], "c0", s_big)
+    try:
+        rsg_lib.rsg_set_launch(gpu_ctx, 20, 0)
+        _, q_alone = Q.solve()
+        _, p_alone = P.solve()
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_schedule(gpu_ctx, policy))
+        tickets = [(Q if i % 10 else P).submit() for i in range(300)]
+        res = [(Q if i % 10 else P).fetch(t) for i, t in enumerate(tickets)]
+    finally:
+        rsg_lib.rsg_set_schedule(gpu_ctx, 0)
+        rsg_lib.rsg_set_launch(gpu_ctx, 200, 0)
+        P.close(); Q.close()
+    for i, (r, a) in enumerate(res):
+        ref = q_alone if i % 10 else p_alone
+        for k in ("U", "p", "k", "omega", "nuT"):
+            assert np.array_equal(a[k], ref[k]), (i, k)
+    assert rsg_lib.rsg_set_schedule(gpu_ctx, 7) != 0
+
+
 @pytest.mark.parametrize("case", ["c0", "bfs"])
 def test_warm_start_matches_oracle(rsg_lib, gpu_ctx, oracle, meshes, case):
     """rsg_init kind 1 (ForwardModel's warm start from a cached state, with its
