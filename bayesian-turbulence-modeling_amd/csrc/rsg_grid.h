@@ -24,8 +24,6 @@
 // The hierarchy is built on the host (the greedy aggregation is sequential);
 // every iteration runs on the device.  Included by rsg_capi.hip.
 #pragma once
-#include <hip/hip_cooperative_groups.h>
-
 #include <chrono>
 #include <vector>
 
@@ -92,40 +90,6 @@ __global__ __launch_bounds__(kBlock) void k_prolong(int n, const int* __restrict
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     x[i] += xc[map[i]];
-}
-
-// The coarsest level's 51 damped-Jacobi sweeps (one from zero, then 25
-// pairs) in one cooperative launch, the grid synchronising between sweeps
-// instead of ending a kernel: on a few-thousand-row level a sweep's own work
-// is a few microseconds, less than a launch.  Rows are strided over the grid;
-// the arithmetic is k_jacobi's.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_coarse_sweeps(int n, const int* __restrict__ rp,
-                                                           const int* __restrict__ ci, const T* __restrict__ v,
-                                                           const T* __restrict__ dg, const T* __restrict__ b, T* x,
-                                                           T* r) {
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
-    const T wJ = T(2) / T(3);
-    const int stride = gridDim.x * kBlock;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const T d = dg[i];
-        const T s = b[i];
-        const T x0 = 0;
-        x[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
-    }
-    for (int q = 0; q < 50; ++q) {
-        grid.sync();
-        const T* xi = (q & 1) ? r : x;
-        T* xo = (q & 1) ? x : r;
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-            T s = b[i];
-            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xi[ci[e]];
-            const T x0 = xi[i];
-            const T d = dg[i];
-            xo[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
-        }
-    }
 }
 
 // x = C^{-1} b on the coarsest level (n <= 64), one block
@@ -370,7 +334,6 @@ template <typename T> struct Hierarchy {
     std::vector<DevLevel<T>> lv;
     T* cinv = nullptr;
     bool dense = false;
-    int coopBlocks = 0;   // blocks of k_coarse_sweeps the device co-schedules (0: per-sweep launches)
 };
 
 template <typename T>
@@ -403,14 +366,6 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
     }
     D.dense = dense;
     if (dense) { D.cinv = A.take<T>(64 * 64); putT(D.cinv, G); }
-    {
-        int dev = 0, coop = 0, cus = 0, per = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && coop &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_coarse_sweeps<T>, kBlock, 0) == hipSuccess)
-            D.coopBlocks = per * cus;
-    }
     if (A.err == hipSuccess) A.err = hipStreamSynchronize(s);
     return A.err;
 }
@@ -443,26 +398,11 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
         if (D.dense) {
             hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, c.n, D.cinv, (const T*)c.b, c.x);
         } else {
-            // one cooperative launch when the device co-schedules the grid,
-            // else one launch per sweep (same arithmetic)
             const dim3 g(blocks(c.n)), bl(kBlock);
-            const int nb = std::min(blocks(c.n), D.coopBlocks);
-            int n = c.n;
-            const int* rp = c.rp; const int* ci = c.ci; const T* v = c.v; const T* dg = c.dg; const T* bb = c.b;
-            T* x = c.x; T* r = c.r;
-            void* args[] = {&n, &rp, &ci, &v, &dg, &bb, &x, &r};
-            const bool coop = nb > 0 && hipLaunchCooperativeKernel((const void*)k_coarse_sweeps<T>, dim3(nb), bl, args,
-                                                                   0, s) == hipSuccess;
-            if (!coop) {
-                (void)hipGetLastError();
-                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b,
-                                   (const T*)nullptr, c.x);
-                for (int q = 0; q < 25; ++q) {
-                    hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b,
-                                       (const T*)c.x, c.r);
-                    hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b,
-                                       (const T*)c.r, c.x);
-                }
+            hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)nullptr, c.x);
+            for (int q = 0; q < 25; ++q) {
+                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.x, c.r);
+                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.r, c.x);
             }
         }
     }
