@@ -60,3 +60,11 @@ def test_argument_errors_without_device(rsg_lib):
                                      A.dptr(np.ones(2)), A.dptr(np.ones(1)), A.iptr(np.zeros(1, np.int32))) == -1
     assert rsg_lib.rsg_context_create(0, C.c_uint64(1 << 20), None) == -1
     assert rsg_lib.rsg_context_destroy(None) == -1
+    # engine tuning and the large-system solve reject a missing context before any device call
+    assert rsg_lib.rsg_set_launch(None, 100, 0) == -1
+    assert rsg_lib.rsg_set_time_slice(None, 1000) == -1
+    assert rsg_lib.rsg_set_schedule(None, 0) == -1
+    st = np.zeros(8)
+    assert rsg_lib.rsg_system_solve(None, 1, 0, A.iptr(np.zeros(1, np.int32)), None, A.dptr(np.ones(1)), None, None,
+                                    A.dptr(np.ones(1)), A.dptr(np.zeros(1)), 0, 0, 10, C.c_double(1e-8),
+                                    A.dptr(st)) == -1
