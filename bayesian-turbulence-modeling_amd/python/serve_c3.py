@@ -11,6 +11,7 @@ Meshes (a request picks one uniformly):
     (python/UQ/datasets/couette_forward.py:49-75).
 
 Usage: python serve_c3.py [--rates 50,100] [--requests N] [--max-iterations M] [--slice-ms T]
+                          [--meshes both|column] [--schedule round_robin|largest_first]
 Prints one JSON line per rate: throughput (converged solves/s over the span
 from first arrival to last result), p50/p99 latency (arrival -> result).
 """
@@ -30,7 +31,7 @@ import serving  # noqa: E402
 THETA_INDICES = [8, 9, 0, 1, 2]      # betaStar, a1, sigma_k1, sigma_w1, beta1
 
 
-def build_models(rs, tol, max_iterations):
+def build_models(rs, tol, max_iterations, meshes="both"):
     ps = rs.InferenceParameterSet.from_indices("c3_five", THETA_INDICES)
     keep, models, names = [], [], []
     # channel column, 128 cells
@@ -43,6 +44,8 @@ def build_models(rs, tol, max_iterations):
     bcs = rs.FlowBoundaryConditions.channel_defaults(col, 1.0, 1e-4, 1.0)
     models.append(rs.ForwardModel(col, ps, rs.ObservationOperator(), bcs, 1.0 / 550.0, s))
     keep.append(col); names.append("channel_column_128")
+    if meshes == "column":
+        return keep, models, names
     # Couette 8 x 192
     nu, uw = 1.0 / 20000.0, 2.0
     ub = 0.5 * uw
@@ -70,10 +73,14 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--slice-ms", type=float, default=100.0,
                     help="engine time slice per launch (0: fixed iteration budget)")
+    ap.add_argument("--meshes", choices=("both", "column"), default="both",
+                    help="both meshes, or the channel column alone (short solves: a load test)")
+    ap.add_argument("--schedule", choices=("round_robin", "largest_first"), default="round_robin")
     args = ap.parse_args()
     import rans_sst_py as rs
     rs.engine_set_time_slice(int(args.slice_ms * 1000))
-    keep, models, names = build_models(rs, args.tol, args.max_iterations)
+    rs.engine_set_schedule(args.schedule)
+    keep, models, names = build_models(rs, args.tol, args.max_iterations, args.meshes)
     for rate in [float(r) for r in args.rates.split(",")]:
         trace = serving.arrival_trace(rate, args.requests, len(models), n_chains=args.chains, seed=args.seed)
         done = [0]
