@@ -33,6 +33,7 @@
 #include "rsg_prep.h"
 #include "rsg_member.h"
 #include "rsg_grid.h"
+#include "rsg_geom.h"
 
 // ------------------------------------------------------------------ kernels
 namespace {
@@ -1048,6 +1049,53 @@ int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
         for (int j = row_ptr[i]; j < row_ptr[i + 1]; ++j)
             if (col_idx[j] < 0 || col_idx[j] >= n) return RSG_ERR_ARG;
     return rsg::amg_aggregate(n, row_ptr, col_idx, values, diag, map);
+}
+
+int rsg_wall_distance(rsg_context* ctx, int32_t n_cells, const double* cell_centers, int32_t n_wall,
+                      const double* wall_points, double* wall_dist, double* kernel_seconds) {
+    using namespace rsg::geom;
+    if (!ctx || n_cells < 0 || n_wall < 0 || (n_cells > 0 && (!cell_centers || !wall_dist)) ||
+        (n_wall > 0 && !wall_points))
+        return RSG_ERR_ARG;
+    if (kernel_seconds) *kernel_seconds = 0.0;
+    if (n_cells == 0) return RSG_OK;
+    if (n_wall == 0) {                                   // Mesh.cpp:358-362
+        std::fill(wall_dist, wall_dist + n_cells, 1e10);
+        return RSG_OK;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "hipSetDevice");
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "stream");
+    hipError_t e = hipSuccess;
+    float ms = 0.0f;
+    {
+        rsg::grid::DevAlloc A;
+        double* cc = A.take<double>(3 * (size_t)n_cells);
+        double* wp = A.take<double>(3 * (size_t)n_wall);
+        double* out = A.take<double>((size_t)n_cells);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (A.err == hipSuccess) A.err = hipEventCreate(&ev[0]);
+        if (A.err == hipSuccess) A.err = hipEventCreate(&ev[1]);
+        if (A.err == hipSuccess) A.err = hipMemcpyAsync(cc, cell_centers, 24 * (size_t)n_cells, hipMemcpyHostToDevice, s);
+        if (A.err == hipSuccess) A.err = hipMemcpyAsync(wp, wall_points, 24 * (size_t)n_wall, hipMemcpyHostToDevice, s);
+        if (A.err == hipSuccess) A.err = hipEventRecord(ev[0], s);
+        if (A.err == hipSuccess) {
+            hipLaunchKernelGGL(k_wall_distance, dim3(wall_blocks(n_cells)), dim3(kWallBlock), 0, s, n_cells, cc,
+                               n_wall, wp, out);
+            A.err = hipGetLastError();
+        }
+        if (A.err == hipSuccess) A.err = hipEventRecord(ev[1], s);
+        if (A.err == hipSuccess) A.err = hipMemcpyAsync(wall_dist, out, 8 * (size_t)n_cells, hipMemcpyDeviceToHost, s);
+        if (A.err == hipSuccess) A.err = hipStreamSynchronize(s);
+        if (A.err == hipSuccess) A.err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        for (hipEvent_t v : ev)
+            if (v) (void)hipEventDestroy(v);
+        e = A.err;
+    }
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
+    if (kernel_seconds) *kernel_seconds = 1e-3 * ms;
+    return RSG_OK;
 }
 
 #ifdef RSG_PROF

@@ -164,7 +164,17 @@ PYBIND11_MODULE(rans_sst_py, m) {
                     py::arg("y_nodes"), py::arg("dx") = 1.0)
         .def_static("load_from_file", &Mesh::loadFromFile)
         .def("save_binary", &Mesh::saveBinary)
-        .def("compute_wall_distance", &Mesh::computeWallDistance)
+        .def("compute_wall_distance",
+             [](Mesh& m, bool onDevice) {
+                 if (onDevice) computeWallDistanceOnDevice(m);
+                 else m.computeWallDistance();
+             },
+             py::arg("on_device") = false,
+             "Mesh::computeWallDistance; on_device=True runs the brute-force search as one GPU kernel "
+             "(the same distances, bit for bit)")
+        .def("compute_wall_distance_timed",
+             [](Mesh& m) { return computeWallDistanceOnDevice(m); },
+             "compute_wall_distance(on_device=True), returning the kernel's seconds")
         .def("set_patch_type", &Mesh::setPatchType, py::arg("name"), py::arg("type"))
         .def("n_cells", &Mesh::nCells)
         .def("n_faces", &Mesh::nFaces)

@@ -102,10 +102,21 @@ ForwardModel::~ForwardModel() {
     }
 }
 
+double computeWallDistanceOnDevice(Mesh& mesh) {
+    rsg_context* c = Engine::get().ctx();
+    const std::vector<double> cc = mesh.cellCenters(), wp = mesh.wallFaceCenters();
+    std::vector<double> d(mesh.nCells());
+    double secs = 0.0;
+    const int rc = rsg_wall_distance(c, mesh.nCells(), cc.data(), (int32_t)(wp.size() / 3), wp.data(), d.data(), &secs);
+    if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+    mesh.setWallDistance(std::move(d));
+    return secs;
+}
+
 void ForwardModel::ensureProblem() {
     if (problemId_ >= 0) return;
     rsg_context* c = Engine::get().ctx();
-    if (!mesh_.hasWallDistance()) const_cast<Mesh&>(mesh_).computeWallDistance();
+    if (!mesh_.hasWallDistance()) computeWallDistanceOnDevice(const_cast<Mesh&>(mesh_));
     int32_t mid = -1, pid = -1;
     int rc = rsg_mesh_upload(c, &mesh_.desc(), &mid);
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));

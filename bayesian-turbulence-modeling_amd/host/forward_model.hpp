@@ -47,6 +47,10 @@ private:
 
 std::string engineError(rsg_context* ctx, int rc);
 
+// Mesh::computeWallDistance on the device (rsg_wall_distance): the same
+// distances, bit for bit; returns the kernel's seconds.
+double computeWallDistanceOnDevice(Mesh& mesh);
+
 // WarmStartCache, ForwardModel.hpp:18-72 (value copies under the lock)
 class WarmStartCache {
 public:

@@ -524,6 +524,33 @@ void Mesh::computeWallDistance() {
     bump();
 }
 
+std::vector<double> Mesh::wallFaceCenters() const {
+    std::vector<double> w;
+    for (const Patch& p : patches_)
+        if (p.type == "wall")
+            for (int f : p.faces) {
+                const Vec3& c = faces_[f].center;
+                w.insert(w.end(), {c.x, c.y, c.z});
+            }
+    return w;
+}
+
+std::vector<double> Mesh::cellCenters() const {
+    std::vector<double> c(3 * (size_t)nCells());
+    for (int i = 0; i < nCells(); ++i) {
+        c[3 * (size_t)i] = cells_[i].center.x;
+        c[3 * (size_t)i + 1] = cells_[i].center.y;
+        c[3 * (size_t)i + 2] = cells_[i].center.z;
+    }
+    return c;
+}
+
+void Mesh::setWallDistance(std::vector<double> d) {
+    if ((int)d.size() != nCells()) throw std::invalid_argument("wall distance needs one value per cell");
+    wallDist_ = std::move(d);
+    bump();
+}
+
 int Mesh::patchByName(const std::string& name) const {
     for (int i = 0; i < nPatches(); ++i)
         if (patches_[i].name == name) return i;

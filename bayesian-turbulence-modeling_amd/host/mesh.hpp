@@ -76,6 +76,12 @@ public:
     void saveBinary(const std::string& path) const;
 
     void computeWallDistance();
+    // Wall-face centres in patch order (the points computeWallDistance searches)
+    // and the cell centres, as x,y,z triples; and installing a wall distance
+    // computed elsewhere (the device kernel).
+    std::vector<double> wallFaceCenters() const;
+    std::vector<double> cellCenters() const;
+    void setWallDistance(std::vector<double> d);
     void setPatchType(const std::string& name, const std::string& type);
     int patchByName(const std::string& name) const;
 

@@ -41,7 +41,7 @@ int ParameterSensitivity::problemFor(const SolverSettings& s) {
     if (it != problems_.end()) return it->second;
     rsg_context* c = Engine::get().ctx();
     if (meshId_ < 0) {
-        if (!mesh_.hasWallDistance()) const_cast<Mesh&>(mesh_).computeWallDistance();
+        if (!mesh_.hasWallDistance()) computeWallDistanceOnDevice(const_cast<Mesh&>(mesh_));
         int32_t mid = -1;
         const int rc = rsg_mesh_upload(c, &mesh_.desc(), &mid);
         if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));

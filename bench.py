@@ -20,6 +20,7 @@ cost per outer iteration, applied to this run's iteration counts.
 --workload c1 instead times the stand-alone 261,600-cell backward-facing-step
 pressure system (rsg_system_solve, AMG-PCG to 1e-10, FP64 and FP32-inner with
 FP64 refinement) next to the reference's CPU PCG on the same system.
+--workload walldist times the device wall-distance kernel on a 2M-cell mesh.
 """
 import argparse
 import glob
@@ -304,6 +305,59 @@ def bench_c0(args):
         "roofline": None, "cpu_baseline": cpu}), flush=True)
 
 
+def bench_walldist(args):
+    """Mesh::computeWallDistance (core/src/Mesh.cpp:345-373) at the 2M-cell
+    size: a 2500 x 800 wall-bounded channel (1.0 x 0.3, wall-clustered), every
+    cell against the 5000 wall-face centres, as one device kernel
+    (rsg_wall_distance through Mesh.compute_wall_distance(on_device=True)).
+    The reference's own loop, compiled from its sources, is timed on a
+    2500 x 80 sample and checked bit for bit against the device there."""
+    import ctypes as C
+    import rans_sst_py as rs
+    nx, ny = 2500, 800
+    m = rs.Mesh.make_channel_2d(nx, ny, 1.0, 0.3, 1e4 / 0.3, 1.0)
+    nw = 2 * nx
+    pairs = float(m.n_cells()) * nw
+    times = []
+    for k in range(args.warmup + args.steps):
+        t = m.compute_wall_distance_timed()
+        print("bench: wall distance kernel %.3f ms" % (1e3 * t), file=sys.stderr, flush=True)
+        if k >= args.warmup:
+            times.append(t)
+    t = float(np.median(times))
+    flops = 8.0 * pairs                                   # 3 sub, 3 mul, 2 add per pair
+    cpu, same = None, None
+    lib_ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if not args.no_cpu and os.path.exists(lib_ref):
+        L = C.CDLL(lib_ref)
+        ms = rs.Mesh.make_channel_2d(nx, 80, 1.0, 0.3, 1e4 / 0.3, 1.0)
+        path = "/tmp/bench_walldist.msh"
+        ms.save_binary(path)
+        sizes = (C.c_int * 4)()
+        t0 = time.perf_counter(); L.ref_mesh_sizes(path.encode(), sizes); t_load = time.perf_counter() - t0
+        wd = np.zeros(ms.n_cells())
+        t0 = time.perf_counter()
+        L.ref_wall_distance(path.encode(), wd.ctypes.data_as(C.POINTER(C.c_double)))
+        t_ref = time.perf_counter() - t0 - t_load
+        ms.compute_wall_distance(on_device=True)
+        same = bool(np.array_equal(np.asarray(ms.wall_distance()), wd))
+        sp = float(ms.n_cells()) * nw
+        cpu = {"value": sp / t_ref, "unit": "cell-wall pairs/s", "cores": 1, "kind": "reference",
+               "sample": "Mesh::computeWallDistance of the reference on the 2500 x 80 channel (%d cells x %d wall "
+                         "faces): %.2f s (mesh load, %.2f s, subtracted)" % (ms.n_cells(), nw, t_ref, t_load)}
+    print(json.dumps({
+        "metric": "wall-distance cell-wall pairs/s", "value": pairs / t, "unit": "cell-wall pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "wall distance of a 2M-cell mesh: channel 2500 x 800 (1.0 x 0.3), 5000 wall faces",
+                   "cells": m.n_cells(), "wall_faces": nw},
+        "bit_equal_to_reference_on_sample": same,
+        "roofline": {"bound": "fp64-valu", "achieved": flops / t / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": flops / t / 1e12 / 78.6, "traffic": None, "kernel": "k_wall_distance",
+                     "flops_per_launch": flops},
+        "cpu_baseline": cpu}), flush=True)
+
+
 def bench_c1(args):
     """The large single system: solves/s of AMG-PCG to 1e-10 (host hierarchy
     setup + device solve + host<->device copies, wall clock), FP64 and mixed."""
@@ -394,9 +448,10 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
     ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=("c2", "c1", "c0"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist"), default="c2",
                     help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
-                         "c0: single-solve latency of the Re_tau 550 channel")
+                         "c0: single-solve latency of the Re_tau 550 channel; walldist: the wall-distance "
+                         "kernel on a 2M-cell mesh")
     ap.add_argument("--max-iterations", type=int, default=200000,
                     help="outer-iteration cap per member (profiling runs shorten it)")
     args = ap.parse_args()
@@ -412,6 +467,9 @@ def main():
         return
     if args.workload == "c0":
         bench_c0(args)
+        return
+    if args.workload == "walldist":
+        bench_walldist(args)
         return
     import rans_sst_py as rs
     import ensembles

@@ -24,6 +24,7 @@
  *                          (core/include/LinearSolver.hpp:602-653, :147-186)
  *   rsg_system_hierarchy   AMGSolver::setup (core/include/LinearSolver.hpp:516-550)
  *   rsg_amg_aggregate      buildAggregates (core/include/LinearSolver.hpp:407-443)
+ *   rsg_wall_distance      Mesh::computeWallDistance (core/src/Mesh.cpp:345-373)
  */
 #ifndef RSG_H
 #define RSG_H
@@ -232,6 +233,15 @@ int rsg_system_hierarchy(int32_t n, int32_t nif, const int32_t* own, const int32
  * Returns the number of aggregates, or a negative status. */
 int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                       const double* values, const double* diag, int32_t* map);
+
+/* Wall distance of every cell on the device: for each of the n_cells cell
+ * centres (cell_centers, 3 doubles x,y,z per cell) the minimum Euclidean
+ * distance to the n_wall wall-face centres (wall_points, 3 doubles per face),
+ * into wall_dist[n_cells]; with no wall face every distance is 1e10.
+ * Bit-identical to Mesh::computeWallDistance (core/src/Mesh.cpp:345-373).
+ * Host buffers.  kernel_seconds (nullable) gets the kernel's duration. */
+int rsg_wall_distance(rsg_context* ctx, int32_t n_cells, const double* cell_centers, int32_t n_wall,
+                      const double* wall_points, double* wall_dist, double* kernel_seconds);
 
 #ifdef __cplusplus
 }
