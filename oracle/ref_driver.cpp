@@ -19,6 +19,7 @@
 #include "SSTModel.hpp"
 #include "SIMPLESolver.hpp"
 #include "StressOperators.hpp"
+#include "ObservationOperator.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -420,6 +421,45 @@ int ref_transpose_stress(const char* path, const double* coeff, const double* U,
         std::vector<double> zero(m.nCells(), 0.0);
         std::vector<double> s = transposeStressSource(m, c, u, comp, wallZero ? &zero : nullptr);
         for (int i = 0; i < m.nCells(); ++i) src[i] = s[i];
+        return 0;
+    } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
+}
+
+// ObservationOperator::evaluate and logLikelihood (ObservationOperator.hpp:
+// 159-378) of the reference on the given U (cells, interleaved), U boundary
+// values and p.  Per observable: type 0 Drag 1 PressureTap 2 VelocityProfile
+// 3 SkinFriction 4 SeparationPoint 5 ReattachmentLength; patch index (or -1),
+// location, component, reference area / velocity / pressure / density, and
+// the measurement (observed, sigma, sigma_model).
+int ref_observe(const char* path, int nObs, const int* type, const int* patch, const double* loc,
+                const int* comp, const double* refArea, const double* refVel, const double* refP,
+                const double* refRho, const double* observed, const double* sigma, const double* sigmaModel,
+                const double* U, const double* Ub, const double* p, double nu, double* pred, double* loglik) {
+    try {
+        Mesh m = loadMesh(path);
+        ObservationOperator op;
+        for (int i = 0; i < nObs; ++i) {
+            const Vec3 L(loc[3 * i], loc[3 * i + 1], loc[3 * i + 2]);
+            const std::string name = patch[i] >= 0 ? m.patch(patch[i]).name : std::string();
+            switch (type[i]) {
+                case 0: op.addDrag(name, observed[i], sigma[i], refArea[i], refVel[i], sigmaModel[i]); break;
+                case 1: op.addPressureTap(L, observed[i], sigma[i], refVel[i], sigmaModel[i], refP[i], refRho[i]); break;
+                case 2: op.addVelocityProfile(L, comp[i], observed[i], sigma[i], sigmaModel[i]); break;
+                case 3: op.addSkinFriction(name, L, observed[i], sigma[i], refVel[i], sigmaModel[i]); break;
+                case 4: op.addSeparationPoint(name, observed[i], sigma[i], sigmaModel[i]); break;
+                case 5: op.addReattachmentLength(name, observed[i], sigma[i], sigmaModel[i]); break;
+                default: return -1;
+            }
+        }
+        FlowFields f(m);
+        for (int i = 0; i < m.nCells(); ++i) {
+            f.U[i] = Vec3(U[3 * i], U[3 * i + 1], U[3 * i + 2]);
+            f.p[i] = p[i];
+        }
+        for (int i = 0; i < f.U.nBnd(); ++i) f.U.bnd(i) = Vec3(Ub[3 * i], Ub[3 * i + 1], Ub[3 * i + 2]);
+        const std::vector<double> r = op.evaluate(m, f, nu);
+        for (int i = 0; i < nObs; ++i) pred[i] = r[i];
+        *loglik = op.logLikelihood(m, f, nu);
         return 0;
     } catch (const std::exception& e) { std::fprintf(stderr, "ref: %s\n", e.what()); return -1; }
 }

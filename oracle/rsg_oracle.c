@@ -1633,3 +1633,16 @@ int om_observe(const om_mesh* m, int nObs, const int* type, const int* patch,
     }
     return 0;
 }
+
+/* ObservationOperator.hpp:364-378 (Measurement::totalVariance, :16) */
+double om_loglik(int nObs, const double* pred, const double* observed, const double* sigma,
+                 const double* sigmaModel) {
+    double ll = 0.0;
+    for (int i = 0; i < nObs; ++i) {
+        double var = sigma[i] * sigma[i] + sigmaModel[i] * sigmaModel[i];
+        if (var < 1e-30) var = 1e-30;
+        double diff = pred[i] - observed[i];
+        ll += -0.5 * diff * diff / var;
+    }
+    return ll;
+}

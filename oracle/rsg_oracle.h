@@ -125,6 +125,11 @@ int om_observe(const om_mesh* m, int nObs, const int* type, const int* patch,
                const double* U, const double* Ub, const double* p, double nu,
                double* pred);
 
+/* Gaussian log-likelihood (ObservationOperator.hpp:364-378): the sum of
+ * -0.5 (pred - observed)^2 / var, var = sigma^2 + sigma_model^2 floored at 1e-30. */
+double om_loglik(int nObs, const double* pred, const double* observed, const double* sigma,
+                 const double* sigmaModel);
+
 #ifdef __cplusplus
 }
 #endif

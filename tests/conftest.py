@@ -42,6 +42,7 @@ def oracle():
     lib = C.CDLL(path)
     lib.om_mesh_load.restype = C.c_void_p
     lib.om_sst_point.restype = C.c_double
+    lib.om_loglik.restype = C.c_double
     return lib
 
 

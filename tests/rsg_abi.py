@@ -220,3 +220,48 @@ def new_result(n, nB, hist_cap=0):
         setattr(r, k, dptr(a))
     r.history_cap = hist_cap
     return r, arrs
+
+
+OBS_ARRAYS = ("type", "patch", "loc", "comp", "refArea", "refVel", "refP", "refRho", "observed", "sigma", "sigmaModel")
+
+
+def observation_set(msh, rng):
+    """Observables of every type on a mesh (ObservationOperator.hpp:20-32): on
+    every wall patch a drag, a separation point, a reattachment length and
+    skin friction at three points; pressure taps and velocity probes (all
+    three components) at random points in the bounding box; measurement
+    values and sigmas random, one sigma pair zero (the 1e-30 variance floor)."""
+    lo, hi = msh.cc.min(axis=0), msh.cc.max(axis=0)
+    rows = []
+    walls = [q for q in range(msh.nP) if msh.ptypes[q] == "wall"]
+    for q in walls:
+        rows.append((0, q, (0.0, 0.0, 0.0), 0, 0.7, 1.3, 0.0, 1.0))
+        rows.append((4, q, (0.0, 0.0, 0.0), 0, 1.0, 1.0, 0.0, 1.0))
+        rows.append((5, q, (0.0, 0.0, 0.0), 0, 1.0, 1.0, 0.0, 1.0))
+        for _ in range(3):
+            rows.append((3, q, tuple(rng.uniform(lo, hi)), 0, 1.0, 0.9, 0.0, 1.0))
+    for _ in range(3):
+        rows.append((1, -1, tuple(rng.uniform(lo, hi)), 0, 1.0, 1.1, 0.2, 1.2))
+    for comp in (0, 1, 2):
+        rows.append((2, -1, tuple(rng.uniform(lo, hi)), comp, 1.0, 1.0, 0.0, 1.0))
+    n = len(rows)
+    o = {"type": np.array([r[0] for r in rows], np.int32), "patch": np.array([r[1] for r in rows], np.int32),
+         "loc": np.array([r[2] for r in rows], float).ravel(), "comp": np.array([r[3] for r in rows], np.int32),
+         "refArea": np.array([r[4] for r in rows]), "refVel": np.array([r[5] for r in rows]),
+         "refP": np.array([r[6] for r in rows]), "refRho": np.array([r[7] for r in rows]),
+         "observed": rng.normal(size=n), "sigma": rng.uniform(0.05, 0.5, n), "sigmaModel": rng.uniform(0.0, 0.2, n)}
+    o["sigma"][-1] = 0.0; o["sigmaModel"][-1] = 0.0
+    return o
+
+
+def wavy_state(msh, rng):
+    """U, U boundary values and p whose wall shear changes sign along x (so
+    that separation and reattachment crossings exist)."""
+    L = msh.cc[:, 0].max() - msh.cc[:, 0].min() + 1e-12
+    U = np.zeros((msh.nC, 3))
+    U[:, 0] = np.sin(6.0 * np.pi * msh.cc[:, 0] / L) + 0.1 * rng.normal(size=msh.nC)
+    U[:, 1] = 0.1 * rng.normal(size=msh.nC)
+    nB = msh.nF - msh.nIF
+    Ub = 0.01 * rng.normal(size=(nB, 3))
+    p = rng.normal(size=msh.nC)
+    return U.ravel(), Ub.ravel(), p

@@ -187,3 +187,35 @@ def test_warm_start_bit_exact(oracle, ref, meshes, case):
             assert np.array_equal(o[k], r[k]), (turb, k)
         for k in ("Ub", "pb", "kb", "omb"):
             assert np.array_equal(o[k], rb[k]), (turb, k)
+
+
+@pytest.mark.parametrize("case", ["bfs", "hill", "channel_bounded", "plate"])
+def test_observation_operator_bit_exact(oracle, ref, meshes, case):
+    """om_observe / om_loglik against ObservationOperator::evaluate and
+    logLikelihood of the reference, every observable type."""
+    path = meshes[case]
+    msh = A.Msh(path)
+    rng = np.random.default_rng(5)
+    o = A.observation_set(msh, rng)
+    U, Ub, p = A.wavy_state(msh, rng)
+    n = len(o["type"])
+    nu = 1.7e-4
+    po, pr, llr = np.zeros(n), np.zeros(n), np.zeros(1)
+    m = oracle.om_mesh_load(path.encode())
+    try:
+        oracle.om_observe(C.c_void_p(m), n, A.iptr(o["type"]), A.iptr(o["patch"]), A.dptr(o["loc"]),
+                          A.iptr(o["comp"]), A.dptr(o["refArea"]), A.dptr(o["refVel"]), A.dptr(o["refP"]),
+                          A.dptr(o["refRho"]), A.dptr(U), A.dptr(Ub), A.dptr(p), C.c_double(nu), A.dptr(po))
+    finally:
+        oracle.om_mesh_free(C.c_void_p(m))
+    llo = oracle.om_loglik(n, A.dptr(po), A.dptr(o["observed"]), A.dptr(o["sigma"]), A.dptr(o["sigmaModel"]))
+    assert ref.ref_observe(path.encode(), n, A.iptr(o["type"]), A.iptr(o["patch"]), A.dptr(o["loc"]),
+                           A.iptr(o["comp"]), A.dptr(o["refArea"]), A.dptr(o["refVel"]), A.dptr(o["refP"]),
+                           A.dptr(o["refRho"]), A.dptr(o["observed"]), A.dptr(o["sigma"]),
+                           A.dptr(o["sigmaModel"]), A.dptr(U), A.dptr(Ub), A.dptr(p), C.c_double(nu),
+                           A.dptr(pr), A.dptr(llr)) == 0
+    assert np.array_equal(po, pr)
+    assert llo == llr[0]
+    # the crossings were found (interpolated x inside the patch), not the defaults
+    sep = po[o["type"] == 4]; rea = po[o["type"] == 5]
+    assert np.any(sep != 0.0) and np.all(np.isfinite(rea))
