@@ -227,6 +227,9 @@ struct ProblemEntry {
     RsgBcDev* bcDev = nullptr;
     RsgSettingsDev* setDev = nullptr;
     int users = 0;
+    int nObs = 0;                   // observables (rsg_problem_set_observations)
+    size_t obsOff = 0, obsBytes = 0;
+    RsgObsDev* obsDev = nullptr;
 };
 
 struct Ticket {
@@ -242,7 +245,7 @@ struct Ticket {
     int state = RSG_TICKET_PENDING;
     // results (host copies)
     RsgScalars sc{};
-    std::vector<double> cells, bnd, hist;
+    std::vector<double> cells, bnd, hist, pred;
     double tSubmit = 0, tStart = 0, tEnd = 0;
     // device workspace
     size_t off = 0, bytes = 0;
@@ -335,7 +338,13 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     MeshEntry& ME = *ctx->meshes[P.meshId];
     const rsg::PrepMesh& pm = ME.prep;
     const int histCap = P.settings.record_history ? P.settings.max_iterations : 0;
-    size_t bytes = rsg::member_bytes(pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap);
+    rsg::Carver sz;
+    {
+        RsgMemberDev tmp;
+        rsg::carve_member(sz, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, tmp);
+        if (P.nObs > 0) sz.take<double>((size_t)P.nObs + 1);
+    }
+    size_t bytes = sz.off;
     size_t off;
     if (!ctx->arena.alloc(bytes, off)) return false;
     T.off = off; T.bytes = bytes;
@@ -343,6 +352,7 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     memset(&M, 0, sizeof M);
     rsg::Carver cv; cv.base = ctx->arena.base + off;
     rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M);
+    if (P.nObs > 0) { M.obs = P.obsDev; M.pred = cv.take<double>((size_t)P.nObs + 1); }
     M.mesh = ME.devStruct; M.bc = P.bcDev; M.set = P.setDev; M.nu = P.nu;
     for (int i = 0; i < 11; ++i) M.coeffs[i] = T.coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = T.init[i];
@@ -386,6 +396,9 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     return true;
 }
 
+// Observables of a ticket's problem (fixed while the problem has solves in flight).
+int obsCount(rsg_context* ctx, const Ticket& T) { return ctx->problems[T.problemId]->nObs; }
+
 // Copies a finished member's state to host (ctx->mu not held).
 hipError_t collect(rsg_context* ctx, Ticket& T, const rsg::PrepMesh& pm, const rsg_settings& st) {
     const int n = pm.nC, nB = pm.nB;
@@ -404,6 +417,10 @@ hipError_t collect(rsg_context* ctx, Ticket& T, const rsg::PrepMesh& pm, const r
         T.hist.resize(6 * (size_t)std::max(len, 1));
         if (len > 0) e = hipMemcpy(T.hist.data(), M.hist, 48 * (size_t)len, hipMemcpyDeviceToHost);
         T.hist.resize(6 * (size_t)len);
+    }
+    if (e == hipSuccess && M.obs) {
+        T.pred.resize((size_t)obsCount(ctx, T) + 1);
+        e = hipMemcpy(T.pred.data(), M.pred, 8 * T.pred.size(), hipMemcpyDeviceToHost);
     }
     return e;
 }
@@ -692,7 +709,73 @@ int rsg_problem_release(rsg_context* ctx, int32_t problem_id) {
             return set_err(ctx, RSG_ERR_STATE, "problem has pending solves");
     ctx->meshes[it->second->meshId]->users--;
     ctx->arena.release(it->second->off, it->second->bytes);
+    if (it->second->obsBytes) ctx->arena.release(it->second->obsOff, it->second->obsBytes);
     ctx->problems.erase(it);
+    return RSG_OK;
+}
+
+int rsg_problem_set_observations(rsg_context* ctx, int32_t problem_id, int32_t n, const rsg_observation* obs,
+                                 int32_t n_faces, const int32_t* faces, const double* face_x) {
+    if (!ctx || n < 0 || (n > 0 && !obs) || n_faces < 0 || (n_faces > 0 && (!faces || !face_x))) return RSG_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->problems.find(problem_id);
+    if (it == ctx->problems.end()) return set_err(ctx, RSG_ERR_STATE, "unknown problem id");
+    ProblemEntry& P = *it->second;
+    if (P.obsBytes) return set_err(ctx, RSG_ERR_STATE, "observations already set for this problem");
+    if (P.users > 0) return set_err(ctx, RSG_ERR_STATE, "problem has solves in flight");
+    const rsg::PrepMesh& pm = ctx->meshes[P.meshId]->prep;
+    for (int j = 0; j < n_faces; ++j)
+        if (faces[j] < pm.nIF || faces[j] >= pm.nF) return set_err(ctx, RSG_ERR_ARG, "observation face is not a boundary face");
+    for (int i = 0; i < n; ++i) {
+        const rsg_observation& o = obs[i];
+        const bool point = o.type == RSG_OBS_PRESSURE_TAP || o.type == RSG_OBS_VELOCITY;
+        const bool patch = o.type == RSG_OBS_DRAG || o.type == RSG_OBS_SEPARATION || o.type == RSG_OBS_REATTACHMENT;
+        if (o.type < 0 || o.type > RSG_OBS_REATTACHMENT) return set_err(ctx, RSG_ERR_ARG, "unknown observation type");
+        if (point && (o.index < 0 || o.index >= pm.nC)) return set_err(ctx, RSG_ERR_ARG, "observation cell out of range");
+        if (o.type == RSG_OBS_SKIN_FRICTION && (o.index < -1 || (o.index >= 0 && (o.index < pm.nIF || o.index >= pm.nF))))
+            return set_err(ctx, RSG_ERR_ARG, "skin-friction face is not a boundary face");
+        if (o.type == RSG_OBS_VELOCITY && (o.component < 0 || o.component > 2))
+            return set_err(ctx, RSG_ERR_ARG, "velocity component must be 0, 1 or 2");
+        if (patch && (o.face_begin < 0 || o.face_end < o.face_begin || o.face_end > n_faces))
+            return set_err(ctx, RSG_ERR_ARG, "observation face range out of bounds");
+    }
+    if (n == 0) return RSG_OK;
+    // image: RsgObsDev, then type idx comp fb fe [n] ints, par [6n], faces [n_faces], fx [n_faces]
+    rsg::Carver cv;
+    cv.take<RsgObsDev>(1);
+    for (int q = 0; q < 5; ++q) cv.take<int>(n);
+    cv.take<double>(6 * (size_t)n);
+    cv.take<int>(std::max(n_faces, 1));
+    cv.take<double>(std::max(n_faces, 1));
+    const size_t bytes = cv.off;
+    size_t off;
+    if (!ctx->arena.alloc(bytes, off)) return set_err(ctx, RSG_ERR_NOMEM, "observations exceed the HBM budget");
+    char* dbase = ctx->arena.base + off;
+    std::vector<char> stage(bytes, 0);
+    rsg::Carver hv; hv.base = stage.data();
+    rsg::Carver dv; dv.base = dbase;
+    RsgObsDev* hs = hv.take<RsgObsDev>(1); RsgObsDev* ds = dv.take<RsgObsDev>(1);
+    int* hi[5]; const int* di[5];
+    for (int q = 0; q < 5; ++q) { hi[q] = hv.take<int>(n); di[q] = dv.take<int>(n); }
+    double* hp = hv.take<double>(6 * (size_t)n); const double* dp = dv.take<double>(6 * (size_t)n);
+    int* hf = hv.take<int>(std::max(n_faces, 1)); const int* df = dv.take<int>(std::max(n_faces, 1));
+    double* hx = hv.take<double>(std::max(n_faces, 1)); const double* dx = dv.take<double>(std::max(n_faces, 1));
+    for (int i = 0; i < n; ++i) {
+        const rsg_observation& o = obs[i];
+        hi[0][i] = o.type; hi[1][i] = o.index; hi[2][i] = o.component; hi[3][i] = o.face_begin; hi[4][i] = o.face_end;
+        double* pr = hp + 6 * (size_t)i;
+        pr[0] = o.ref_velocity; pr[1] = o.ref_area; pr[2] = o.ref_density; pr[3] = o.ref_pressure;
+        pr[4] = o.observed;
+        double var = o.sigma * o.sigma + o.sigma_model * o.sigma_model;   // Measurement::totalVariance
+        pr[5] = var < 1e-30 ? 1e-30 : var;
+    }
+    for (int j = 0; j < n_faces; ++j) { hf[j] = faces[j]; hx[j] = face_x[j]; }
+    hs->n = n; hs->type = di[0]; hs->idx = di[1]; hs->comp = di[2]; hs->fb = di[3]; hs->fe = di[4];
+    hs->par = dp; hs->faces = df; hs->fx = dx;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpy(dbase, stage.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { ctx->arena.release(off, bytes); return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e)); }
+    P.nObs = n; P.obsOff = off; P.obsBytes = bytes; P.obsDev = ds;
     return RSG_OK;
 }
 
@@ -805,6 +888,14 @@ int rsg_fetch(rsg_context* ctx, int64_t ticket, rsg_result* out) {
         int len = std::min<int>((int)(T.hist.size() / 6), out->history_cap);
         memcpy(out->history, T.hist.data(), 48 * (size_t)len);
         out->history_len = len;
+    }
+    out->n_predictions = 0;
+    out->log_lik = 0.0;
+    if (!T.pred.empty()) {
+        const int nObs = (int)T.pred.size() - 1;
+        out->n_predictions = nObs;
+        out->log_lik = T.pred[nObs];
+        if (out->predictions && out->predictions_cap >= nObs) memcpy(out->predictions, T.pred.data(), 8 * (size_t)nObs);
     }
     return RSG_OK;
 }

@@ -157,6 +157,25 @@ struct RsgAmgDev {
     int* aCol;             // [aCap]
 };
 
+// Observation operator of one problem (ObservationOperator.hpp:20-32),
+// resolved against the mesh on the host: the nearest cell / wall face of each
+// point observable and the patch face lists (with face-centre x) of the
+// patch observables.  Codes as RSG_OBS_* in rsg.h.
+enum { RSGO_DRAG = 0, RSGO_PRESSURE = 1, RSGO_VELOCITY = 2, RSGO_SKIN_FRICTION = 3, RSGO_SEPARATION = 4,
+       RSGO_REATTACHMENT = 5 };
+
+struct RsgObsDev {
+    int n;
+    const int* type;       // [n]
+    const int* idx;        // [n] cell (pressure, velocity), face (skin friction; -1: none)
+    const int* comp;       // [n] velocity component
+    const int* fb;         // [n] first entry of the observable's faces (drag, separation, reattachment)
+    const int* fe;         // [n] one past the last
+    const double* par;     // [6 n] refVelocity, referenceArea, refDensity, refPressure, observed, variance
+    const int* faces;      // patch faces, patch order (boundary faces)
+    const double* fx;      // face-centre x of each listed face
+};
+
 struct RsgMemberDev {
     const RsgMeshDev* mesh;
     const RsgBcDev* bc;
@@ -182,4 +201,6 @@ struct RsgMemberDev {
     double* hist;          // [6 * histCap] (optional)
     int histCap;
     RsgScalars* sc;
+    const RsgObsDev* obs;  // observables of the problem (null: none)
+    double* pred;          // [obs->n + 1]: predictions, then the log-likelihood
 };

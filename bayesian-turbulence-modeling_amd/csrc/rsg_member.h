@@ -1397,6 +1397,95 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
 // Runs up to M->iterBudget outer iterations of one member, and no longer than
 // M->sliceTicks of wall clock when that is set (the iteration in progress
 // finishes; a member's results do not depend on where launches split it).
+// ObservationOperator::evaluate and logLikelihood (ObservationOperator.hpp:
+// 159-378) on the member's final fields, by one thread in the reference's
+// operation order (so the predictions are bit-identical to the host's).  The
+// mesh-only searches (nearest cell, nearest face of the patch) and the patch
+// face lists come resolved from the host; pred[n] gets the log-likelihood.
+RSG_D void observe_member(const RsgMemberDev* M) {
+    const RsgObsDev* O = M->obs;
+    const RsgMeshDev* m = M->mesh;
+    const int nIF = m->nIF;
+    const double nu = M->nu;
+    double ll = 0.0;
+    for (int i = 0; i < O->n; ++i) {
+        const double* par = O->par + 6 * i;
+        const double refU = par[0];
+        double out = 0.0;
+        switch (O->type[i]) {
+            case RSGO_DRAG: {                                   // computeDrag :186-223
+                double force = 0.0;
+                for (int j = O->fb[i]; j < O->fe[i]; ++j) {
+                    const int f = O->faces[j], ow = m->own[f];
+                    const double Sf = m->area[f];
+                    force += M->p[ow] * m->nx[f] * Sf;
+                    const double tau_x = nu * (M->Ux[ow] - M->Ubx[f - nIF]) / m->dlt[f];
+                    force += tau_x * Sf;
+                }
+                const double dyn = 0.5 * refU * refU;
+                out = force / (dyn * par[1]);
+                break;
+            }
+            case RSGO_PRESSURE: {                               // computePressureAt :261-271
+                const double dyn = 0.5 * par[2] * refU * refU;
+                out = (M->p[O->idx[i]] - par[3]) / dmax(dyn, 1e-20);
+                break;
+            }
+            case RSGO_VELOCITY: {                               // computeVelocityAt :251-258
+                const int c = O->idx[i], q = O->comp[i];
+                out = q == 0 ? M->Ux[c] : q == 1 ? M->Uy[c] : M->Uz[c];
+                break;
+            }
+            case RSGO_SKIN_FRICTION: {                          // computeSkinFrictionAt :226-248
+                const int f = O->idx[i];
+                if (f < 0) break;
+                const int ow = m->own[f];
+                const double ux = M->Ux[ow], uy = M->Uy[ow], uz = M->Uz[ow];
+                const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f];
+                const double dn = ux * nx + uy * ny + uz * nz;
+                const double tx = ux - nx * dn, ty = uy - ny * dn, tz = uz - nz * dn;
+                const double tau = nu * sqrt(tx * tx + ty * ty + tz * tz) / m->dlt[f];
+                out = tau / dmax(0.5 * refU * refU, 1e-20);
+                break;
+            }
+            case RSGO_SEPARATION:                               // computeSeparationPoint :277-307
+            case RSGO_REATTACHMENT: {                           // computeReattachmentLength :311-340
+                const bool sep = O->type[i] == RSGO_SEPARATION;
+                out = sep ? 0.0 : (O->fe[i] > O->fb[i] ? O->fx[O->fe[i] - 1] : 0.0);
+                bool havePrev = false;
+                double prevTau = sep ? 1.0 : -1.0, prevX = 0.0;
+                for (int j = O->fb[i]; j < O->fe[i]; ++j) {
+                    const int f = O->faces[j], ow = m->own[f];
+                    const double ux = M->Ux[ow], uy = M->Uy[ow], uz = M->Uz[ow];
+                    const double nx = m->nx[f];
+                    const double dn = ux * nx + uy * m->ny[f] + uz * m->nz[f];
+                    const double tau = nu * (ux - nx * dn) / m->dlt[f];
+                    const double x = O->fx[j];
+                    if (sep && havePrev && prevTau > 0 && tau <= 0) {
+                        const double denom = prevTau - tau;
+                        const double frac = (fabs(denom) > 1e-30) ? prevTau / denom : 0.0;
+                        out = prevX + frac * (x - prevX);
+                        break;
+                    }
+                    if (!sep && havePrev && prevTau < 0 && tau >= 0) {
+                        const double denom = tau - prevTau;
+                        const double frac = (fabs(denom) > 1e-30) ? (-prevTau) / denom : 0.0;
+                        out = prevX + frac * (x - prevX);
+                        break;
+                    }
+                    prevTau = tau; prevX = x; havePrev = true;
+                }
+                break;
+            }
+            default: break;
+        }
+        M->pred[i] = out;
+        const double diff = out - par[4];                       // logLikelihood :364-378
+        ll += -0.5 * diff * diff / par[5];
+    }
+    M->pred[O->n] = ll;
+}
+
 template <int CMAX>
 RSG_D void run_member(Thr t, RsgMemberDev* M) {
     Ctx C;
@@ -1454,6 +1543,11 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
             if (stop) break;
         }
 #endif
+    }
+    // a finished member's observables, from the fields every thread wrote
+    if (M->obs && S.status != RSGM_RUNNING) {
+        RSG_SYNC();
+        if (C.t.tid == 0) observe_member(M);
     }
     if (C.t.tid == 0) {
         sc->iter = S.iter; sc->status = S.status; sc->turbEstablished = S.turbEstablished;

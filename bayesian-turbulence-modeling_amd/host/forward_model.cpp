@@ -113,6 +113,15 @@ double computeWallDistanceOnDevice(Mesh& mesh) {
     return secs;
 }
 
+void setProblemObservations(rsg_context* c, int problem, const Mesh& mesh, const ObservationOperator& op) {
+    if (op.nObs() == 0) return;
+    ResolvedObservations R;
+    op.resolve(mesh, R);
+    const int rc = rsg_problem_set_observations(c, problem, (int32_t)R.obs.size(), R.obs.data(),
+                                                (int32_t)R.faces.size(), R.faces.data(), R.faceX.data());
+    if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+}
+
 void ForwardModel::ensureProblem() {
     if (problemId_ >= 0) return;
     rsg_context* c = Engine::get().ctx();
@@ -125,6 +134,13 @@ void ForwardModel::ensureProblem() {
     rsg_settings s = settings_.toC();
     rc = rsg_problem_create(c, mid, &R.desc, nu_, &s, &pid);
     if (rc != RSG_OK) { rsg_mesh_release(c, mid); throw std::runtime_error(engineError(c, rc)); }
+    try {
+        setProblemObservations(c, pid, mesh_, obsOp_);
+    } catch (...) {
+        rsg_problem_release(c, pid);
+        rsg_mesh_release(c, mid);
+        throw;
+    }
     meshId_ = mid; problemId_ = pid;
 }
 
@@ -166,6 +182,9 @@ EvaluationResult ForwardModel::complete(int64_t ticket, const std::vector<double
     r.U_bnd = f.Ub.data(); r.p_bnd = f.pb.data(); r.k_bnd = f.kb.data(); r.om_bnd = f.omb.data();
     r.history = conv.history.empty() ? nullptr : conv.history.data();
     r.history_cap = (int32_t)(conv.history.size() / 6);
+    std::vector<double> pred(obsOp_.nObs());
+    r.predictions = pred.empty() ? nullptr : pred.data();
+    r.predictions_cap = (int32_t)pred.size();
     rc = rsg_fetch(c, ticket, &r);
     rsg_release(c, ticket);
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
@@ -195,8 +214,10 @@ EvaluationResult ForwardModel::complete(int64_t ticket, const std::vector<double
     }
     lastFields_ = std::move(f);
     hasLastFields_ = true;
-    res.predictions = obsOp_.evaluate(mesh_, lastFields_, nu_);
-    res.loglik = obsOp_.logLikelihood(res.predictions);
+    // ForwardModel.cpp:95-97, evaluated on the device at the end of the solve
+    if (r.n_predictions != obsOp_.nObs()) throw std::runtime_error("device observables missing");
+    res.predictions = std::move(pred);
+    res.loglik = obsOp_.nObs() > 0 ? r.log_lik : 0.0;
     return res;
 }
 

@@ -51,6 +51,10 @@ std::string engineError(rsg_context* ctx, int rc);
 // distances, bit for bit; returns the kernel's seconds.
 double computeWallDistanceOnDevice(Mesh& mesh);
 
+// Hands a problem its observables (resolved against the mesh) so every solve
+// evaluates them on the device when it finishes.
+void setProblemObservations(rsg_context* ctx, int problem, const Mesh& mesh, const ObservationOperator& op);
+
 // WarmStartCache, ForwardModel.hpp:18-72 (value copies under the lock)
 class WarmStartCache {
 public:

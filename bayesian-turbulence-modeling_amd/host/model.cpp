@@ -365,6 +365,52 @@ std::vector<double> ObservationOperator::evaluate(const Mesh& m, const FlowField
 }
 
 // Gaussian log-likelihood, ObservationOperator.hpp:367-378
+// The searches of findNearestCell / findNearestWallFace (ObservationOperator.hpp:
+// 160-183) and the patch face lists that computeDrag / computeSeparationPoint /
+// computeReattachmentLength walk; the device evaluates the rest.
+void ObservationOperator::resolve(const Mesh& m, ResolvedObservations& out) const {
+    out.obs.clear(); out.faces.clear(); out.faceX.clear();
+    for (const Observable& o : obs_) {
+        rsg_observation r{};
+        r.ref_velocity = o.refVelocity; r.ref_area = o.referenceArea;
+        r.ref_density = o.refDensity; r.ref_pressure = o.refPressure;
+        r.observed = o.meas.observed; r.sigma = o.meas.sigma_obs; r.sigma_model = o.meas.sigma_model;
+        r.index = -1;
+        switch (o.type) {
+            case ObsType::Drag:
+            case ObsType::SeparationPoint:
+            case ObsType::ReattachmentLength: {
+                r.type = o.type == ObsType::Drag ? RSG_OBS_DRAG
+                       : o.type == ObsType::SeparationPoint ? RSG_OBS_SEPARATION : RSG_OBS_REATTACHMENT;
+                const Patch& pat = m.patch(m.patchByName(o.patchName));
+                r.face_begin = (int32_t)out.faces.size();
+                for (int fi : pat.faces) { out.faces.push_back(fi); out.faceX.push_back(m.face(fi).center.x); }
+                r.face_end = (int32_t)out.faces.size();
+                break;
+            }
+            case ObsType::PressureTap:
+            case ObsType::VelocityProfile:
+                r.type = o.type == ObsType::PressureTap ? RSG_OBS_PRESSURE_TAP : RSG_OBS_VELOCITY;
+                r.index = nearestCell(m, o.location);
+                r.component = o.component == 0 ? 0 : o.component == 1 ? 1 : 2;
+                break;
+            case ObsType::SkinFriction: {
+                r.type = RSG_OBS_SKIN_FRICTION;
+                const Patch& pat = m.patch(m.patchByName(o.patchName));
+                int best = pat.faces.empty() ? -1 : pat.faces[0];
+                double bd = 1e30;
+                for (int fi : pat.faces) {
+                    double d = (m.face(fi).center - o.location).norm2();
+                    if (d < bd) { bd = d; best = fi; }
+                }
+                r.index = best;
+                break;
+            }
+        }
+        out.obs.push_back(r);
+    }
+}
+
 double ObservationOperator::logLikelihood(const std::vector<double>& pred) const {
     double ll = 0.0;
     for (size_t i = 0; i < obs_.size(); ++i) {

@@ -143,8 +143,17 @@ struct Observable {
     double referenceArea = 1.0, refVelocity = 1.0, refPressure = 0.0, refDensity = 1.0;
 };
 
+// Observables resolved against one mesh for the device (rsg_problem_set_observations).
+struct ResolvedObservations {
+    std::vector<rsg_observation> obs;
+    std::vector<int32_t> faces;
+    std::vector<double> faceX;
+};
+
 class ObservationOperator {
 public:
+    // The mesh-only part of evaluate: nearest cells / faces and patch face lists.
+    void resolve(const Mesh& mesh, ResolvedObservations& out) const;
     void addDrag(const std::string& wallPatch, double Cd_obs, double sigma, double refArea, double refVel,
                  double sigma_model = 0.0);
     void addPressureTap(const Vec3& loc, double Cp_obs, double sigma, double refVel, double sigma_model = 0.0,

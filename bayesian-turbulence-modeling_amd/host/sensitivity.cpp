@@ -53,6 +53,12 @@ int ParameterSensitivity::problemFor(const SolverSettings& s) {
     int32_t pid = -1;
     const int rc = rsg_problem_create(c, meshId_, &R.desc, nu_, &cs, &pid);
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+    try {
+        setProblemObservations(c, pid, mesh_, obs_);
+    } catch (...) {
+        rsg_problem_release(c, pid);
+        throw;
+    }
     problems_[key] = pid;
     return pid;
 }
@@ -80,7 +86,8 @@ int64_t ParameterSensitivity::submit(int problem, const std::vector<double>& the
     return t;
 }
 
-void ParameterSensitivity::fetch(int64_t ticket, FlowFields& f, ConvergenceInfo& conv) {
+void ParameterSensitivity::fetch(int64_t ticket, FlowFields& f, ConvergenceInfo& conv, std::vector<double>* pred,
+                                 double* logLik) {
     rsg_context* c = Engine::get().ctx();
     int rc = rsg_wait(c, &ticket, 1, -1.0);
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
@@ -89,10 +96,17 @@ void ParameterSensitivity::fetch(int64_t ticket, FlowFields& f, ConvergenceInfo&
     r.U = f.U.data(); r.p = f.p.data(); r.k = f.k.data(); r.omega = f.omega.data(); r.nuT = f.nuT.data();
     r.F1 = f.F1.data(); r.F2 = f.F2.data(); r.Pk = f.Pk.data(); r.CDkw = f.CDkw.data();
     r.U_bnd = f.Ub.data(); r.p_bnd = f.pb.data(); r.k_bnd = f.kb.data(); r.om_bnd = f.omb.data();
+    if (pred) {
+        pred->assign(obs_.nObs(), 0.0);
+        r.predictions = pred->empty() ? nullptr : pred->data();
+        r.predictions_cap = (int32_t)pred->size();
+    }
     rc = rsg_fetch(c, ticket, &r);
     rsg_release(c, ticket);
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
     if (r.status != RSG_TICKET_DONE) throw std::runtime_error("device solve failed: " + std::string(rsg_last_error(c)));
+    if (pred && r.n_predictions != obs_.nObs()) throw std::runtime_error("device observables missing");
+    if (logLik) *logLik = obs_.nObs() > 0 ? r.log_lik : 0.0;
     conv.finalIter = r.final_iter; conv.converged = r.converged; conv.diverged = r.diverged;
     for (int i = 0; i < 6; ++i) conv.innerIters[i] = r.inner_iters[i];
     f.turbEstablished = r.turb_established != 0;
@@ -164,11 +178,9 @@ TangentGradientResult ParameterSensitivity::etaJacobianWarmFD(const std::vector<
         for (int s = 0; s < 2; ++s) {
             FlowFields f;
             ConvergenceInfo conv;
-            fetch(tickets[2 * j + s], f, conv);
+            fetch(tickets[2 * j + s], f, conv, &eta[s], &ll[s]);     // observables evaluated on the device
             out.nResidualEvals += conv.finalIter;
             ok[s] = !conv.diverged;
-            eta[s] = obs_.evaluate(mesh_, f, nu_);
-            ll[s] = obs_.logLikelihood(eta[s]);
         }
         out.krylovConverged[j] = (ok[0] && ok[1]) ? 1 : 0;
         if (!(ok[0] && ok[1])) continue;                      // the column stays 0 on divergence

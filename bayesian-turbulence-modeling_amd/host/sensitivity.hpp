@@ -68,7 +68,9 @@ private:
 
     int problemFor(const SolverSettings& s);
     int64_t submit(int problem, const std::vector<double>& theta11, const FlowFields* warm);
-    void fetch(int64_t ticket, FlowFields& f, ConvergenceInfo& conv);
+    // pred / logLik (optional): the observables the device evaluated at the end of the solve
+    void fetch(int64_t ticket, FlowFields& f, ConvergenceInfo& conv, std::vector<double>* pred = nullptr,
+               double* logLik = nullptr);
 };
 
 }  // namespace rsh

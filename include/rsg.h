@@ -25,6 +25,10 @@
  *   rsg_system_hierarchy   AMGSolver::setup (core/include/LinearSolver.hpp:516-550)
  *   rsg_amg_aggregate      buildAggregates (core/include/LinearSolver.hpp:407-443)
  *   rsg_wall_distance      Mesh::computeWallDistance (core/src/Mesh.cpp:345-373)
+ *   rsg_problem_set_observations
+ *                          ObservationOperator::evaluate / logLikelihood
+ *                          (core/include/ObservationOperator.hpp:159-378) as called
+ *                          from ForwardModel::evaluate (ForwardModel.cpp:95-97)
  */
 #ifndef RSG_H
 #define RSG_H
@@ -150,6 +154,10 @@ typedef struct rsg_result {
     int32_t history_len;      /* entries written */
     double inner_iters[6];    /* cumulative inner iterations per equation */
     double t_submit, t_start, t_end;   /* engine clock, seconds */
+    double* predictions;      /* observables evaluated on the device (NULL: not copied) */
+    int32_t predictions_cap;  /* entries available in predictions */
+    int32_t n_predictions;    /* observables of the problem (0: none set) */
+    double log_lik;           /* Gaussian log-likelihood of the predictions */
 } rsg_result;
 
 int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** out);
@@ -165,6 +173,35 @@ int rsg_mesh_release(rsg_context* ctx, int32_t mesh_id);
 int rsg_problem_create(rsg_context* ctx, int32_t mesh_id, const rsg_bc_desc* bc, double nu,
                        const rsg_settings* settings, int32_t* problem_id);
 int rsg_problem_release(rsg_context* ctx, int32_t problem_id);
+
+/* Observables of a problem, resolved against its mesh by the caller
+ * (Observable, core/include/ObservationOperator.hpp:20-32).  Every solve of
+ * the problem admitted afterwards evaluates them on the device when it
+ * finishes (converged, diverged or at the iteration cap), in the reference's
+ * operation order, and rsg_fetch returns the predictions and the Gaussian
+ * log-likelihood.  index: the nearest cell (pressure tap, velocity; the
+ * reference's findNearestCell) or the nearest face of the patch (skin
+ * friction, findNearestWallFace; -1 for an empty patch).  Drag, separation
+ * and reattachment read faces[face_begin, face_end): the patch's boundary
+ * faces in patch order, face_x their centres' x.  variance = sigma^2 +
+ * sigma_model^2, floored at 1e-30 as the reference does.  Set once per
+ * problem, before its first solve is admitted. */
+#define RSG_OBS_DRAG 0
+#define RSG_OBS_PRESSURE_TAP 1
+#define RSG_OBS_VELOCITY 2
+#define RSG_OBS_SKIN_FRICTION 3
+#define RSG_OBS_SEPARATION 4
+#define RSG_OBS_REATTACHMENT 5
+typedef struct rsg_observation {
+    int32_t type;                 /* RSG_OBS_* */
+    int32_t index;                /* cell or face (see above) */
+    int32_t component;            /* velocity component 0, 1, 2 */
+    int32_t face_begin, face_end; /* range in faces[] (patch observables) */
+    double ref_velocity, ref_area, ref_density, ref_pressure;
+    double observed, sigma, sigma_model;
+} rsg_observation;
+int rsg_problem_set_observations(rsg_context* ctx, int32_t problem_id, int32_t n, const rsg_observation* obs,
+                                 int32_t n_faces, const int32_t* faces, const double* face_x);
 
 /* Enqueue one solve; returns immediately.  coeffs: the 11 SST coefficients
  * in InferenceParameterSet order (core/include/InferenceParameters.hpp:12-14). */

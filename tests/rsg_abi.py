@@ -61,7 +61,16 @@ class Result(C.Structure):
                 ("F2", PD), ("Pk", PD), ("CDkw", PD), ("U_bnd", PD), ("p_bnd", PD),
                 ("k_bnd", PD), ("om_bnd", PD), ("history", PD), ("history_cap", C.c_int32),
                 ("history_len", C.c_int32), ("inner_iters", C.c_double * 6),
-                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double)]
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+                ("predictions", PD), ("predictions_cap", C.c_int32), ("n_predictions", C.c_int32),
+                ("log_lik", C.c_double)]
+
+
+class Observation(C.Structure):
+    _fields_ = [("type", C.c_int32), ("index", C.c_int32), ("component", C.c_int32),
+                ("face_begin", C.c_int32), ("face_end", C.c_int32), ("ref_velocity", C.c_double),
+                ("ref_area", C.c_double), ("ref_density", C.c_double), ("ref_pressure", C.c_double),
+                ("observed", C.c_double), ("sigma", C.c_double), ("sigma_model", C.c_double)]
 
 
 def dptr(a):
@@ -265,3 +274,34 @@ def wavy_state(msh, rng):
     Ub = 0.01 * rng.normal(size=(nB, 3))
     p = rng.normal(size=msh.nC)
     return U.ravel(), Ub.ravel(), p
+
+
+def resolve_observations(msh, o):
+    """rsg_observation records for observation_set(): the reference's nearest
+    cell / nearest patch face searches (first minimum of the squared distance,
+    ObservationOperator.hpp:160-183) and the patch face lists."""
+    n = len(o["type"])
+    recs = (Observation * n)()
+    faces, fx = [], []
+    loc = o["loc"].reshape(-1, 3)
+    for i in range(n):
+        r = recs[i]
+        r.type = int(o["type"][i]); r.component = int(o["comp"][i]); r.index = -1
+        r.ref_velocity = o["refVel"][i]; r.ref_area = o["refArea"][i]; r.ref_density = o["refRho"][i]
+        r.ref_pressure = o["refP"][i]; r.observed = o["observed"][i]; r.sigma = o["sigma"][i]
+        r.sigma_model = o["sigmaModel"][i]
+        q = int(o["patch"][i])
+        if r.type in (1, 2):
+            d = msh.cc - loc[i]
+            r.index = int(np.argmin(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]))
+        elif r.type == 3:
+            pf = msh.pfaces[q]
+            if len(pf):
+                d = msh.fc[pf] - loc[i]
+                r.index = int(pf[np.argmin(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])])
+        else:
+            r.face_begin = len(faces)
+            faces += [int(f) for f in msh.pfaces[q]]
+            fx += [float(msh.fc[f][0]) for f in msh.pfaces[q]]
+            r.face_end = len(faces)
+    return recs, np.array(faces, np.int32), np.array(fx)

@@ -68,5 +68,6 @@ def test_argument_errors_without_device(rsg_lib):
     assert rsg_lib.rsg_system_solve(None, 1, 0, A.iptr(np.zeros(1, np.int32)), None, A.dptr(np.ones(1)), None, None,
                                     A.dptr(np.ones(1)), A.dptr(np.zeros(1)), 0, 0, 10, C.c_double(1e-8),
                                     A.dptr(st)) == -1
+    assert rsg_lib.rsg_problem_set_observations(None, 0, 0, None, 0, None, None) == -1
     assert rsg_lib.rsg_wall_distance(None, 1, A.dptr(np.zeros(3)), 1, A.dptr(np.zeros(3)), A.dptr(np.zeros(1)),
                                      None) == -1
