@@ -126,6 +126,14 @@ def meshes(rs, tmp_path_factory):
     yb = 0.1 * (1.0 - np.cos(2 * np.pi * x / 2.0))
     m = rs.Mesh.make_curved_channel_periodic_2d(x, yb, 1.0, 10, 2000.0, 1.0)
     out["hill"] = str(d / "hill.msh"); m.save_binary(out["hill"])
+    # unstructured: jittered quads and triangles extruded one layer, read as an
+    # ASCII OpenFOAM polyMesh (front/back faces an "empty" patch)
+    from test_openfoam_mesh import polygons, write_polymesh
+    xy, cells = polygons(nx=12, ny=8, jitter=0.25, split=True, seed=2)
+    write_polymesh(str(d / "tri_quad" / "constant" / "polyMesh"), xy, cells)
+    m = rs.Mesh.load_from_file(str(d / "tri_quad"))
+    m.set_patch_type("inlet", "inlet"); m.set_patch_type("outlet", "outlet")
+    out["tri_quad"] = str(d / "tri_quad.msh"); m.save_binary(out["tri_quad"])
     return out
 
 
@@ -137,6 +145,7 @@ CASES = {
     "bfs": (0, (1.0, 1e-4, 10.0), 2e-4, (0.0, 0.0, 0.0)),
     "plate": (2, (1.0, 1e-4, 10.0), 1e-5, (0.0, 0.0, 0.0)),
     "hill": (0, (1.0, 1e-4, 1.0), 5e-4, (5e-3, 0.0, 0.0)),
+    "tri_quad": (0, (1.0, 1e-4, 10.0), 2e-4, (0.0, 0.0, 0.0)),
 }
 
 FIELD_KEYS = ("U", "p", "k", "om", "nuT", "F1", "F2", "Pk", "CDkw")

@@ -27,7 +27,7 @@ def emu_solve(emu, path, case, s):
 
 @pytest.mark.parametrize("resident", [1, 0])
 @pytest.mark.parametrize("case,iters", [("c0", 700), ("channel_bounded", 150), ("couette", 150),
-                                        ("bfs", 80), ("plate", 80), ("hill", 120)])
+                                        ("bfs", 80), ("plate", 80), ("hill", 120), ("tri_quad", 80)])
 def test_emulated_device_trajectory_bit_exact(oracle, emu, meshes, case, iters, resident):
     """Both linear-solver paths of the device code: the register/LDS-resident
     solvers and the global-memory ones."""

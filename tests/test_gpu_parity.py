@@ -107,7 +107,7 @@ def test_device_linear_solvers(rsg_lib, gpu_ctx, oracle, meshes, solver):
 
 
 @pytest.mark.parametrize("case,iters", [("c0", 400), ("channel_bounded", 150), ("couette", 150),
-                                        ("bfs", 80), ("plate", 80), ("hill", 120)])
+                                        ("bfs", 80), ("plate", 80), ("hill", 120), ("tri_quad", 80)])
 def test_trajectory_matches_oracle(rsg_lib, gpu_ctx, oracle, meshes, case, iters):
     s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=1,
                      nut_floor_iters=min(500, iters // 2), record_history=1)

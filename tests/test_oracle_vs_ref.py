@@ -31,7 +31,7 @@ def ref_solve(ref, path, case, settings, hist_cap):
 
 
 @pytest.mark.parametrize("case,iters", [("c0", 600), ("channel_bounded", 150), ("couette", 150),
-                                        ("bfs", 80), ("plate", 80), ("hill", 120)])
+                                        ("bfs", 80), ("plate", 80), ("hill", 120), ("tri_quad", 80)])
 @pytest.mark.parametrize("psolver", [0, 1])
 def test_trajectory_bit_exact(oracle, ref, meshes, case, iters, psolver):
     s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=psolver,
