@@ -396,11 +396,8 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     return true;
 }
 
-// Observables of a ticket's problem (fixed while the problem has solves in flight).
-int obsCount(rsg_context* ctx, const Ticket& T) { return ctx->problems[T.problemId]->nObs; }
-
 // Copies a finished member's state to host (ctx->mu not held).
-hipError_t collect(rsg_context* ctx, Ticket& T, const rsg::PrepMesh& pm, const rsg_settings& st) {
+hipError_t collect(Ticket& T, const rsg::PrepMesh& pm, const rsg_settings& st, int nObs) {
     const int n = pm.nC, nB = pm.nB;
     RsgMemberDev& M = T.M;
     T.cells.assign(11 * (size_t)n, 0.0);
@@ -419,7 +416,7 @@ hipError_t collect(rsg_context* ctx, Ticket& T, const rsg::PrepMesh& pm, const r
         T.hist.resize(6 * (size_t)len);
     }
     if (e == hipSuccess && M.obs) {
-        T.pred.resize((size_t)obsCount(ctx, T) + 1);
+        T.pred.resize((size_t)nObs + 1);
         e = hipMemcpy(T.pred.data(), M.pred, 8 * T.pred.size(), hipMemcpyDeviceToHost);
     }
     return e;
@@ -521,7 +518,7 @@ void engine_loop(rsg_context* ctx) {
             T.sc = ctx->hStatus[i];
             if (T.sc.status != RSGM_RUNNING) {
                 ProblemEntry& P = *ctx->problems[T.problemId];
-                e = collect(ctx, T, ctx->meshes[P.meshId]->prep, P.settings);
+                e = collect(T, ctx->meshes[P.meshId]->prep, P.settings, P.nObs);
                 done.push_back(batch[i]);
             }
         }
