@@ -87,3 +87,35 @@ def test_observation_arguments(rsg_lib, gpu_ctx, meshes):
         assert rsg_lib.rsg_problem_set_observations(gpu_ctx, P.pid, 1, rec, 0, None, None) != 0   # once only
     finally:
         P.close()
+
+
+def test_empty_patch_observables(rsg_lib, gpu_ctx, meshes):
+    """Observables on an empty patch take the reference's defaults
+    (ObservationOperator.hpp: skin friction with no face 0, drag over no face 0,
+    separation 0, reattachment 0 with no last face); the log-likelihood sums
+    -0.5 (0 - observed)^2 / var."""
+    P = Problem(rsg_lib, gpu_ctx, meshes["bfs"], "bfs", settings_for("bfs", max_iterations=5))
+    try:
+        recs = (A.Observation * 4)()
+        observed = [0.3, -0.2, 0.5, 1.5]
+        for i, typ in enumerate((3, 0, 4, 5)):
+            recs[i].type = typ; recs[i].index = -1; recs[i].face_begin = 0; recs[i].face_end = 0
+            recs[i].ref_velocity = 1.0; recs[i].ref_area = 1.0; recs[i].ref_density = 1.0
+            recs[i].observed = observed[i]; recs[i].sigma = 0.1 * (i + 1); recs[i].sigma_model = 0.0
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_problem_set_observations(gpu_ctx, P.pid, 4, recs, 0, None, None))
+        t = P.submit()
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_wait(gpu_ctx, (C.c_int64 * 1)(t), 1, C.c_double(120.0)))
+        r, _ = A.new_result(P.msh.nC, P.msh.nF - P.msh.nIF)
+        pred = np.full(4, np.nan)
+        r.predictions = A.dptr(pred); r.predictions_cap = 4
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_fetch(gpu_ctx, C.c_int64(t), C.byref(r)))
+        rsg_lib.rsg_release(gpu_ctx, C.c_int64(t))
+        assert r.n_predictions == 4 and np.all(pred == 0.0)
+        ll = 0.0
+        for i in range(4):
+            sg = 0.1 * (i + 1)
+            var = sg * sg + 0.0 * 0.0
+            ll += -0.5 * (0.0 - observed[i]) * (0.0 - observed[i]) / var
+        assert r.log_lik == ll
+    finally:
+        P.close()
