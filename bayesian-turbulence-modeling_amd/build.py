@@ -1,6 +1,6 @@
 """Builds the native parts of the package in place.
 
-  librsg.so                      HIP engine for gfx950 (csrc/rsg_capi.hip), C-ABI of include/rsg.h
+  librsg.so                      HIP engine for gfx950 (csrc/rsg_capi.hip, csrc/rsg_team.hip), C-ABI of include/rsg.h
   rans_sst_py*.so                pybind11 host module (host/*.cpp) linked against librsg.so
 
 Run as a script or call build_all().  Both artefacts are written next to this
@@ -34,8 +34,9 @@ def _run(cmd):
 
 def build_engine():
     out = os.path.join(HERE, "librsg.so")
-    src = os.path.join(HERE, "csrc", "rsg_capi.hip")
-    _run([HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", os.path.join(HERE, "csrc"), src, "-o", out + ".tmp", "-lpthread"])
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("rsg_capi.hip", "rsg_team.hip")]
+    _run([HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", os.path.join(HERE, "csrc")] + srcs +
+         ["-o", out + ".tmp", "-lpthread"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -35,6 +35,10 @@
 #include "rsg_grid.h"
 #include "rsg_geom.h"
 
+// rsg_team.hip
+int rsg_team_capacity(int device);
+hipError_t rsg_team_launch(RsgMemberDev* dM, RsgScalars* dOut, double* gred, int blocks, hipStream_t s);
+
 // ------------------------------------------------------------------ kernels
 namespace {
 
@@ -62,6 +66,7 @@ struct LdsAreaWide {
 __device__ rsg::Thr make_thr(LdsArea& L, int bs) {
     rsg::Thr t;
     t.tid = threadIdx.x; t.bs = bs; t.red = L.red; t.lds = L.lds; t.phase = 0;
+    t.ltid = threadIdx.x; t.lbs = bs; t.blk = 0; t.nblk = 1; t.gred = nullptr;
     t.xb0 = L.xb; t.xb1 = L.xb + RSG_RES_MAXN; t.xsel = 0;
     t.cvec = L.cv; t.cvecCap = RSG_RES_MAXC;
     t.ceVal = nullptr; t.ceCol = nullptr; t.ceCap = 0; t.caCol = nullptr; t.caCap = 0;
@@ -277,6 +282,12 @@ struct rsg_context {
     int blockSize = 0;       // 0: automatic (512)
     int sliceUs = 0;         // > 0: each launch runs every member for this long (no iteration cap)
     int clockKHz = 0;        // device wall clock (wall_clock64) frequency
+    // team solves (rsg_set_team): members on meshes of at least teamMinCells
+    // cells run alone on teamBlocks workgroups, teamIters outer iterations a launch
+    int teamMinCells = 0, teamBlocks = 0, teamIters = 20, teamCap = 0;
+    double* dGred = nullptr;
+    RsgMemberDev* dTeamM = nullptr;
+    RsgScalars* dTeamOut = nullptr;
     int resident256 = 0, resident512 = 0;   // member workgroups the device holds at once, per shape
     size_t rot = 0;          // first active member of the next launch (rotating window)
     int schedule = RSG_SCHEDULE_ROUND_ROBIN;
@@ -486,9 +497,30 @@ void engine_loop(rsg_context* ctx) {
             for (int i = 0; i < nl; ++i) batch[i] = ctx->active[(r0 + i) % na];
             ctx->rot = r0 + nl;
         }
-        ctx->hMembers.resize(nl);
-        ctx->hStatus.resize(nl);
-        for (int i = 0; i < nl; ++i) {
+        // members on meshes of at least teamMinCells cells run as teams, one
+        // cooperative launch each, before the workgroup-per-member launch
+        std::vector<int64_t> team;
+        if (ctx->teamMinCells > 0) {
+            std::vector<int64_t> reg;
+            for (int64_t id : batch) {
+                const int nC = ctx->meshes[ctx->problems[ctx->tickets[id]->problemId]->meshId]->prep.nC;
+                (nC >= ctx->teamMinCells ? team : reg).push_back(id);
+            }
+            batch.swap(reg);
+        }
+        const int nr = (int)batch.size();
+        std::vector<RsgMemberDev> teamM(team.size());
+        std::vector<RsgScalars> teamSt(team.size());
+        for (size_t q = 0; q < team.size(); ++q) {
+            Ticket& T = *ctx->tickets[team[q]];
+            T.M.iterBudget = ctx->teamIters;
+            T.M.sliceTicks = 0;
+            teamM[q] = T.M;
+        }
+        const int teamBlocks = ctx->teamBlocks > 0 ? std::min(ctx->teamBlocks, ctx->teamCap) : ctx->teamCap;
+        ctx->hMembers.resize(nr);
+        ctx->hStatus.resize(nr);
+        for (int i = 0; i < nr; ++i) {
             Ticket& T = *ctx->tickets[batch[i]];
             if (ctx->sliceUs > 0 && ctx->clockKHz > 0) {
                 T.M.iterBudget = 1 << 30;
@@ -500,20 +532,51 @@ void engine_loop(rsg_context* ctx) {
             ctx->hMembers[i] = T.M;
         }
         lk.unlock();
-        hipError_t e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * nl,
-                                      hipMemcpyHostToDevice, ctx->stream);
-        hipEventRecord(ctx->ev0, ctx->stream);
-        hipLaunchKernelGGL(chunk_kernel(bs), dim3(nl), dim3(bs), 0, ctx->stream, ctx->dMembers, ctx->dStatus);
-        hipEventRecord(ctx->ev1, ctx->stream);
-        if (e == hipSuccess) e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(ctx->hStatus.data(), ctx->dStatus, sizeof(RsgScalars) * nl, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        hipError_t e = hipSuccess;
         float ms = 0.f;
-        if (e == hipSuccess) hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        if (!team.empty() && teamBlocks <= 0) e = hipErrorInvalidConfiguration;
+        if (!team.empty() && e == hipSuccess && !ctx->dGred) {
+            e = hipMalloc(&ctx->dGred, sizeof(double) * 2 * RSG_TEAM_MAXB * 8);
+            if (e == hipSuccess) e = hipMalloc(&ctx->dTeamM, sizeof(RsgMemberDev));
+            if (e == hipSuccess) e = hipMalloc(&ctx->dTeamOut, sizeof(RsgScalars));
+        }
+        for (size_t q = 0; q < team.size() && e == hipSuccess; ++q) {
+            e = hipMemcpyAsync(ctx->dTeamM, &teamM[q], sizeof(RsgMemberDev), hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+            if (e == hipSuccess) e = rsg_team_launch(ctx->dTeamM, ctx->dTeamOut, ctx->dGred, teamBlocks, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&teamSt[q], ctx->dTeamOut, sizeof(RsgScalars), hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            float tm = 0.f;
+            if (e == hipSuccess && hipEventElapsedTime(&tm, ctx->ev0, ctx->ev1) == hipSuccess) ms += tm;
+        }
+        if (nr > 0 && e == hipSuccess) {
+            e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * nr,
+                               hipMemcpyHostToDevice, ctx->stream);
+            hipEventRecord(ctx->ev0, ctx->stream);
+            hipLaunchKernelGGL(chunk_kernel(bs), dim3(nr), dim3(bs), 0, ctx->stream, ctx->dMembers, ctx->dStatus);
+            hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(ctx->hStatus.data(), ctx->dStatus, sizeof(RsgScalars) * nr, hipMemcpyDeviceToHost,
+                                   ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            float cm = 0.f;
+            if (e == hipSuccess && hipEventElapsedTime(&cm, ctx->ev0, ctx->ev1) == hipSuccess) ms += cm;
+        }
         // retire finished members
         std::vector<int64_t> done;
-        for (int i = 0; i < nl && e == hipSuccess; ++i) {
+        for (size_t q = 0; q < team.size() && e == hipSuccess; ++q) {
+            Ticket& T = *ctx->tickets[team[q]];
+            T.sc = teamSt[q];
+            if (T.sc.status != RSGM_RUNNING) {
+                ProblemEntry& P = *ctx->problems[T.problemId];
+                e = collect(T, ctx->meshes[P.meshId]->prep, P.settings, P.nObs);
+                done.push_back(team[q]);
+            }
+        }
+        for (int i = 0; i < nr && e == hipSuccess; ++i) {
             Ticket& T = *ctx->tickets[batch[i]];
             T.sc = ctx->hStatus[i];
             if (T.sc.status != RSGM_RUNNING) {
@@ -567,6 +630,7 @@ int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** 
     c->cuCount = prop.multiProcessorCount;
     c->warpSize = prop.warpSize;
     if (hipDeviceGetAttribute(&c->clockKHz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->clockKHz = 0;
+    c->teamCap = rsg_team_capacity(device);
     {
         int per = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chunk_kernel(256), 256, 0) == hipSuccess && per > 0)
@@ -600,6 +664,9 @@ int rsg_context_destroy(rsg_context* ctx) {
     hipStreamSynchronize(ctx->stream);
     if (ctx->dMembers) hipFree(ctx->dMembers);
     if (ctx->dStatus) hipFree(ctx->dStatus);
+    if (ctx->dGred) hipFree(ctx->dGred);
+    if (ctx->dTeamM) hipFree(ctx->dTeamM);
+    if (ctx->dTeamOut) hipFree(ctx->dTeamOut);
     if (ctx->arena.base) hipFree(ctx->arena.base);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
@@ -917,6 +984,16 @@ int rsg_set_launch(rsg_context* ctx, int32_t iters_per_launch, int32_t block_siz
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->itersPerLaunch = iters_per_launch;
     ctx->blockSize = block_size;
+    return RSG_OK;
+}
+
+int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t iters_per_launch) {
+    if (!ctx) return RSG_ERR_ARG;
+    if (min_cells < 0 || blocks < 0 || blocks > RSG_TEAM_MAXB || iters_per_launch < 1)
+        return set_err(ctx, RSG_ERR_ARG, "team: min_cells >= 0, 0 <= blocks <= 256, iters_per_launch >= 1");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (min_cells > 0 && ctx->teamCap <= 0) return set_err(ctx, RSG_ERR_UNSUPPORTED, "team launches are not available");
+    ctx->teamMinCells = min_cells; ctx->teamBlocks = blocks; ctx->teamIters = iters_per_launch;
     return RSG_OK;
 }
 

@@ -39,12 +39,21 @@
 #define RSG_HD static inline
 #define RSG_DM inline
 #define RSG_SYNC() ((void)0)
+#define RSG_BLOCK_SYNC() ((void)0)
 #else
 #include <hip/hip_runtime.h>
 #define RSG_D __device__ static __attribute__((always_inline)) inline
 #define RSG_HD __host__ __device__ static inline
 #define RSG_DM __device__ __attribute__((always_inline)) inline   // member functions
+#ifdef RSG_TEAM
+// Team build: one solve spread over every workgroup of a cooperative launch;
+// a phase boundary is a grid-wide barrier.
+#include <hip/hip_cooperative_groups.h>
+#define RSG_SYNC() cooperative_groups::this_grid().sync()
+#else
 #define RSG_SYNC() __syncthreads()
+#endif
+#define RSG_BLOCK_SYNC() __syncthreads()
 #endif
 
 // Section timers (built only with RSG_PROF): thread 0 of every member adds the
@@ -85,7 +94,13 @@ struct Thr {
     // LDS copies of the coarse ELL images (null: read them from global memory)
     double* ceVal; short* ceCol; int ceCap;
     short* caCol; int caCap;
+    // team build: tid/bs above span the whole team; these are the thread's
+    // place in its own workgroup and the workgroup's place in the team
+    int ltid, lbs, blk, nblk;
+    double* gred;        // global reduction partials [2 phases][RSG_TEAM_MAXB * 8]
 };
+
+#define RSG_TEAM_MAXB 256     // workgroups in a team at most
 
 #ifdef RSG_EMULATE
 template <int NV> RSG_D void bsum(Thr&, double (&)[NV]) {}
@@ -102,6 +117,7 @@ RSG_D double wmax(double v) {
     for (int off = 32; off >= 1; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
     return v;
 }
+#ifndef RSG_TEAM
 // Block sum of NV values, result identical in every thread: lane 0 of each
 // wave deposits its wave total, then every thread adds the wave totals in
 // wave order.  Double-buffered LDS so consecutive calls need one barrier.
@@ -141,6 +157,71 @@ template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
         v[i] = s;
     }
 }
+
+#else
+// Team sum / max: each workgroup reduces as above, its thread 0 stores
+// the workgroup's value, and after the grid barrier every thread combines the
+// workgroup values in workgroup order -- the same value in every thread of
+// the team.  Double-buffered like the workgroup version.
+template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
+    static_assert(NV <= 8, "team reductions carry at most 8 values");
+    const int lane = t.ltid & 63, wid = t.ltid >> 6, nw = (t.lbs + 63) >> 6;
+    double* buf = t.red + t.phase * 64;
+    double* gbuf = t.gred + t.phase * (RSG_TEAM_MAXB * 8);
+    t.phase ^= 1;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wsum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) buf[wid * NV + i] = v[i];
+    }
+    __syncthreads();
+    if (t.ltid == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += buf[w * NV + i];
+            gbuf[t.blk * NV + i] = s;
+        }
+    }
+    RSG_SYNC();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.0;
+        for (int b = 0; b < t.nblk; ++b) s += gbuf[b * NV + i];
+        v[i] = s;
+    }
+}
+template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
+    static_assert(NV <= 8, "team reductions carry at most 8 values");
+    const int lane = t.ltid & 63, wid = t.ltid >> 6, nw = (t.lbs + 63) >> 6;
+    double* buf = t.red + t.phase * 64;
+    double* gbuf = t.gred + t.phase * (RSG_TEAM_MAXB * 8);
+    t.phase ^= 1;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wmax(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) buf[wid * NV + i] = v[i];
+    }
+    __syncthreads();
+    if (t.ltid == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = buf[i];
+            for (int w = 1; w < nw; ++w) s = dmax(s, buf[w * NV + i]);
+            gbuf[t.blk * NV + i] = s;
+        }
+    }
+    RSG_SYNC();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = gbuf[i];
+        for (int b = 1; b < t.nblk; ++b) s = dmax(s, gbuf[b * NV + i]);
+        v[i] = s;
+    }
+}
+#endif  // RSG_TEAM
 #endif
 
 #define FOR_CELLS(c) for (int c = t.tid; c < nC; c += t.bs)
@@ -674,21 +755,15 @@ RSG_D void amg_refresh_values(Ctx& C, bool coarse) {
 // owning column j; LDS accesses of one wave execute in program order, so the
 // in-place elimination needs no workgroup barrier.  The inverse is written
 // to global memory for the V-cycle's coarse solve.
-RSG_D void amg_coarse_inverse(Ctx& C) {
-    Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg;
-    const int L = A.lvN[RSG_MAX_LEVELS];
-    const int n = A.lvN[L - 1];
+// The dense Gauss-Jordan inverse of the coarsest level in LDS by the threads
+// of one workgroup (ltid, lbs: their place in it); returns 0 on a zero pivot.
+RSG_D int coarse_dense_block(double* G, const int* rp, const int* ci, const double* v, int n, int ltid, int lbs) {
     const int DN = RSG_COARSE_DENSE;
-    if (n > DN) { if (t.tid == 0) M->sc->coarseDense = 0; RSG_SYNC(); return; }
-    const int* rp = A.rp + A.lvRp[L - 1];
-    const int* ci = A.ci + A.lvNz[L - 1];
-    const double* v = A.v + A.lvNz[L - 1];
-    double* G = t.lds;
-    for (int idx = t.tid; idx < n * DN; idx += t.bs) G[idx] = 0.0;
-    RSG_SYNC();
-    for (int i = t.tid; i < n; i += t.bs)
+    for (int idx = ltid; idx < n * DN; idx += lbs) G[idx] = 0.0;
+    RSG_BLOCK_SYNC();
+    for (int i = ltid; i < n; i += lbs)
         for (int e = rp[i]; e < rp[i + 1]; ++e) G[i * DN + ci[e]] += v[e];
-    RSG_SYNC();
+    RSG_BLOCK_SYNC();
     int ok = 1;
 #ifdef RSG_EMULATE
     for (int k = 0; k < n; ++k) {
@@ -706,7 +781,7 @@ RSG_D void amg_coarse_inverse(Ctx& C) {
     // factor G[i][k] is read before any lane of the (same) wave writes it; the
     // pivot row is read by every wave before the barrier and rewritten after.
     {
-        const int j = t.tid & 63, w = t.tid >> 6, nw = t.bs >> 6;
+        const int j = ltid & 63, w = ltid >> 6, nw = lbs >> 6;
         for (int k = 0; k < n; ++k) {
             const double piv = G[k * DN + k];
             if (!(fabs(piv) > 1e-300)) { ok = 0; break; }   // same value in every thread
@@ -726,9 +801,31 @@ RSG_D void amg_coarse_inverse(Ctx& C) {
         }
     }
 #endif
-    RSG_SYNC();
-    for (int idx = t.tid; idx < n * DN; idx += t.bs) A.cinv[idx] = G[idx];
+    RSG_BLOCK_SYNC();
+    return ok;
+}
+
+RSG_D void amg_coarse_inverse(Ctx& C) {
+    Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg;
+    const int L = A.lvN[RSG_MAX_LEVELS];
+    const int n = A.lvN[L - 1];
+    const int DN = RSG_COARSE_DENSE;
+    if (n > DN) { if (t.tid == 0) M->sc->coarseDense = 0; RSG_SYNC(); return; }
+    const int* rp = A.rp + A.lvRp[L - 1];
+    const int* ci = A.ci + A.lvNz[L - 1];
+    const double* v = A.v + A.lvNz[L - 1];
+#ifdef RSG_TEAM
+    // the first workgroup of the team factors; the others wait at the barrier
+    if (t.blk == 0) {
+        const int ok = coarse_dense_block(t.lds, rp, ci, v, n, t.ltid, t.lbs);
+        for (int idx = t.ltid; idx < n * DN; idx += t.lbs) A.cinv[idx] = t.lds[idx];
+        if (t.ltid == 0) M->sc->coarseDense = ok;
+    }
+#else
+    const int ok = coarse_dense_block(t.lds, rp, ci, v, n, t.tid, t.bs);
+    for (int idx = t.tid; idx < n * DN; idx += t.bs) A.cinv[idx] = t.lds[idx];
     if (t.tid == 0) M->sc->coarseDense = ok;
+#endif
     RSG_SYNC();
 }
 
@@ -1521,20 +1618,24 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
     C.tmark = wall_clock64();
 #endif
     // the coarsest dense inverse lives in LDS: restore it for this launch
+    // (a team reads it from global memory)
+#ifndef RSG_TEAM
     if (S.amgBuiltIter >= 0 && sc->coarseDense) {
         const RsgAmgDev& A = M->amg;
         const int nL = A.lvN[A.lvN[RSG_MAX_LEVELS] - 1];
         for (int idx = C.t.tid; idx < nL * RSG_COARSE_DENSE; idx += C.t.bs) C.t.lds[idx] = A.cinv[idx];
         RSG_SYNC();
     }
+#endif
 #ifndef RSG_EMULATE
     const long long tStart = wall_clock64();
 #endif
     for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) {
         simple_iteration<CMAX>(C, S);
-#ifndef RSG_EMULATE
+#if !defined(RSG_EMULATE) && !defined(RSG_TEAM)
         // time slice: thread 0 reads the clock and every thread follows its
-        // decision, so the workgroup leaves the loop together
+        // decision, so the workgroup leaves the loop together (a team runs
+        // its fixed iteration budget)
         if (M->sliceTicks > 0) {
             if (C.t.tid == 0) C.t.red[0] = (wall_clock64() - tStart >= M->sliceTicks) ? 1.0 : 0.0;
             RSG_SYNC();

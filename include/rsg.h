@@ -17,7 +17,7 @@
  *   rsg_fetch              FlowFields + ConvergenceHistory returned by solve
  *                          (core/include/FlowFields.hpp:9-49)
  *   rsg_ldu_solve          ILinearSolver::solve (core/include/LinearSolver.hpp:115-120)
- *   rsg_set_launch / rsg_set_time_slice / rsg_set_schedule
+ *   rsg_set_launch / rsg_set_time_slice / rsg_set_schedule / rsg_set_team
  *                          engine tuning (no reference counterpart: the reference
  *                          solves one member at a time)
  *   rsg_system_solve       AMGSolver::solve / PCGSolver::solve on one large system
@@ -228,6 +228,16 @@ int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds);
  * members on the largest meshes first (longest-processing-time order, which
  * shortens an ensemble's makespan; smaller members wait).  Results do not
  * depend on the policy. */
+/* Team solves: a member whose mesh has at least min_cells cells (0: never)
+ * runs alone on `blocks` 512-thread workgroups of one cooperative launch
+ * (0: as many as the device holds at once, at most 256), iters_per_launch
+ * outer iterations per launch -- the same SIMPLE code with its loops spread
+ * over the whole team, phase boundaries as grid-wide barriers and reductions
+ * combined in workgroup order.  For meshes far beyond one workgroup's reach
+ * (10^4-10^6 cells).  Results differ from the one-workgroup solve only by
+ * reduction-order round-off; time slices do not apply to team solves. */
+int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t iters_per_launch);
+
 #define RSG_SCHEDULE_ROUND_ROBIN 0
 #define RSG_SCHEDULE_LARGEST_FIRST 1
 int rsg_set_schedule(rsg_context* ctx, int32_t policy);

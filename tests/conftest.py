@@ -67,9 +67,11 @@ def emu():
             os.path.join(PKG, "csrc", "rsg_resident.h")]
     if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(s) for s in srcs):
         os.makedirs(out_dir, exist_ok=True)
+        tmp = "%s.%d" % (path, os.getpid())           # parallel workers: build aside, rename
         subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-shared", "-fPIC", "-w",
                         "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc"),
-                        srcs[0], "-o", path], check=True)
+                        srcs[0], "-o", tmp], check=True)
+        os.replace(tmp, path)
     return C.CDLL(path)
 
 

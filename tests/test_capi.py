@@ -69,5 +69,6 @@ def test_argument_errors_without_device(rsg_lib):
                                     A.dptr(np.ones(1)), A.dptr(np.zeros(1)), 0, 0, 10, C.c_double(1e-8),
                                     A.dptr(st)) == -1
     assert rsg_lib.rsg_problem_set_observations(None, 0, 0, None, 0, None, None) == -1
+    assert rsg_lib.rsg_set_team(None, 0, 0, 20) == -1
     assert rsg_lib.rsg_wall_distance(None, 1, A.dptr(np.zeros(3)), 1, A.dptr(np.zeros(3)), A.dptr(np.zeros(1)),
                                      None) == -1
