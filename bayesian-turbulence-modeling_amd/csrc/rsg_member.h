@@ -160,9 +160,11 @@ template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
 
 #else
 // Team sum / max: each workgroup reduces as above, its thread 0 stores
-// the workgroup's value, and after the grid barrier every thread combines the
-// workgroup values in workgroup order -- the same value in every thread of
-// the team.  Double-buffered like the workgroup version.
+// the workgroup's value, and after the grid barrier every wave combines the
+// workgroup values in one fixed order -- lane l takes workgroups l, l + 64,
+// ... in turn, then the xor butterfly across the lanes -- and takes lane 0's
+// result, so every thread of the team holds the same value (NaNs included).
+// Double-buffered like the workgroup version.
 template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
     static_assert(NV <= 8, "team reductions carry at most 8 values");
     const int lane = t.ltid & 63, wid = t.ltid >> 6, nw = (t.lbs + 63) >> 6;
@@ -188,8 +190,8 @@ template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         double s = 0.0;
-        for (int b = 0; b < t.nblk; ++b) s += gbuf[b * NV + i];
-        v[i] = s;
+        for (int b = lane; b < t.nblk; b += 64) s += gbuf[b * NV + i];
+        v[i] = __shfl(wsum(s), 0, 64);
     }
 }
 template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
@@ -216,9 +218,9 @@ template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
     RSG_SYNC();
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        double s = gbuf[i];
-        for (int b = 1; b < t.nblk; ++b) s = dmax(s, gbuf[b * NV + i]);
-        v[i] = s;
+        double s = gbuf[i];   // workgroup 0's value: lanes past the last workgroup keep it
+        for (int b = lane; b < t.nblk; b += 64) s = dmax(s, gbuf[b * NV + i]);
+        v[i] = __shfl(wmax(s), 0, 64);
     }
 }
 #endif  // RSG_TEAM

@@ -376,6 +376,11 @@ PYBIND11_MODULE(rans_sst_py, m) {
 
     m.def("engine_set_launch", [](int iters, int block) { Engine::get().setLaunch(iters, block); },
           py::arg("iters_per_launch"), py::arg("block_size") = 0);
+    m.def("engine_set_team", [](int minCells, int blocks, int iters) { Engine::get().setTeam(minCells, blocks, iters); },
+          py::arg("min_cells"), py::arg("blocks") = 0, py::arg("iters_per_launch") = 20,
+          "Members on meshes of at least min_cells cells are solved by a team of `blocks` workgroups "
+          "(0: every CU) in cooperative launches of iters_per_launch outer iterations; min_cells 0 turns "
+          "team solves off.");
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
     m.def("engine_set_schedule", [](const std::string& p) {
               if (p == "round_robin") Engine::get().setSchedule(RSG_SCHEDULE_ROUND_ROBIN);

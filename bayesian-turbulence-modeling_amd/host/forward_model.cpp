@@ -56,6 +56,12 @@ void Engine::setTimeSlice(int microseconds) {
     if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
 }
 
+void Engine::setTeam(int minCells, int blocks, int itersPerLaunch) {
+    rsg_context* c = ctx();
+    int rc = rsg_set_team(c, minCells, blocks, itersPerLaunch);
+    if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
+}
+
 std::vector<double> Engine::stats() {
     std::vector<double> s(5, 0.0);
     rsg_context* c = ctx();

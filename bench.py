@@ -21,6 +21,8 @@ cost per outer iteration, applied to this run's iteration counts.
 pressure system (rsg_system_solve, AMG-PCG to 1e-10, FP64 and FP32-inner with
 FP64 refinement) next to the reference's CPU PCG on the same system.
 --workload walldist times the device wall-distance kernel on a 2M-cell mesh.
+--workload bump times team solves (one member on the whole device) of the
+periodic bump channel at 40k, 160k and 2M cells per outer iteration.
 """
 import argparse
 import glob
@@ -361,6 +363,111 @@ def bench_walldist(args):
 C4P_GRID = dict(nx_up=415, nx_dn=1245, ny_up=553, ny_low=111)   # 2,072,510 cells
 
 
+def bump_mesh(rs, nx, ny, length=8.0, height=2.0, bump_h=0.3, bump_w=1.0, re=1667.0):
+    """Streamwise-periodic channel (walls y = y_b(x) and y = height) with a
+    Gaussian bump of height bump_h and width bump_w at mid-length on the lower
+    wall, nx x ny quads stored unstructured (the curved periodic factory,
+    tanh wall clustering for y+ = 1 at Re)."""
+    x = np.linspace(0.0, length, nx + 1)
+    g = np.exp(-((x - 0.5 * length) / bump_w) ** 2)
+    yb = bump_h * (g - g[0])                    # exactly periodic: zero at both ends
+    yb[-1] = yb[0]
+    return rs.Mesh.make_curved_channel_periodic_2d(x, yb, height, ny, re, 1.0)
+
+
+def bench_bump(args):
+    """Team solves of single large meshes: the periodic bump channel at 40k,
+    160k and 2M cells (nu = 6e-4, unit body force: Re_tau ~ 1670, bump-height
+    Reynolds number ~ 1e4 at the bulk velocity), SIMPLE + SST with the AMG-PCG
+    pressure solve, the whole device on one member (rsg_set_team, cooperative
+    launches of 20 outer iterations).  Each size is solved cold for K1 and for
+    K1 + K outer iterations; the difference gives the cost of one outer
+    iteration without set-up (mesh upload, AMG aggregation on the device).  At
+    40k cells the one-workgroup member engine is timed the same way."""
+    import rans_sst_py as rs
+    import ensembles
+    nu = 6e-4
+    sizes = [(200, 200, 40), (400, 400, 40), (2000, 1000, 20)]
+    if args.bump_sizes:
+        keep = {int(s) for s in args.bump_sizes.split(",")}
+        sizes = [z for z in sizes if z[0] * z[1] in keep]
+    ps = rs.InferenceParameterSet.from_indices("c2_five", ensembles.THETA_INDICES)
+    obs = rs.ObservationOperator()
+    rs.engine_set_launch(20, 0)
+    rs.engine_set_time_slice(0)
+
+    def run(mesh, iters, team, blocks=0):
+        rs.engine_set_team(20000 if team else 0, blocks, 20)
+        s = rs.SolverSettings()
+        s.max_iterations = iters
+        s.convergence_tol = 1e-12
+        s.body_force = rs.Vec3(1.0, 0.0, 0.0)
+        s.verbose = False
+        bcs = rs.FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+        fm = rs.ForwardModel(mesh, ps, obs, bcs, nu, s)
+        fm.clear_cache()
+        st0 = rs.engine_stats()
+        t0 = time.perf_counter()
+        r = fm.evaluate(list(ensembles.THETA_DEFAULTS))
+        dt = time.perf_counter() - t0
+        st1 = rs.engine_stats()
+        inner = fm.last_inner_iterations()
+        return dict(wall=dt, kernel=(st1[3] - st0[3]) * 1e-3, iters=int(r.simple_iters),
+                    status=str(r.status).split(".")[-1], inner=inner)
+
+    rows = []
+    for nx, ny, K in sizes:
+        t0 = time.perf_counter()
+        mesh = bump_mesh(rs, nx, ny)
+        mesh.compute_wall_distance(True)
+        n, nif = mesh.n_cells(), mesh.n_internal_faces()
+        nb = mesh.n_faces() - nif
+        print("bench: bump %d x %d = %d cells, mesh %.1f s" % (nx, ny, n, time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+        blocks = [int(b) for b in args.team_blocks.split(",")]
+        legs = [("team" if b == 0 else "team_%d" % b, True, b) for b in blocks]
+        if n <= 40000 and not args.no_member:
+            legs.append(("member", False, 0))
+        row = {"cells": n, "internal_faces": nif, "nx": nx, "ny": ny}
+        for name, team, nb_team in legs:
+            k1 = 20
+            a = run(mesh, k1, team, nb_team)
+            print("bench: bump %d %s %d iterations: %.2f s wall, %.2f s kernel (%s)"
+                  % (n, name, a["iters"], a["wall"], a["kernel"], a["status"]), file=sys.stderr, flush=True)
+            b = run(mesh, k1 + K, team, nb_team)
+            print("bench: bump %d %s %d iterations: %.2f s wall, %.2f s kernel (%s)"
+                  % (n, name, b["iters"], b["wall"], b["kernel"], b["status"]), file=sys.stderr, flush=True)
+            di = b["iters"] - a["iters"]
+            leg = {"outer_iters": [a["iters"], b["iters"]], "status": b["status"]}
+            if di > 0:
+                dk = b["kernel"] - a["kernel"]
+                din = [x - y for x, y in zip(b["inner"], a["inner"])]
+                byt = algorithmic_bytes(n, nif, nb, di, din)
+                leg.update(ms_per_outer_iter_wall=1e3 * (b["wall"] - a["wall"]) / di,
+                           ms_per_outer_iter_kernel=1e3 * dk / di,
+                           inner_per_outer=[x / di for x in din],
+                           algorithmic_GBps=byt / dk * 1e-9 if dk > 0 else None,
+                           setup_s=a["wall"] - k1 * (b["wall"] - a["wall"]) / di)
+            row[name] = leg
+        rows.append(row)
+    big = rows[-1]
+    teams = [v for k, v in big.items() if k.startswith("team") and v.get("ms_per_outer_iter_kernel")]
+    best = min(teams, key=lambda v: v["ms_per_outer_iter_kernel"]) if teams else {}
+    t = best.get("ms_per_outer_iter_kernel")
+    gbs = best.get("algorithmic_GBps")
+    print(json.dumps({
+        "metric": "outer SIMPLE+SST iterations/s of one large-mesh member (team solve)",
+        "value": (1e3 / t) if t else None, "unit": "outer_iters/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t, "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "periodic Gaussian-bump channel (bump 0.3 x 1, length 8, height 2), "
+                               "%d cells, nu 6e-4, AMG-PCG pressure, team of every CU" % big["cells"]},
+        "sizes": rows,
+        "roofline": ({"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": gbs / HBM_PEAK_GBS, "traffic": None} if gbs else None),
+        "cpu_baseline": None}), flush=True)
+
+
 def bench_c1(args, big=False):
     """The large single system: solves/s of AMG-PCG to 1e-10 (host hierarchy
     setup + device solve + host<->device copies, wall clock), FP64 and mixed.
@@ -461,10 +568,15 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
     ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump"), default="c2",
                     help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
                          "c0: single-solve latency of the Re_tau 550 channel; walldist: the wall-distance "
-                         "kernel on a 2M-cell mesh; c4p: the pressure system at 2M cells")
+                         "kernel on a 2M-cell mesh; c4p: the pressure system at 2M cells; bump: team solves "
+                         "of one bump-channel member at 40k, 160k and 2M cells")
+    ap.add_argument("--bump-sizes", default="", help="comma-separated cell counts of the bump workload to run")
+    ap.add_argument("--team-blocks", default="0",
+                    help="bump workload: comma-separated team sizes in workgroups (0: every CU)")
+    ap.add_argument("--no-member", action="store_true", help="bump workload: skip the one-workgroup leg")
     ap.add_argument("--max-iterations", type=int, default=200000,
                     help="outer-iteration cap per member (profiling runs shorten it)")
     args = ap.parse_args()
@@ -486,6 +598,9 @@ def main():
         return
     if args.workload == "walldist":
         bench_walldist(args)
+        return
+    if args.workload == "bump":
+        bench_bump(args)
         return
     import rans_sst_py as rs
     import ensembles

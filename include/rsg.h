@@ -230,10 +230,10 @@ int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds);
  * depend on the policy. */
 /* Team solves: a member whose mesh has at least min_cells cells (0: never)
  * runs alone on `blocks` 512-thread workgroups of one cooperative launch
- * (0: as many as the device holds at once, at most 256), iters_per_launch
- * outer iterations per launch -- the same SIMPLE code with its loops spread
- * over the whole team, phase boundaries as grid-wide barriers and reductions
- * combined in workgroup order.  For meshes far beyond one workgroup's reach
+ * (0: one workgroup per 4096 cells, at least 16, at most what the device holds
+ * at once and 256), iters_per_launch outer iterations per launch -- the same
+ * SIMPLE code with its loops spread over the whole team, phase boundaries as
+ * grid-wide barriers and reductions combined in one fixed order.  For meshes far beyond one workgroup's reach
  * (10^4-10^6 cells).  Results differ from the one-workgroup solve only by
  * reduction-order round-off; time slices do not apply to team solves. */
 int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t iters_per_launch);

@@ -64,3 +64,42 @@ def test_team_arguments(rsg_lib, gpu_ctx):
     assert rsg_lib.rsg_set_team(gpu_ctx, 100, 257, 20) == -1
     assert rsg_lib.rsg_set_team(gpu_ctx, 100, 0, 0) == -1
     assert rsg_lib.rsg_set_team(gpu_ctx, 0, 0, 20) == 0
+
+
+def _bump_mesh(rs, nx, ny):
+    """Periodic channel (height 2, length 8) with a Gaussian bump 0.3 high and
+    1 wide on the lower wall, nx x ny quads."""
+    x = np.linspace(0.0, 8.0, nx + 1)
+    g = np.exp(-((x - 4.0) / 1.0) ** 2)
+    yb = 0.3 * (g - g[0])
+    yb[-1] = yb[0]
+    return rs.Mesh.make_curved_channel_periodic_2d(x, yb, 2.0, ny, 1667.0, 1.0)
+
+
+def test_team_drop_in_matches_member(rs):
+    """rans_sst_py.engine_set_team: a 16,000-cell bump channel solved by an
+    automatically sized team (16 workgroups) through ForwardModel.evaluate
+    matches the one-workgroup solve, AMG pressure solve included."""
+    mesh = _bump_mesh(rs, 160, 100)
+    mesh.compute_wall_distance(True)
+    s = rs.SolverSettings()
+    s.max_iterations = 60; s.convergence_tol = 1e-12; s.verbose = False
+    s.body_force = rs.Vec3(1.0, 0.0, 0.0)
+    bcs = rs.FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+    ps = rs.InferenceParameterSet.a1_betaStar()
+    out = {}
+    for name, min_cells in (("member", 0), ("team", 10000)):
+        rs.engine_set_team(min_cells, 0, 20)
+        try:
+            print("drop-in bump 160x100", name, flush=True)
+            fm = rs.ForwardModel(mesh, ps, rs.ObservationOperator(), bcs, 6e-4, s)
+            r = fm.evaluate([0.31, 0.09])
+            out[name] = (r.simple_iters, str(r.status).split(".")[-1], fm.last_fields())
+        finally:
+            rs.engine_set_team(0, 0, 20)
+    assert out["team"][0] == out["member"][0] == 60
+    assert out["team"][1] == out["member"][1] == "Unconverged"
+    for k in ("U", "k", "omega", "nuT"):
+        a, b = out["team"][2][k].ravel(), out["member"][2][k].ravel()
+        assert np.all(np.isfinite(a))
+        assert rel(a, b) < 1e-6, (k, rel(a, b))
