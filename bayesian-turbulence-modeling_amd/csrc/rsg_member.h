@@ -450,6 +450,26 @@ RSG_D double ldu_row(const RsgMeshDev* m, const double* diag, const double* uppe
                      const double* lower, const double* x, int c) {
     double y = diag[c] * x[c];
     const int nIF = m->nIF;
+    if (m->ellK) {
+        // every internal face of the cell sits in its ELL slots, in ascending
+        // face order: the same terms added in the same order as the face-list
+        // loop below, with every slot's loads issued before the first add
+        const int nC = m->nC;
+        int code[RSG_ELL_K], nb[RSG_ELL_K];
+        double cf[RSG_ELL_K], xv[RSG_ELL_K];
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k) { code[k] = m->ellFace[k * nC + c]; nb[k] = m->ellNb[k * nC + c]; }
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k) {
+            const int f = code[k] >> 1;
+            cf[k] = code[k] < 0 ? 0.0 : ((code[k] & 1) ? lower[f] : upper[f]);
+            xv[k] = x[nb[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k)
+            if (code[k] >= 0) y += cf[k] * xv[k];
+        return y;
+    }
     for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
         const int code = m->cfIdx[e];
         const int f = code >> 1;
@@ -834,13 +854,34 @@ RSG_D void amg_coarse_inverse(Ctx& C) {
 // V-cycle preconditioner z = M^{-1} r (AMGSolver::vcycle :553-589 with
 // damped-Jacobi smoothing in place of the sequential Gauss-Seidel sweep, and
 // an exact dense coarsest solve in place of 50 Gauss-Seidel sweeps).
+// s -/+ sum over CSR entries e0..e1-1 of v[e] x[ci[e]], term by term in CSR
+// order (the same roundings as the plain loop); the loads of four entries
+// are issued together.
+template <bool SUB>
+RSG_D double csr_row_acc(const int* ci, const double* v, const double* x, int e0, int e1, double s) {
+    for (int e = e0; e < e1; e += 4) {
+        int c[4]; double a[4], xv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool ok = e + k < e1;
+            c[k] = ok ? ci[e + k] : 0;
+            a[k] = ok ? v[e + k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) xv[k] = (e + k < e1) ? x[c[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (e + k < e1) { if (SUB) s -= a[k] * xv[k]; else s += a[k] * xv[k]; }
+    }
+    return s;
+}
+
 RSG_D void csr_jacobi(Thr& t, int n, const int* rp, const int* ci, const double* v, const double* dg,
                       const double* b, const double* xin, double* xout, bool xzero) {
     const double wJ = 2.0 / 3.0;
     for (int i = t.tid; i < n; i += t.bs) {
         double s = b[i];
-        if (!xzero)
-            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xin[ci[e]];
+        if (!xzero) s = csr_row_acc<true>(ci, v, xin, rp[i], rp[i + 1], s);
         const double d = dg[i];
         const double x0 = xzero ? 0.0 : xin[i];
         xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
@@ -851,8 +892,7 @@ RSG_D void csr_jacobi(Thr& t, int n, const int* rp, const int* ci, const double*
 // CSR order.
 RSG_D double csr_row_residual(const int* rp, const int* ci, const double* v, const double* b, const double* x,
                               int i) {
-    double s = 0.0;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+    const double s = csr_row_acc<false>(ci, v, x, rp[i], rp[i + 1], 0.0);
     return b[i] - s;
 }
 
