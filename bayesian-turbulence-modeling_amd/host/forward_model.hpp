@@ -40,7 +40,7 @@ public:
     void setTimeSlice(int microseconds);
     void setSchedule(int policy);
     // members on meshes of at least minCells cells run as teams of `blocks`
-    // workgroups (0: the whole device); minCells 0 turns team solves off
+    // workgroups (0: sized by the cell count); minCells 0 turns team solves off
     void setTeam(int minCells, int blocks, int itersPerLaunch);
     std::vector<double> stats();
 private:

@@ -575,7 +575,7 @@ def main():
                          "of one bump-channel member at 40k, 160k and 2M cells")
     ap.add_argument("--bump-sizes", default="", help="comma-separated cell counts of the bump workload to run")
     ap.add_argument("--team-blocks", default="0",
-                    help="bump workload: comma-separated team sizes in workgroups (0: every CU)")
+                    help="bump workload: comma-separated team sizes in workgroups (0: the engine's choice)")
     ap.add_argument("--no-member", action="store_true", help="bump workload: skip the one-workgroup leg")
     ap.add_argument("--max-iterations", type=int, default=200000,
                     help="outer-iteration cap per member (profiling runs shorten it)")
