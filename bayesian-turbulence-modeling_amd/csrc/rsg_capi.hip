@@ -42,13 +42,14 @@ hipError_t rsg_team_launch(RsgMemberDev* dM, RsgScalars* dOut, double* gred, int
 // ------------------------------------------------------------------ kernels
 namespace {
 
-// Widest cells-per-thread instantiation of the resident solvers for a block
-// size (it bounds the kernel's register allocation; RSG_RES_MAXN cells at most).
-// team sizing when rsg_set_team's blocks is 0 (measured on bump channels of
+// Team sizing when rsg_set_team's blocks is 0 (measured on bump channels of
 // 40k and 160k cells: 16-40 workgroups ran an outer iteration 2.6-3x faster
-// than all 256)
+// than all 256).
 constexpr int kTeamCellsPerBlock = 4096;
 constexpr int kTeamMinBlocks = 16;
+
+// Widest cells-per-thread instantiation of the resident solvers for a block
+// size (it bounds the kernel's register allocation; RSG_RES_MAXN cells at most).
 constexpr int cmax_for(int bs) { return bs >= 1024 ? 1 : (bs >= 512 ? 3 : 6); }
 
 // Workgroup LDS: reduction scratch, coarsest dense inverse, the resident
@@ -288,7 +289,8 @@ struct rsg_context {
     int sliceUs = 0;         // > 0: each launch runs every member for this long (no iteration cap)
     int clockKHz = 0;        // device wall clock (wall_clock64) frequency
     // team solves (rsg_set_team): members on meshes of at least teamMinCells
-    // cells run alone on teamBlocks workgroups, teamIters outer iterations a launch
+    // cells run alone on teamBlocks workgroups (0: sized by the cell count),
+    // teamIters outer iterations a launch
     int teamMinCells = 0, teamBlocks = 0, teamIters = 20, teamCap = 0;
     double* dGred = nullptr;
     RsgMemberDev* dTeamM = nullptr;
@@ -531,7 +533,6 @@ void engine_loop(rsg_context* ctx) {
                                                  : std::max(kTeamMinBlocks, (nC + kTeamCellsPerBlock - 1) / kTeamCellsPerBlock);
             teamNb[q] = std::min(want, ctx->teamCap);
         }
-        const int teamBlocks = ctx->teamCap;
         ctx->hMembers.resize(nr);
         ctx->hStatus.resize(nr);
         for (int i = 0; i < nr; ++i) {
@@ -548,7 +549,7 @@ void engine_loop(rsg_context* ctx) {
         lk.unlock();
         hipError_t e = hipSuccess;
         float ms = 0.f;
-        if (!team.empty() && teamBlocks <= 0) e = hipErrorInvalidConfiguration;
+        if (!team.empty() && ctx->teamCap <= 0) e = hipErrorInvalidConfiguration;
         if (!team.empty() && e == hipSuccess && !ctx->dGred) {
             e = hipMalloc(&ctx->dGred, sizeof(double) * 2 * RSG_TEAM_MAXB * 8);
             if (e == hipSuccess) e = hipMalloc(&ctx->dTeamM, sizeof(RsgMemberDev));

@@ -21,7 +21,7 @@ cost per outer iteration, applied to this run's iteration counts.
 pressure system (rsg_system_solve, AMG-PCG to 1e-10, FP64 and FP32-inner with
 FP64 refinement) next to the reference's CPU PCG on the same system.
 --workload walldist times the device wall-distance kernel on a 2M-cell mesh.
---workload bump times team solves (one member on the whole device) of the
+--workload bump times team solves (one member on a team of workgroups) of the
 periodic bump channel at 40k, 160k and 2M cells per outer iteration.
 """
 import argparse
@@ -379,10 +379,11 @@ def bench_bump(args):
     """Team solves of single large meshes: the periodic bump channel at 40k,
     160k and 2M cells (nu = 6e-4, unit body force: Re_tau ~ 1670, bump-height
     Reynolds number ~ 1e4 at the bulk velocity), SIMPLE + SST with the AMG-PCG
-    pressure solve, the whole device on one member (rsg_set_team, cooperative
-    launches of 20 outer iterations).  Each size is solved cold for K1 and for
-    K1 + K outer iterations; the difference gives the cost of one outer
-    iteration without set-up (mesh upload, AMG aggregation on the device).  At
+    pressure solve, one member on a team of workgroups (rsg_set_team: the
+    engine's size unless --team-blocks names others; cooperative launches of
+    20 outer iterations).  Each size is solved cold for K1 and for K1 + K outer
+    iterations; the difference gives the cost of one outer iteration without
+    the cold start (mesh upload, initialisation, AMG set-up on the device).  At
     40k cells the one-workgroup member engine is timed the same way."""
     import rans_sst_py as rs
     import ensembles
