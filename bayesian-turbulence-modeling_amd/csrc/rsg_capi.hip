@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
@@ -42,10 +43,12 @@ hipError_t rsg_team_launch(RsgMemberDev* dM, RsgScalars* dOut, double* gred, int
 // ------------------------------------------------------------------ kernels
 namespace {
 
-// Team sizing when rsg_set_team's blocks is 0 (measured on bump channels of
-// 40k and 160k cells: 16-40 workgroups ran an outer iteration 2.6-3x faster
-// than all 256).
-constexpr int kTeamCellsPerBlock = 4096;
+// Team sizing when rsg_set_team's blocks is 0: a phase's work falls as
+// cells / workgroups while its grid barrier costs more with every workgroup,
+// so the best team grows as the square root of the cell count (measured on
+// bump channels: 16 workgroups best at 40k cells, 32-40 at 160k, 128 ahead of
+// 256 at 2M).
+constexpr double kTeamBlocksAt40k = 16.0;
 constexpr int kTeamMinBlocks = 16;
 
 // Widest cells-per-thread instantiation of the resident solvers for a block
@@ -524,13 +527,12 @@ void engine_loop(rsg_context* ctx) {
             T.M.iterBudget = ctx->teamIters;
             T.M.sliceTicks = 0;
             teamM[q] = T.M;
-            // team size: the set one, else one workgroup per kTeamCellsPerBlock
-            // cells (at least kTeamMinBlocks): every grid barrier and team
-            // reduction costs more the more workgroups take part, so a team
-            // is no larger than its cell loops need
+            // team size: the set one, else 16 sqrt(cells / 40,000) workgroups
+            // (at least kTeamMinBlocks)
             const int nC = ctx->meshes[ctx->problems[T.problemId]->meshId]->prep.nC;
-            const int want = ctx->teamBlocks > 0 ? ctx->teamBlocks
-                                                 : std::max(kTeamMinBlocks, (nC + kTeamCellsPerBlock - 1) / kTeamCellsPerBlock);
+            const int want = ctx->teamBlocks > 0
+                                 ? ctx->teamBlocks
+                                 : std::max(kTeamMinBlocks, (int)std::lround(kTeamBlocksAt40k * std::sqrt(nC / 40000.0)));
             teamNb[q] = std::min(want, ctx->teamCap);
         }
         ctx->hMembers.resize(nr);

@@ -379,7 +379,7 @@ PYBIND11_MODULE(rans_sst_py, m) {
     m.def("engine_set_team", [](int minCells, int blocks, int iters) { Engine::get().setTeam(minCells, blocks, iters); },
           py::arg("min_cells"), py::arg("blocks") = 0, py::arg("iters_per_launch") = 20,
           "Members on meshes of at least min_cells cells are solved by a team of `blocks` workgroups "
-          "(0: one per 4096 cells, at least 16) in cooperative launches of iters_per_launch outer iterations; min_cells 0 turns "
+          "(0: 16 sqrt(cells / 40000), at least 16) in cooperative launches of iters_per_launch outer iterations; min_cells 0 turns "
           "team solves off.");
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
     m.def("engine_set_schedule", [](const std::string& p) {

@@ -230,7 +230,7 @@ int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds);
  * depend on the policy. */
 /* Team solves: a member whose mesh has at least min_cells cells (0: never)
  * runs alone on `blocks` 512-thread workgroups of one cooperative launch
- * (0: one workgroup per 4096 cells, at least 16, at most what the device holds
+ * (0: 16 sqrt(cells / 40000) workgroups, at least 16, at most what the device holds
  * at once and 256), iters_per_launch outer iterations per launch -- the same
  * SIMPLE code with its loops spread over the whole team, phase boundaries as
  * grid-wide barriers and reductions combined in one fixed order.  For meshes far beyond one workgroup's reach
