@@ -51,144 +51,36 @@ namespace {
 constexpr double kTeamBlocksAt40k = 16.0;
 constexpr int kTeamMinBlocks = 16;
 
-// Widest cells-per-thread instantiation of the resident solvers for a block
-// size (it bounds the kernel's register allocation; RSG_RES_MAXN cells at most).
-constexpr int cmax_for(int bs) { return bs >= 1024 ? 1 : (bs >= 512 ? 3 : 6); }
+}  // namespace
 
-// Workgroup LDS: reduction scratch, coarsest dense inverse, the resident
-// solvers' two exchange buffers and the coarse-level x (~69 KB, so two
-// member workgroups share a CU).
-struct LdsArea {
-    double red[128];
-    double lds[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
-    double xb[2 * RSG_RES_MAXN];
-    double cv[RSG_RES_MAXC];
-};
-// One member per CU: room for LDS copies of the coarse AMG levels (~150 KB).
-struct LdsAreaWide {
-    LdsArea a;
-    double ceVal[RSG_RES_CE];
-    short ceCol[RSG_RES_CE];
-    short caCol[RSG_RES_CA];
-};
+// rsg_chunk.hip, one object per (workgroup size, precision): the member kernels
+hipError_t rsg_chunk_launch_256_0(int, RsgMemberDev*, RsgScalars*, hipStream_t);
+hipError_t rsg_chunk_launch_256_1(int, RsgMemberDev*, RsgScalars*, hipStream_t);
+hipError_t rsg_chunk_launch_512_0(int, RsgMemberDev*, RsgScalars*, hipStream_t);
+hipError_t rsg_chunk_launch_512_1(int, RsgMemberDev*, RsgScalars*, hipStream_t);
+int rsg_chunk_occupancy_256_0();
+int rsg_chunk_occupancy_256_1();
+int rsg_chunk_occupancy_512_0();
+int rsg_chunk_occupancy_512_1();
+hipError_t rsg_ldu_launch(RsgMemberDev* M, int solver, int maxIter, double tol, double* stats, hipStream_t s);
 
-__device__ rsg::Thr make_thr(LdsArea& L, int bs) {
-    rsg::Thr t;
-    t.tid = threadIdx.x; t.bs = bs; t.red = L.red; t.lds = L.lds; t.phase = 0;
-    t.ltid = threadIdx.x; t.lbs = bs; t.blk = 0; t.nblk = 1; t.gred = nullptr;
-    t.xb0 = L.xb; t.xb1 = L.xb + RSG_RES_MAXN; t.xsel = 0;
-    t.cvec = L.cv; t.cvecCap = RSG_RES_MAXC;
-    t.ceVal = nullptr; t.ceCol = nullptr; t.ceCap = 0; t.caCol = nullptr; t.caCap = 0;
-    return t;
+static hipError_t rsg_chunk_launch(int bs, int prec, int grid, RsgMemberDev* members, RsgScalars* statusOut,
+                                   hipStream_t s) {
+    if (bs == 512) return prec ? rsg_chunk_launch_512_1(grid, members, statusOut, s)
+                               : rsg_chunk_launch_512_0(grid, members, statusOut, s);
+    return prec ? rsg_chunk_launch_256_1(grid, members, statusOut, s)
+                : rsg_chunk_launch_256_0(grid, members, statusOut, s);
 }
 
-__device__ rsg::Thr make_thr(LdsAreaWide& L, int bs) {
-    rsg::Thr t = make_thr(L.a, bs);
-    t.ceVal = L.ceVal; t.ceCol = L.ceCol; t.ceCap = RSG_RES_CE;
-    t.caCol = L.caCol; t.caCap = RSG_RES_CA;
-    return t;
+// Member workgroups of one shape a CU holds at once (the smaller of the two
+// precisions' kernels).
+static int rsg_chunk_occupancy(int bs) {
+    const int a = bs == 512 ? rsg_chunk_occupancy_512_0() : rsg_chunk_occupancy_256_0();
+    const int b = bs == 512 ? rsg_chunk_occupancy_512_1() : rsg_chunk_occupancy_256_1();
+    return std::min(a, b);
 }
 
-// 256 threads: two members per CU share the LDS; 512: one member per CU.
-template <int BS> struct LdsFor { typedef LdsArea type; };
-template <> struct LdsFor<512> { typedef LdsAreaWide type; };
-
-// The member's descriptors (every phase dereferences them) and its AMG
-// level tables, copied into LDS for the launch so that these lookups cost an
-// LDS access instead of an L2 round trip.  The tables are written by the
-// hierarchy setup and copied back at the end of the launch.
-constexpr int kLvTabs = 8;   // lvN lvRow lvRp lvNz lvK lvE lvAK lvAE
-struct LdsDesc {
-    RsgMemberDev M;
-    RsgMeshDev mesh;
-    RsgBcDev bc;
-    RsgSettingsDev set;
-    int lv[kLvTabs][RSG_MAX_LEVELS + 1];
-    int used[2];
-    int* lvG[kLvTabs];       // the tables' global homes
-    int* usedG;
-};
-
-__device__ void copy_words(void* dst, const void* src, int bytes) {
-    int* d = static_cast<int*>(dst);
-    const int* s = static_cast<const int*>(src);
-    for (int i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
-}
-
-__device__ RsgMemberDev* stage_descriptors(LdsDesc& D, const RsgMemberDev* Mg) {
-    static_assert(sizeof(RsgMemberDev) % 4 == 0 && sizeof(RsgMeshDev) % 4 == 0 &&
-                  sizeof(RsgBcDev) % 4 == 0 && sizeof(RsgSettingsDev) % 4 == 0, "word-sized descriptors");
-    copy_words(&D.M, Mg, sizeof(RsgMemberDev));
-    __syncthreads();
-    // a stand-alone linear solve (k_ldu_solve) has no BCs or settings: null pointers stay null
-    copy_words(&D.mesh, D.M.mesh, sizeof(RsgMeshDev));
-    if (D.M.bc) copy_words(&D.bc, D.M.bc, sizeof(RsgBcDev));
-    if (D.M.set) copy_words(&D.set, D.M.set, sizeof(RsgSettingsDev));
-    RsgAmgDev& A = D.M.amg;
-    int* g[kLvTabs] = {A.lvN, A.lvRow, A.lvRp, A.lvNz, A.lvK, A.lvE, A.lvAK, A.lvAE};
-    for (int q = 0; q < kLvTabs; ++q) copy_words(D.lv[q], g[q], 4 * (RSG_MAX_LEVELS + 1));
-    copy_words(D.used, A.used, 8);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < kLvTabs; ++q) D.lvG[q] = g[q];
-        D.usedG = A.used;
-        D.M.mesh = &D.mesh;
-        if (D.M.bc) D.M.bc = &D.bc;
-        if (D.M.set) D.M.set = &D.set;
-        A.lvN = D.lv[0]; A.lvRow = D.lv[1]; A.lvRp = D.lv[2]; A.lvNz = D.lv[3];
-        A.lvK = D.lv[4]; A.lvE = D.lv[5]; A.lvAK = D.lv[6]; A.lvAE = D.lv[7];
-        A.used = D.used;
-    }
-    __syncthreads();
-    return &D.M;
-}
-
-__device__ void unstage_tables(LdsDesc& D) {
-    __syncthreads();
-    for (int q = 0; q < kLvTabs; ++q) copy_words(D.lvG[q], D.lv[q], 4 * (RSG_MAX_LEVELS + 1));
-    copy_words(D.usedG, D.used, 8);
-}
-
-template <int BS>
-__global__ __launch_bounds__(BS) void k_simple_chunk(RsgMemberDev* members, RsgScalars* statusOut) {
-    __shared__ typename LdsFor<BS>::type L;
-    __shared__ LdsDesc D;
-    rsg::Thr t = make_thr(L, BS);
-#ifdef RSG_PROF
-    const unsigned long long t0 = wall_clock64();
-#endif
-    RsgMemberDev* M = stage_descriptors(D, members + blockIdx.x);
-    rsg::run_member<cmax_for(BS)>(t, M);
-    unstage_tables(D);
-#ifdef RSG_PROF
-    if (threadIdx.x == 0 && blockIdx.x < 4096) {
-        rsg_prof_wg[3 * blockIdx.x] = t0;
-        rsg_prof_wg[3 * blockIdx.x + 1] = wall_clock64();
-        rsg_prof_wg[3 * blockIdx.x + 2] = (unsigned long long)__smid();
-    }
-#endif
-    __syncthreads();
-    if (threadIdx.x == 0) statusOut[blockIdx.x] = *M->sc;
-}
-
-// One linear solve on the LDU system already stored in the workspace.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_ldu_solve(RsgMemberDev* Mg, int solver, int maxIter, double tol,
-                                                  double* stats) {
-    __shared__ LdsArea L;
-    __shared__ LdsDesc D;
-    rsg::Thr t = make_thr(L, BS);
-    RsgMemberDev* M = stage_descriptors(D, Mg);
-    int nl = 0;
-    rsg::SolveRes r = rsg::ldu_solve_member<cmax_for(BS)>(t, M, solver, maxIter, tol, &nl);
-    unstage_tables(D);
-    if (t.tid == 0) {
-        stats[0] = r.iterations; stats[1] = r.finalRes; stats[2] = r.initialRes; stats[3] = r.converged;
-        stats[4] = nl;
-    }
-}
-
-typedef void (*ChunkFn)(RsgMemberDev*, RsgScalars*);
+namespace {
 
 // --------------------------------------------------------------- arena
 struct Arena {
@@ -336,12 +228,6 @@ int set_err(rsg_context* c, int code, const std::string& msg) {
 constexpr size_t kMeshStructSlot = (sizeof(RsgMeshDev) + 255) & ~size_t(255);
 static_assert(sizeof(RsgBcDev) <= 256, "problem staging reserves 256 bytes for RsgBcDev");
 
-// 256 threads: two member workgroups per CU (256 VGPRs, ~75 KB LDS each);
-// 512 threads: one per CU, for the lowest single-member latency.
-ChunkFn chunk_kernel(int bs) {
-    return bs == 512 ? k_simple_chunk<512> : k_simple_chunk<256>;
-}
-
 // host pointers of the result region of a member workspace
 void member_init_scalars(RsgScalars& s, int turbEstablished) {
     memset(&s, 0, sizeof s);
@@ -378,6 +264,7 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     for (int i = 0; i < 11; ++i) M.coeffs[i] = T.coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = T.init[i];
     M.initFromFields = T.warm;
+    M.prec = T.precision;
     M.planarZ = rsg::planar_z(pm, P.bcZeroZ, &P.settings, T.warm ? 0.0 : T.init[2],
                               T.warm ? T.warmCells.data() : nullptr,
                               T.warm && pm.nB > 0 ? T.warmBnd.data() : nullptr) ? 1 : 0;
@@ -535,6 +422,11 @@ void engine_loop(rsg_context* ctx) {
                                  : std::max(kTeamMinBlocks, (int)std::lround(kTeamBlocksAt40k * std::sqrt(nC / 40000.0)));
             teamNb[q] = std::min(want, ctx->teamCap);
         }
+        // FP64 members first, then mixed ones: each group runs the kernel
+        // built for its precision (one launch each, back to back)
+        std::stable_partition(batch.begin(), batch.end(), [&](int64_t id) { return ctx->tickets[id]->precision == 0; });
+        int n64 = 0;
+        for (int64_t id : batch) n64 += ctx->tickets[id]->precision == 0;
         ctx->hMembers.resize(nr);
         ctx->hStatus.resize(nr);
         for (int i = 0; i < nr; ++i) {
@@ -572,9 +464,12 @@ void engine_loop(rsg_context* ctx) {
             e = hipMemcpyAsync(ctx->dMembers, ctx->hMembers.data(), sizeof(RsgMemberDev) * nr,
                                hipMemcpyHostToDevice, ctx->stream);
             hipEventRecord(ctx->ev0, ctx->stream);
-            hipLaunchKernelGGL(chunk_kernel(bs), dim3(nr), dim3(bs), 0, ctx->stream, ctx->dMembers, ctx->dStatus);
+            hipError_t el = hipSuccess;
+            if (n64 > 0) el = rsg_chunk_launch(bs, 0, n64, ctx->dMembers, ctx->dStatus, ctx->stream);
+            if (el == hipSuccess && nr > n64)
+                el = rsg_chunk_launch(bs, 1, nr - n64, ctx->dMembers + n64, ctx->dStatus + n64, ctx->stream);
             hipEventRecord(ctx->ev1, ctx->stream);
-            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess) e = el;
             if (e == hipSuccess)
                 e = hipMemcpyAsync(ctx->hStatus.data(), ctx->dStatus, sizeof(RsgScalars) * nr, hipMemcpyDeviceToHost,
                                    ctx->stream);
@@ -648,14 +543,8 @@ int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** 
     c->warpSize = prop.warpSize;
     if (hipDeviceGetAttribute(&c->clockKHz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->clockKHz = 0;
     c->teamCap = rsg_team_capacity(device);
-    {
-        int per = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chunk_kernel(256), 256, 0) == hipSuccess && per > 0)
-            c->resident256 = per * c->cuCount;
-        per = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chunk_kernel(512), 512, 0) == hipSuccess && per > 0)
-            c->resident512 = per * c->cuCount;
-    }
+    c->resident256 = rsg_chunk_occupancy(256) * c->cuCount;
+    c->resident512 = rsg_chunk_occupancy(512) * c->cuCount;
     if (hbm_budget_bytes == 0) hbm_budget_bytes = 8ull << 30;
     if (hipMalloc(&c->arena.base, hbm_budget_bytes) != hipSuccess) return RSG_ERR_NOMEM;
     c->arena.size = hbm_budget_bytes;
@@ -863,8 +752,8 @@ int rsg_problem_set_observations(rsg_context* ctx, int32_t problem_id, int32_t n
 int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const rsg_init* init,
                int32_t precision, int64_t* ticket) {
     if (!ctx || !coeffs || !init || !ticket) return RSG_ERR_ARG;
-    if (precision != RSG_PREC_FP64)
-        return set_err(ctx, RSG_ERR_UNSUPPORTED, "only RSG_PREC_FP64 members are implemented");
+    if (precision != RSG_PREC_FP64 && precision != RSG_PREC_MIXED)
+        return set_err(ctx, RSG_ERR_ARG, "unknown precision code");
     std::unique_ptr<Ticket> T(new Ticket());
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1047,7 +936,8 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     if (!ctx || n <= 0 || nif < 0 || !own || (nif > 0 && !nbr) || !diag || (nif > 0 && (!upper || !lower)) ||
         !b || !x || !stats)
         return RSG_ERR_ARG;
-    if (precision != RSG_PREC_FP64) return set_err(ctx, RSG_ERR_UNSUPPORTED, "only FP64 is implemented");
+    if (precision != RSG_PREC_FP64 && precision != RSG_PREC_MIXED)
+        return set_err(ctx, RSG_ERR_ARG, "unknown precision code");
     if (solver != RSG_SOLVER_AMG && solver != RSG_SOLVER_PCG && solver != RSG_SOLVER_BICGSTAB)
         return set_err(ctx, RSG_ERR_ARG, "unknown solver code");
     for (int f = 0; f < nif; ++f)
@@ -1076,6 +966,7 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     rsg::Carver cv; cv.base = wbase;
     rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, 0, M);
     M.mesh = reinterpret_cast<RsgMeshDev*>(mstruct);
+    M.prec = precision;
     hipError_t e = hipMemcpy(base, stage.data(), mbytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dM, &M, sizeof M, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(M.diag, diag, 8 * (size_t)n, hipMemcpyHostToDevice);
@@ -1087,9 +978,7 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     if (e == hipSuccess) e = hipMemcpy(M.sc, &s, sizeof s, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         // the stats land in grad-p scratch, which no linear solver touches
-        hipLaunchKernelGGL(k_ldu_solve<256>, dim3(1), dim3(256), 0, ctx->stream, dM, solver, max_iter, tol,
-                           M.gp[0]);
-        e = hipGetLastError();
+        e = rsg_ldu_launch(dM, solver, max_iter, tol, M.gp[0], ctx->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     double st[5] = {0, 0, 0, 0, 0};
@@ -1275,6 +1164,46 @@ int rsg_wall_distance(rsg_context* ctx, int32_t n_cells, const double* cell_cent
         e = A.err;
     }
     (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
+    if (kernel_seconds) *kernel_seconds = 1e-3 * ms;
+    return RSG_OK;
+}
+
+// Plain device copy of `bytes` bytes (16-byte vector loads and stores,
+// grid-stride), used to scale hardware byte counters: a run of it under
+// FETCH_SIZE / WRITE_SIZE reads what the counters report for a known traffic.
+__global__ void k_copy_calibration(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+int rsg_copy_calibration(rsg_context* ctx, uint64_t bytes, double* kernel_seconds) {
+    if (!ctx || bytes < 16 || (bytes & 15)) return RSG_ERR_ARG;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "hipSetDevice");
+    const size_t n = bytes / 16;
+    float ms = 0.0f;
+    hipError_t e;
+    {
+        rsg::grid::DevAlloc A;
+        double2* a = A.take<double2>(n);
+        double2* b = A.take<double2>(n);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (A.err == hipSuccess) A.err = hipMemsetAsync(a, 0, bytes, ctx->stream);
+        if (A.err == hipSuccess) A.err = hipEventCreate(&ev[0]);
+        if (A.err == hipSuccess) A.err = hipEventCreate(&ev[1]);
+        if (A.err == hipSuccess) A.err = hipEventRecord(ev[0], ctx->stream);
+        if (A.err == hipSuccess) {
+            const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+            hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, ctx->stream, a, b, n);
+            A.err = hipGetLastError();
+        }
+        if (A.err == hipSuccess) A.err = hipEventRecord(ev[1], ctx->stream);
+        if (A.err == hipSuccess) A.err = hipStreamSynchronize(ctx->stream);
+        if (A.err == hipSuccess) A.err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        for (hipEvent_t v : ev)
+            if (v) (void)hipEventDestroy(v);
+        e = A.err;
+    }
     if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
     if (kernel_seconds) *kernel_seconds = 1e-3 * ms;
     return RSG_OK;

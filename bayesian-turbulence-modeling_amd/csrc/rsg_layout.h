@@ -3,7 +3,9 @@
 // Plain structs of device pointers shared by the HIP kernels (rsg_member.h),
 // the C-ABI implementation (rsg_capi.hip) and the sequential emulation build
 // used by the CPU test suite.  Everything is structure-of-arrays, FP64 for
-// the fields and the outer defect, int32 for topology.
+// the fields, the assembled systems and the outer defect, int32 for topology;
+// the FP32 inner solves of mixed-precision members view scratch vectors
+// (Krylov and AMG level vectors) as float arrays of twice the length.
 //
 // One mesh = one RsgMeshDev (read-only, shared by every member that solves on
 // it).  One solve in flight = one RsgMemberDev (its fields, LDU system,
@@ -185,6 +187,7 @@ struct RsgMemberDev {
     double init[6];        // uniform initial state Ux Uy Uz p k omega
     int initFromFields;    // state already uploaded (warm start)
     int planarZ;           // Uz stays exactly zero: its momentum solve is skipped
+    int prec;              // inner linear solves: 0 FP64, 1 FP32 with FP64 defect and refinement
     int iterBudget;        // outer iterations allowed in this launch
     long long sliceTicks;  // wall-clock ticks this launch may run the member (0: no time limit)
     // fields (cell) and boundary values

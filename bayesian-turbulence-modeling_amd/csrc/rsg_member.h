@@ -102,12 +102,14 @@ struct Thr {
 
 #define RSG_TEAM_MAXB 256     // workgroups in a team at most
 
+// Block reductions of NV values of type T (double, or float for the FP32
+// inner solves of mixed-precision members: FP32 accumulation throughout).
 #ifdef RSG_EMULATE
-template <int NV> RSG_D void bsum(Thr&, double (&)[NV]) {}
-template <int NV> RSG_D void bmax(Thr&, double (&)[NV]) {}
+template <int NV, class T> RSG_D void bsum(Thr&, T (&)[NV]) {}
+template <int NV, class T> RSG_D void bmax(Thr&, T (&)[NV]) {}
 RSG_D double wave_bcast0(double v) { return v; }
 #else
-RSG_D double wsum(double v) {
+template <class T> RSG_D T wsum(T v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -121,9 +123,10 @@ RSG_D double wmax(double v) {
 // Block sum of NV values, result identical in every thread: lane 0 of each
 // wave deposits its wave total, then every thread adds the wave totals in
 // wave order.  Double-buffered LDS so consecutive calls need one barrier.
-template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
+template <int NV, class T> RSG_D void bsum(Thr& t, T (&v)[NV]) {
+    static_assert(NV <= 4, "block reductions carry at most 4 values");
     const int lane = t.tid & 63, wid = t.tid >> 6, nw = (t.bs + 63) >> 6;
-    double* buf = t.red + t.phase * 64;
+    T* buf = reinterpret_cast<T*>(t.red + t.phase * 64);
     t.phase ^= 1;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = wsum(v[i]);
@@ -134,7 +137,7 @@ template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        double s = 0.0;
+        T s = T(0);
         for (int w = 0; w < nw; ++w) s += buf[w * NV + i];
         v[i] = s;
     }
@@ -165,11 +168,11 @@ template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
 // ... in turn, then the xor butterfly across the lanes -- and takes lane 0's
 // result, so every thread of the team holds the same value (NaNs included).
 // Double-buffered like the workgroup version.
-template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
-    static_assert(NV <= 8, "team reductions carry at most 8 values");
+template <int NV, class T> RSG_D void bsum(Thr& t, T (&v)[NV]) {
+    static_assert(NV <= 4, "team reductions carry at most 4 values");
     const int lane = t.ltid & 63, wid = t.ltid >> 6, nw = (t.lbs + 63) >> 6;
-    double* buf = t.red + t.phase * 64;
-    double* gbuf = t.gred + t.phase * (RSG_TEAM_MAXB * 8);
+    T* buf = reinterpret_cast<T*>(t.red + t.phase * 64);
+    T* gbuf = reinterpret_cast<T*>(t.gred + t.phase * (RSG_TEAM_MAXB * 8));
     t.phase ^= 1;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = wsum(v[i]);
@@ -181,7 +184,7 @@ template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
     if (t.ltid == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
-            double s = 0.0;
+            T s = T(0);
             for (int w = 0; w < nw; ++w) s += buf[w * NV + i];
             gbuf[t.blk * NV + i] = s;
         }
@@ -189,13 +192,13 @@ template <int NV> RSG_D void bsum(Thr& t, double (&v)[NV]) {
     RSG_SYNC();
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        double s = 0.0;
+        T s = T(0);
         for (int b = lane; b < t.nblk; b += 64) s += gbuf[b * NV + i];
         v[i] = __shfl(wsum(s), 0, 64);
     }
 }
 template <int NV> RSG_D void bmax(Thr& t, double (&v)[NV]) {
-    static_assert(NV <= 8, "team reductions carry at most 8 values");
+    static_assert(NV <= 4, "team reductions carry at most 4 values");
     const int lane = t.ltid & 63, wid = t.ltid >> 6, nw = (t.lbs + 63) >> 6;
     double* buf = t.red + t.phase * 64;
     double* gbuf = t.gred + t.phase * (RSG_TEAM_MAXB * 8);
@@ -445,10 +448,17 @@ RSG_D void om_bc(Ctx& C, double beta1) {
 
 // ------------------------------------------------------- LDU operations
 // y = A x (LinearSystem::matvec LinearSolver.hpp:68-77) for cell c, gathering
-// the off-diagonal terms in ascending face order.
-RSG_D double ldu_row(const RsgMeshDev* m, const double* diag, const double* upper,
-                     const double* lower, const double* x, int c) {
-    double y = diag[c] * x[c];
+// the off-diagonal terms in ascending face order.  T is the arithmetic type:
+// double, or float for the FP32 inner solves of mixed-precision members (the
+// FP64 coefficients are rounded to FP32 as they are read).
+template <class T = double>
+RSG_D T ldu_row_f32(const RsgMeshDev* m, const double* diag, const double* upper, const double* lower, const T* x,
+                    int c);
+template <class T = double>
+RSG_D T ldu_row(const RsgMeshDev* m, const double* diag, const double* upper, const double* lower, const T* x,
+                int c) {
+    if constexpr (sizeof(T) == sizeof(float)) return ldu_row_f32<T>(m, diag, upper, lower, x, c);
+    T y = (T)diag[c] * x[c];
     const int nIF = m->nIF;
     if (m->ellK) {
         // every internal face of the cell sits in its ELL slots, in ascending
@@ -456,13 +466,13 @@ RSG_D double ldu_row(const RsgMeshDev* m, const double* diag, const double* uppe
         // loop below, with every slot's loads issued before the first add
         const int nC = m->nC;
         int code[RSG_ELL_K], nb[RSG_ELL_K];
-        double cf[RSG_ELL_K], xv[RSG_ELL_K];
+        T cf[RSG_ELL_K], xv[RSG_ELL_K];
 #pragma unroll
         for (int k = 0; k < RSG_ELL_K; ++k) { code[k] = m->ellFace[k * nC + c]; nb[k] = m->ellNb[k * nC + c]; }
 #pragma unroll
         for (int k = 0; k < RSG_ELL_K; ++k) {
             const int f = code[k] >> 1;
-            cf[k] = code[k] < 0 ? 0.0 : ((code[k] & 1) ? lower[f] : upper[f]);
+            cf[k] = code[k] < 0 ? T(0) : (T)((code[k] & 1) ? lower[f] : upper[f]);
             xv[k] = x[nb[k]];
         }
 #pragma unroll
@@ -474,95 +484,160 @@ RSG_D double ldu_row(const RsgMeshDev* m, const double* diag, const double* uppe
         const int code = m->cfIdx[e];
         const int f = code >> 1;
         if (f >= nIF) break;   // boundary faces sort after every internal face
-        if (code & 1) y += lower[f] * x[m->own[f]];
-        else          y += upper[f] * x[m->nbr[f]];
+        if (code & 1) y += (T)lower[f] * x[m->own[f]];
+        else          y += (T)upper[f] * x[m->nbr[f]];
     }
     return y;
 }
 
+// The FP32 row product in the form ex x_c + sum a (x_nb - x_c), ex the FP64
+// row sum rounded once (the resident solvers' res_row explains why).
+template <class T>
+RSG_D T ldu_row_f32(const RsgMeshDev* m, const double* diag, const double* upper, const double* lower, const T* x,
+                    int c) {
+    const int nIF = m->nIF;
+    const T xc = x[c];
+    double ex = diag[c];
+    T y = T(0);
+    if (m->ellK) {
+        const int nC = m->nC;
+        int code[RSG_ELL_K], nb[RSG_ELL_K];
+        double cf[RSG_ELL_K];
+        T xv[RSG_ELL_K];
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k) { code[k] = m->ellFace[k * nC + c]; nb[k] = m->ellNb[k * nC + c]; }
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k) {
+            const int f = code[k] >> 1;
+            cf[k] = code[k] < 0 ? 0.0 : ((code[k] & 1) ? lower[f] : upper[f]);
+            xv[k] = x[nb[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < RSG_ELL_K; ++k)
+            if (code[k] >= 0) { ex += cf[k]; y += (T)cf[k] * (xv[k] - xc); }
+    } else {
+        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+            const int code = m->cfIdx[e];
+            const int f = code >> 1;
+            if (f >= nIF) break;
+            const double a = (code & 1) ? lower[f] : upper[f];
+            ex += a;
+            y += (T)a * (x[(code & 1) ? m->own[f] : m->nbr[f]] - xc);
+        }
+    }
+    return (T)ex * xc + y;
+}
+
 struct SolveRes { int iterations; double finalRes, initialRes; int converged; };
 
-RSG_D double jac(double r, double d) { return (fabs(d) > 1e-30) ? r / d : r; }
+template <class T> RSG_D T jac(T r, T d) { return (fabs(d) > T(1e-30)) ? r / d : r; }
 
-// BiCGSTABSolver::solve LinearSolver.hpp:193-245 (Jacobi right preconditioning)
-RSG_D SolveRes bicgstab(Ctx& C, double* x, int maxIter, double tol) {
+// Krylov work vectors of the global-memory solvers.  FP64 members use
+// M->kv[0..7]; the FP32 inner solves of mixed members carve up to ten float
+// vectors from the storage of kv[0..4] (each holds two), which leaves kv[5..7]
+// for the FP64 defect and update of the refinement.
+template <class T> RSG_D T* kvec(RsgMemberDev* M, int i);
+template <> RSG_DM double* kvec<double>(RsgMemberDev* M, int i) { return M->kv[i]; }
+template <> RSG_DM float* kvec<float>(RsgMemberDev* M, int i) {
+    return reinterpret_cast<float*>(M->kv[i >> 1]) + (i & 1) * M->mesh->nC;
+}
+
+// BiCGSTABSolver::solve LinearSolver.hpp:193-245 (Jacobi right
+// preconditioning) from the residual r already at x (rr = r.r block-summed,
+// every thread holding it): x and r in/out.
+template <class T>
+RSG_D SolveRes bicgstab_from(Ctx& C, T* x, T* r, T rr, int maxIter, double tol) {
     Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const int nC = C.nC;
-    const double *dg = M->diag, *up = M->upper, *lo = M->lower, *b = M->src;
-    double *r = M->kv[0], *rh = M->kv[1], *p = M->kv[2], *v = M->kv[3];
-    double *s = M->kv[4], *tt_ = M->kv[5], *ph = M->kv[6], *sh = M->kv[7];
+    const double *dg = M->diag, *up = M->upper, *lo = M->lower;
+    T *rh = kvec<T>(M, 1), *p = kvec<T>(M, 2), *v = kvec<T>(M, 3);
+    T *s = kvec<T>(M, 4), *tt_ = kvec<T>(M, 5), *ph = kvec<T>(M, 6), *sh = kvec<T>(M, 7);
     SolveRes res = {0, 0.0, 0.0, 0};
-    double acc[1] = {0.0};
-    FOR_CELLS(c) {
-        const double rc = b[c] - ldu_row(m, dg, up, lo, x, c);
-        r[c] = rc; rh[c] = rc; p[c] = 0.0; v[c] = 0.0;
-        acc[0] += rc * rc;
-    }
-    bsum(t, acc);
-    const double r0 = sqrt(acc[0]);
+    const T r0 = sqrt(rr);
     res.initialRes = r0;
-    if (r0 < 1e-30) { res.converged = 1; return res; }
-    double rho = 1.0, alpha = 1.0, omega = 1.0;
-    double rhoN = acc[0];   // dot(rh, r) with rh == r: the same sum
+    if (r0 < T(1e-30)) { res.converged = 1; return res; }
+    FOR_CELLS(c) { rh[c] = r[c]; p[c] = T(0); v[c] = T(0); }
+    T rho = T(1), alpha = T(1), omega = T(1);
+    T rhoN = rr;   // dot(rh, r) with rh == r: the same sum
     for (int it = 0; it < maxIter; ++it) {
-        if (fabs(rhoN) < 1e-30) break;
-        const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
+        if (fabs(rhoN) < T(1e-30)) break;
+        const T beta = (rhoN / (rho + T(1e-30))) * (alpha / (omega + T(1e-30)));
         FOR_CELLS(c) {
-            const double pc = r[c] + beta * (p[c] - omega * v[c]);
+            const T pc = r[c] + beta * (p[c] - omega * v[c]);
             p[c] = pc;
-            ph[c] = jac(pc, dg[c]);
+            ph[c] = jac<T>(pc, (T)dg[c]);
         }
         RSG_SYNC();
-        double a1[1] = {0.0};
+        T a1[1] = {T(0)};
         FOR_CELLS(c) {
-            const double vc = ldu_row(m, dg, up, lo, ph, c);
+            const T vc = ldu_row<T>(m, dg, up, lo, ph, c);
             v[c] = vc;
             a1[0] += rh[c] * vc;
         }
         bsum(t, a1);
-        const double rv = a1[0];
-        if (fabs(rv) < 1e-30) break;
+        const T rv = a1[0];
+        if (fabs(rv) < T(1e-30)) break;
         alpha = rhoN / rv;
-        double a2[1] = {0.0};
+        T a2[1] = {T(0)};
         FOR_CELLS(c) {
-            const double sc = r[c] - alpha * v[c];
+            const T sc = r[c] - alpha * v[c];
             s[c] = sc;
-            sh[c] = jac(sc, dg[c]);
+            sh[c] = jac<T>(sc, (T)dg[c]);
             a2[0] += sc * sc;
         }
         bsum(t, a2);
-        const double sn = sqrt(a2[0]);
+        const T sn = sqrt(a2[0]);
         if (sn / r0 < tol) {
             FOR_CELLS(c) x[c] += alpha * ph[c];
             res.iterations = it + 1; res.finalRes = sn / r0; res.initialRes = r0; res.converged = 1;
             return res;
         }
-        double a3[2] = {0.0, 0.0};
+        T a3[2] = {T(0), T(0)};
         FOR_CELLS(c) {
-            const double tc = ldu_row(m, dg, up, lo, sh, c);
+            const T tc = ldu_row<T>(m, dg, up, lo, sh, c);
             tt_[c] = tc;
             a3[0] += tc * tc;
             a3[1] += tc * s[c];
         }
         bsum(t, a3);
-        omega = (a3[0] > 1e-30) ? a3[1] / a3[0] : 0.0;
-        double a4[2] = {0.0, 0.0};
+        omega = (a3[0] > T(1e-30)) ? a3[1] / a3[0] : T(0);
+        T a4[2] = {T(0), T(0)};
         FOR_CELLS(c) {
             x[c] += alpha * ph[c] + omega * sh[c];
-            const double rc = s[c] - omega * tt_[c];
+            const T rc = s[c] - omega * tt_[c];
             r[c] = rc;
             a4[0] += rc * rc;
             a4[1] += rh[c] * rc;
         }
         bsum(t, a4);
-        const double rn = sqrt(a4[0]);
+        const T rn = sqrt(a4[0]);
         res.iterations = it + 1;
         res.finalRes = rn / r0;
         if (res.finalRes < tol) { res.converged = 1; return res; }
-        if (fabs(omega) < 1e-30) break;
+        if (fabs(omega) < T(1e-30)) break;
         rho = rhoN;
         rhoN = a4[1];
     }
     return res;
+}
+
+// FP64 defect r = b - A x of the member's LDU system, returns r.r (block sum).
+RSG_D double ldu_defect(Ctx& C, const double* x, double* r) {
+    Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const int nC = C.nC;
+    double acc[1] = {0.0};
+    FOR_CELLS(c) {
+        const double rc = M->src[c] - ldu_row<double>(m, M->diag, M->upper, M->lower, x, c);
+        r[c] = rc;
+        acc[0] += rc * rc;
+    }
+    bsum(t, acc);
+    return acc[0];
+}
+
+// BiCGSTAB on the member's system from x (FP64).
+RSG_D SolveRes bicgstab(Ctx& C, double* x, int maxIter, double tol) {
+    double* r = C.M->kv[0];
+    const double rr = ldu_defect(C, x, r);
+    return bicgstab_from<double>(C, x, r, rr, maxIter, tol);
 }
 
 // ------------------------------------------------------------------ AMG
@@ -856,19 +931,19 @@ RSG_D void amg_coarse_inverse(Ctx& C) {
 // an exact dense coarsest solve in place of 50 Gauss-Seidel sweeps).
 // s -/+ sum over CSR entries e0..e1-1 of v[e] x[ci[e]], term by term in CSR
 // order (the same roundings as the plain loop); the loads of four entries
-// are issued together.
-template <bool SUB>
-RSG_D double csr_row_acc(const int* ci, const double* v, const double* x, int e0, int e1, double s) {
+// are issued together.  T: arithmetic type (values rounded to it as read).
+template <bool SUB, class T>
+RSG_D T csr_row_acc(const int* ci, const double* v, const T* x, int e0, int e1, T s) {
     for (int e = e0; e < e1; e += 4) {
-        int c[4]; double a[4], xv[4];
+        int c[4]; T a[4], xv[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const bool ok = e + k < e1;
             c[k] = ok ? ci[e + k] : 0;
-            a[k] = ok ? v[e + k] : 0.0;
+            a[k] = ok ? (T)v[e + k] : T(0);
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) xv[k] = (e + k < e1) ? x[c[k]] : 0.0;
+        for (int k = 0; k < 4; ++k) xv[k] = (e + k < e1) ? x[c[k]] : T(0);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (e + k < e1) { if (SUB) s -= a[k] * xv[k]; else s += a[k] * xv[k]; }
@@ -876,78 +951,87 @@ RSG_D double csr_row_acc(const int* ci, const double* v, const double* x, int e0
     return s;
 }
 
+template <class T>
 RSG_D void csr_jacobi(Thr& t, int n, const int* rp, const int* ci, const double* v, const double* dg,
-                      const double* b, const double* xin, double* xout, bool xzero) {
-    const double wJ = 2.0 / 3.0;
+                      const T* b, const T* xin, T* xout, bool xzero) {
+    const T wJ = T(2.0 / 3.0);
     for (int i = t.tid; i < n; i += t.bs) {
-        double s = b[i];
-        if (!xzero) s = csr_row_acc<true>(ci, v, xin, rp[i], rp[i + 1], s);
-        const double d = dg[i];
-        const double x0 = xzero ? 0.0 : xin[i];
-        xout[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+        T s = b[i];
+        if (!xzero) s = csr_row_acc<true, T>(ci, v, xin, rp[i], rp[i + 1], s);
+        const T d = (T)dg[i];
+        const T x0 = xzero ? T(0) : xin[i];
+        xout[i] = (fabs(d) > T(1e-30)) ? x0 + wJ * s / d : x0;
     }
 }
 
 // Residual row i of a CSR level: b_i - sum of the row's entries times x, in
 // CSR order.
-RSG_D double csr_row_residual(const int* rp, const int* ci, const double* v, const double* b, const double* x,
-                              int i) {
-    const double s = csr_row_acc<false>(ci, v, x, rp[i], rp[i + 1], 0.0);
+template <class T>
+RSG_D T csr_row_residual(const int* rp, const int* ci, const double* v, const T* b, const T* x, int i) {
+    const T s = csr_row_acc<false, T>(ci, v, x, rp[i], rp[i + 1], T(0));
     return b[i] - s;
 }
+
+// Level vectors of the hierarchy in type T: FP64 members use A.x / A.b / A.r
+// as doubles; FP32 inner solves view the same storage as floats (the level
+// offsets A.lvRow index elements either way).
+template <class T> RSG_D T* lvec(double* base) { return reinterpret_cast<T*>(base); }
 
 // Level 0's right-hand side is rin itself.  Going down, each coarse row sums
 // the residuals of its aggregate's rows straight from x (a residual pass, a
 // barrier and the restriction, rounded the same, without the barrier); going
 // up, the first post-smoothing sweep reads every x value as the prolonged
-// x[j] + xc[map[j]], so the prolongation pass and its barrier go away.
-RSG_D void amg_vcycle(Ctx& C, const double* rin, double* z) {
+// x[j] + xc[map[j]], so the prolongation pass and its barrier go away.  The
+// coarsest level's dense inverse is applied in FP64 whatever T is.
+template <class T>
+RSG_D void amg_vcycle(Ctx& C, const T* rin, T* z) {
     Thr& t = C.t; RsgMemberDev* M = C.M; RsgAmgDev& A = M->amg;
     const int L = A.lvN[RSG_MAX_LEVELS];
-    const double wJ = 2.0 / 3.0;
+    const T wJ = T(2.0 / 3.0);
+    T* Ax = lvec<T>(A.x); T* Ab = lvec<T>(A.b); T* Ar = lvec<T>(A.r);
     for (int l = 0; l + 1 < L; ++l) {
         const int n = A.lvN[l];
         const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
         const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
-        double* x = A.x + A.lvRow[l]; double* r = A.r + A.lvRow[l];
-        const double* b = (l == 0) ? rin : A.b + A.lvRow[l];
-        csr_jacobi(t, n, rp, ci, v, dg, b, x, r, true);
+        T* x = Ax + A.lvRow[l]; T* r = Ar + A.lvRow[l];
+        const T* b = (l == 0) ? rin : Ab + A.lvRow[l];
+        csr_jacobi<T>(t, n, rp, ci, v, dg, b, x, r, true);
         RSG_SYNC();
-        csr_jacobi(t, n, rp, ci, v, dg, b, r, x, false);
+        csr_jacobi<T>(t, n, rp, ci, v, dg, b, r, x, false);
         RSG_SYNC();
         const int nc = A.lvN[l + 1];
         const int* aP = A.aggPtr + A.lvRp[l + 1];
         const int* aI = A.aggIdx + A.lvRow[l];
-        double* cb = A.b + A.lvRow[l + 1];
+        T* cb = Ab + A.lvRow[l + 1];
         for (int I = t.tid; I < nc; I += t.bs) {
-            double s = 0.0;
-            for (int q = aP[I]; q < aP[I + 1]; ++q) s += csr_row_residual(rp, ci, v, b, x, aI[q]);
+            T s = T(0);
+            for (int q = aP[I]; q < aP[I + 1]; ++q) s += csr_row_residual<T>(rp, ci, v, b, x, aI[q]);
             cb[I] = s;
         }
     }
     RSG_SYNC();
     {   // coarsest level
         const int l = L - 1, n = A.lvN[l];
-        double* x = (L == 1) ? z : A.x + A.lvRow[l];
-        const double* b = (l == 0) ? rin : A.b + A.lvRow[l];
+        T* x = (L == 1) ? z : Ax + A.lvRow[l];
+        const T* b = (l == 0) ? rin : Ab + A.lvRow[l];
         if (M->sc->coarseDense) {
             const int DN = RSG_COARSE_DENSE;
             for (int i = t.tid; i < n; i += t.bs) {
                 double s = 0.0;
-                for (int j = 0; j < n; ++j) s += A.cinv[i * DN + j] * b[j];
-                x[i] = s;
+                for (int j = 0; j < n; ++j) s += A.cinv[i * DN + j] * (double)b[j];
+                x[i] = (T)s;
             }
             RSG_SYNC();
         } else {
             const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
             const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
-            double* r = A.r + A.lvRow[l];
-            csr_jacobi(t, n, rp, ci, v, dg, b, x, x, true);
+            T* r = Ar + A.lvRow[l];
+            csr_jacobi<T>(t, n, rp, ci, v, dg, b, x, x, true);
             RSG_SYNC();
             for (int s = 0; s < 25; ++s) {
-                csr_jacobi(t, n, rp, ci, v, dg, b, x, r, false);
+                csr_jacobi<T>(t, n, rp, ci, v, dg, b, x, r, false);
                 RSG_SYNC();
-                csr_jacobi(t, n, rp, ci, v, dg, b, r, x, false);
+                csr_jacobi<T>(t, n, rp, ci, v, dg, b, r, x, false);
                 RSG_SYNC();
             }
         }
@@ -956,81 +1040,127 @@ RSG_D void amg_vcycle(Ctx& C, const double* rin, double* z) {
         const int n = A.lvN[l];
         const int* rp = A.rp + A.lvRp[l]; const int* ci = A.ci + A.lvNz[l];
         const double* v = A.v + A.lvNz[l]; const double* dg = A.dg + A.lvRow[l];
-        double* x = A.x + A.lvRow[l]; double* r = A.r + A.lvRow[l];
-        const double* b = (l == 0) ? rin : A.b + A.lvRow[l];
+        T* x = Ax + A.lvRow[l]; T* r = Ar + A.lvRow[l];
+        const T* b = (l == 0) ? rin : Ab + A.lvRow[l];
         const int* map = A.map + A.lvRow[l];
-        const double* cx = A.x + A.lvRow[l + 1];
+        const T* cx = Ax + A.lvRow[l + 1];
         for (int i = t.tid; i < n; i += t.bs) {
-            double s = b[i];
-            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * (x[ci[e]] + cx[map[ci[e]]]);
-            const double d = dg[i];
-            const double x0 = x[i] + cx[map[i]];
-            r[i] = (fabs(d) > 1e-30) ? x0 + wJ * s / d : x0;
+            T s = b[i];
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= (T)v[e] * (x[ci[e]] + cx[map[ci[e]]]);
+            const T d = (T)dg[i];
+            const T x0 = x[i] + cx[map[i]];
+            r[i] = (fabs(d) > T(1e-30)) ? x0 + wJ * s / d : x0;
         }
         RSG_SYNC();
-        double* out = (l == 0) ? z : x;
-        csr_jacobi(t, n, rp, ci, v, dg, b, r, out, false);
+        T* out = (l == 0) ? z : x;
+        csr_jacobi<T>(t, n, rp, ci, v, dg, b, r, out, false);
         RSG_SYNC();
     }
 }
 
-// PCG (:147-186) with Jacobi or AMG V-cycle preconditioning (:602-653).
-RSG_D SolveRes pcg(Ctx& C, double* x, int maxIter, double tol, bool useAmg) {
+// PCG (:147-186) with Jacobi or AMG V-cycle preconditioning (:602-653), from
+// the residual r already at x (rr = r.r block-summed): x and r in/out.
+template <class T>
+RSG_D SolveRes pcg_from(Ctx& C, T* x, T* r, T rr, int maxIter, double tol, bool useAmg) {
     Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const int nC = C.nC;
-    const double *dg = M->diag, *up = M->upper, *lo = M->lower, *b = M->src;
-    double *r = M->kv[0], *z = M->kv[1], *p = M->kv[2], *Ap = M->kv[3];
+    const double *dg = M->diag, *up = M->upper, *lo = M->lower;
+    T *z = kvec<T>(M, 1), *p = kvec<T>(M, 2), *Ap = kvec<T>(M, 3);
     SolveRes res = {0, 0.0, 0.0, 0};
-    double acc[1] = {0.0};
-    FOR_CELLS(c) {
-        const double rc = b[c] - ldu_row(m, dg, up, lo, x, c);
-        r[c] = rc;
-        acc[0] += rc * rc;
-    }
-    bsum(t, acc);
-    const double r0 = sqrt(acc[0]);
+    const T r0 = sqrt(rr);
     res.initialRes = r0;
-    if (r0 < 1e-30) { res.converged = 1; return res; }
-    double rzv[1] = {0.0};
+    if (r0 < T(1e-30)) { res.converged = 1; return res; }
+    T rzv[1] = {T(0)};
     if (useAmg) {
-        amg_vcycle(C, r, z);
+        amg_vcycle<T>(C, r, z);
         FOR_CELLS(c) { p[c] = z[c]; rzv[0] += r[c] * z[c]; }
     } else {
-        FOR_CELLS(c) { const double zc = jac(r[c], dg[c]); z[c] = zc; p[c] = zc; rzv[0] += r[c] * zc; }
+        FOR_CELLS(c) { const T zc = jac<T>(r[c], (T)dg[c]); z[c] = zc; p[c] = zc; rzv[0] += r[c] * zc; }
     }
     bsum(t, rzv);
-    double rz = rzv[0];
+    T rz = rzv[0];
     for (int it = 0; it < maxIter; ++it) {
-        double a1[1] = {0.0};
-        FOR_CELLS(c) { const double q = ldu_row(m, dg, up, lo, p, c); Ap[c] = q; a1[0] += p[c] * q; }
+        T a1[1] = {T(0)};
+        FOR_CELLS(c) { const T q = ldu_row<T>(m, dg, up, lo, p, c); Ap[c] = q; a1[0] += p[c] * q; }
         bsum(t, a1);
-        const double pAp = a1[0];
-        if (fabs(pAp) < 1e-30) break;
-        const double alpha = rz / pAp;
-        double a2[1] = {0.0};
+        const T pAp = a1[0];
+        if (fabs(pAp) < T(1e-30)) break;
+        const T alpha = rz / pAp;
+        T a2[1] = {T(0)};
         FOR_CELLS(c) {
             x[c] += alpha * p[c];
-            const double rc = r[c] - alpha * Ap[c];
+            const T rc = r[c] - alpha * Ap[c];
             r[c] = rc;
             a2[0] += rc * rc;
         }
         bsum(t, a2);
-        const double rn = sqrt(a2[0]);
+        const T rn = sqrt(a2[0]);
         res.iterations = it + 1;
         res.finalRes = rn / r0;
         if (res.finalRes < tol) { res.converged = 1; return res; }
-        double a3[1] = {0.0};
+        T a3[1] = {T(0)};
         if (useAmg) {
-            amg_vcycle(C, r, z);
+            amg_vcycle<T>(C, r, z);
             FOR_CELLS(c) a3[0] += r[c] * z[c];
         } else {
-            FOR_CELLS(c) { const double zc = jac(r[c], dg[c]); z[c] = zc; a3[0] += r[c] * zc; }
+            FOR_CELLS(c) { const T zc = jac<T>(r[c], (T)dg[c]); z[c] = zc; a3[0] += r[c] * zc; }
         }
         bsum(t, a3);
-        const double rzNew = a3[0];
-        const double beta = rzNew / (rz + 1e-30);
+        const T rzNew = a3[0];
+        const T beta = rzNew / (rz + T(1e-30));
         FOR_CELLS(c) p[c] = z[c] + beta * p[c];
         rz = rzNew;
         RSG_SYNC();
+    }
+    return res;
+}
+
+RSG_D SolveRes pcg(Ctx& C, double* x, int maxIter, double tol, bool useAmg) {
+    double* r = C.M->kv[0];
+    const double rr = ldu_defect(C, x, r);
+    return pcg_from<double>(C, x, r, rr, maxIter, tol, useAmg);
+}
+
+// Mixed precision on the global-memory path: FP64 defect r = b - A x, an FP32
+// inner solve of A d = r from d = 0 (BiCGSTAB, or PCG with Jacobi / the
+// V-cycle) asked for the reduction still missing, x += d in FP64, and the
+// FP64 defect again -- iterative refinement until the true FP64 residual is
+// below tol relative to the first defect (the reference's inner tolerance,
+// LinearSolver.hpp:147-247), at most RSG_MIXED_SWEEPS sweeps and maxIter
+// inner iterations in all.  initialRes is the FP64 first defect, as in the
+// FP64 solvers, so the outer iteration's residual accounting is unchanged.
+#define RSG_MIXED_SWEEPS 4
+// Whether a member's inner solves run mixed: fixed at compile time in the
+// kernels built for one precision (PREC 0 or 1), read from the member
+// otherwise (PREC -1), so an FP64 kernel carries no FP32 solver code.
+template <int PREC> RSG_D bool is_mixed(const Ctx& C) { return PREC < 0 ? C.M->prec != 0 : PREC == 1; }
+RSG_D SolveRes lsolve_mixed_global(Ctx& C, int code, double* x, bool amg, int maxIter, double tol) {
+    Thr& t = C.t; RsgMemberDev* M = C.M; const int nC = C.nC;
+    double* r = M->kv[5];
+    float* rf = kvec<float>(M, 0);
+    float* d = kvec<float>(M, 8);
+    SolveRes res = {0, 0.0, 0.0, 0};
+    double rn = sqrt(ldu_defect(C, x, r));
+    const double r0 = rn;
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return res; }
+    for (int sweep = 0; sweep < RSG_MIXED_SWEEPS && res.iterations < maxIter; ++sweep) {
+        // the FP32 solve sees the defect scaled to unit norm (see lsolve_mixed_res)
+        const double inv = 1.0 / rn;
+        FOR_CELLS(c) { rf[c] = (float)(r[c] * inv); d[c] = 0.0f; }
+        RSG_SYNC();
+        const float rr = 1.0f;
+        const double tolIn = tol * r0 / rn;
+        const SolveRes in = code == 2 ? bicgstab_from<float>(C, d, rf, rr, maxIter - res.iterations, tolIn)
+                                      : pcg_from<float>(C, d, rf, rr, maxIter - res.iterations, tolIn, code == 0 && amg);
+        res.iterations += in.iterations;
+        RSG_SYNC();
+        FOR_CELLS(c) x[c] += rn * (double)d[c];
+        RSG_SYNC();
+        const double rnNew = sqrt(ldu_defect(C, x, r));
+        res.finalRes = rnNew / r0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+        if (!(rnNew < rn) || in.iterations == 0) break;   // the FP32 solve stalls: stop refining
+        rn = rnNew;
     }
     return res;
 }
@@ -1042,12 +1172,14 @@ namespace rsg {
 // One linear solve: the resident solvers when the member's cells fit the
 // workgroup's registers (and, for the V-cycle, its coarse vectors fit LDS),
 // the global-memory solvers otherwise.
-template <int CMAX>
-RSG_D SolveRes lsolve(Ctx& C, int code, double* x, bool amgReady) {
+// allowMixed false keeps this solve in FP64 for a mixed member.
+template <int CMAX, int PREC = -1>
+RSG_D SolveRes lsolve(Ctx& C, int code, double* x, bool amgReady, bool allowMixed = true) {
     const RsgSettingsDev* s = C.s;
     const bool amg = code == 0 && amgReady;
     if (res_fits(C, CMAX) && (!amg || res_amg_fits(C)))
-        return lsolve_res<CMAX>(C, code, x, amg, s->innerIter, s->innerTol);
+        return lsolve_res<CMAX, PREC>(C, code, x, amg, s->innerIter, s->innerTol, allowMixed);
+    if (allowMixed && is_mixed<PREC>(C)) return lsolve_mixed_global(C, code, x, amgReady, s->innerIter, s->innerTol);
     if (code == 0) return pcg(C, x, s->innerIter, s->innerTol, amgReady);
     if (code == 1) return pcg(C, x, s->innerIter, s->innerTol, false);
     return bicgstab(C, x, s->innerIter, s->innerTol);
@@ -1327,7 +1459,7 @@ struct Scal {
     double normUx0, normUy0, normP0, lastK, lastOm;
 };
 
-template <int CMAX>
+template <int CMAX, int PREC = -1>
 RSG_D void simple_iteration(Ctx& C, Scal& S) {
     Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const RsgSettingsDev* st = C.s;
     const int nC = C.nC, nIF = C.nIF, nB = C.nB;
@@ -1382,7 +1514,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
         if (comp > 0) { grad_u(C, comp - 1, comp); RSG_SYNC(); }
         assemble_momentum(C, comp);
         RSG_TMARK(1);
-        const SolveRes r = lsolve<CMAX>(C, st->mSolver, ucomp(M, comp), false);
+        const SolveRes r = lsolve<CMAX, PREC>(C, st->mSolver, ucomp(M, comp), false);
         if (comp == 0) rU0 = r; else if (comp == 1) rU1 = r; else rU2 = r;
         RSG_SYNC();
         RSG_TMARK(2);
@@ -1423,7 +1555,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     }
     FOR_CELLS(c) M->pp[c] = 0.0;
     RSG_SYNC();
-    SolveRes rP = lsolve<CMAX>(C, st->pSolver, M->pp, amgReady);
+    SolveRes rP = lsolve<CMAX, PREC>(C, st->pSolver, M->pp, amgReady);
     RSG_SYNC();
     RSG_TMARK(7);
     FOR_BFACES(b) M->ppb[b] = (m->bFlags[b] & RSG_BF_OUTLET) ? 0.0 : M->pp[m->own[nIF + b]];
@@ -1449,7 +1581,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     if (turbUpdate) {
         assemble_turb(C, false);
         RSG_TMARK(9);
-        rK = lsolve<CMAX>(C, st->tSolver, M->k, false);
+        rK = lsolve<CMAX, PREC>(C, st->tSolver, M->k, false);
         RSG_SYNC();
         RSG_TMARK(10);
         FOR_CELLS(c) M->k[c] = dmax(st->kMin, dmin(st->kMax, M->k[c]));
@@ -1458,7 +1590,14 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
         RSG_SYNC();
         assemble_turb(C, true);
         RSG_TMARK(11);
-        rOm = lsolve<CMAX>(C, st->tSolver, M->om, false);
+        // The omega system stays FP64 in mixed members: it carries the
+        // near-wall treatment (wall cells pinned at 60 nu / (beta1 y^2), 10^5-10^8
+        // times the interior omega), and an FP32 solve cannot resolve the
+        // interior corrections next to those rows -- its noise floor holds the
+        // outer residuals above 1e-8 (measured: a Re_tau 181 member stalls at
+        // p' ~5e-8 with FP32 omega solves and converges in the FP64 solve's
+        // iteration count, within 2e-11 of it, with FP64 ones).
+        rOm = lsolve<CMAX, PREC>(C, st->tSolver, M->om, false, false);
         RSG_SYNC();
         RSG_TMARK(12);
         FOR_CELLS(c) M->om[c] = dmax(st->omMin, dmin(1e15, M->om[c]));
@@ -1625,7 +1764,7 @@ RSG_D void observe_member(const RsgMemberDev* M) {
     M->pred[O->n] = ll;
 }
 
-template <int CMAX>
+template <int CMAX, int PREC = -1>
 RSG_D void run_member(Thr t, RsgMemberDev* M) {
     Ctx C;
     C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
@@ -1673,7 +1812,7 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
     const long long tStart = wall_clock64();
 #endif
     for (int b = 0; b < M->iterBudget && S.status == RSGM_RUNNING; ++b) {
-        simple_iteration<CMAX>(C, S);
+        simple_iteration<CMAX, PREC>(C, S);
 #if !defined(RSG_EMULATE) && !defined(RSG_TEAM)
         // time slice: thread 0 reads the clock and every thread follows its
         // decision, so the workgroup leaves the loop together (a team runs
@@ -1725,6 +1864,7 @@ RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter,
     }
     SolveRes r;
     if (res_fits(C, CMAX) && (!amg || res_amg_fits(C))) r = lsolve_res<CMAX>(C, solver, M->pp, amg, maxIter, tol);
+    else if (is_mixed<-1>(C)) r = lsolve_mixed_global(C, solver, M->pp, amg, maxIter, tol);
     else if (solver == 2) r = bicgstab(C, M->pp, maxIter, tol);
     else r = pcg(C, M->pp, maxIter, tol, amg);
     RSG_SYNC();

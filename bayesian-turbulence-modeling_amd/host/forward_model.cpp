@@ -30,7 +30,18 @@ rsg_context* Engine::ctx() {
         throw std::runtime_error("cannot start the MI355X engine (rsg_context_create returned " +
                                  std::to_string(rc) + "): no usable HIP device");
     ctx_ = c;
+    ++gen_;
     return ctx_;
+}
+
+rsg_context* Engine::existing(uint64_t gen) {
+    std::lock_guard<std::mutex> l(mu_);
+    return (ctx_ && gen == gen_) ? ctx_ : nullptr;
+}
+
+uint64_t Engine::generation() {
+    std::lock_guard<std::mutex> l(mu_);
+    return gen_;
 }
 
 void Engine::shutdown() {
@@ -98,12 +109,12 @@ ForwardModel::ForwardModel(const Mesh& mesh, const InferenceParameterSet& paramS
       Uinit_(Uinit), pInit_(pInit), kInit_(kInit), omegaInit_(omegaInit), cache_(50, 0.5) {}
 
 ForwardModel::~ForwardModel() {
+    // never starts an engine: at interpreter exit the context may already be
+    // shut down, and its ids went with it
     if (problemId_ >= 0 || meshId_ >= 0) {
-        try {
-            rsg_context* c = Engine::get().ctx();
+        if (rsg_context* c = Engine::get().existing(ctxGen_)) {
             if (problemId_ >= 0) rsg_problem_release(c, problemId_);
             if (meshId_ >= 0) rsg_mesh_release(c, meshId_);
-        } catch (...) {
         }
     }
 }
@@ -148,6 +159,7 @@ void ForwardModel::ensureProblem() {
         throw;
     }
     meshId_ = mid; problemId_ = pid;
+    ctxGen_ = Engine::get().generation();
 }
 
 int64_t ForwardModel::submit(const std::vector<double>& theta, bool useCache) {

@@ -35,6 +35,12 @@ class Engine {
 public:
     static Engine& get();
     rsg_context* ctx();           // throws if the GPU engine cannot start
+    // The running context if it is still the one of generation `gen` (the
+    // value generation() had when the caller's ids were created), else null;
+    // never starts an engine.  For destructors: ids of a context that has
+    // been shut down are gone with it.
+    rsg_context* existing(uint64_t gen);
+    uint64_t generation();
     void shutdown();
     void setLaunch(int itersPerLaunch, int blockSize);
     void setTimeSlice(int microseconds);
@@ -46,6 +52,7 @@ public:
 private:
     std::mutex mu_;
     rsg_context* ctx_ = nullptr;
+    uint64_t gen_ = 0;            // bumped every time a context is created
 };
 
 std::string engineError(rsg_context* ctx, int rc);
@@ -120,6 +127,7 @@ private:
     ConvergenceInfo lastConv_;
     int precision_ = RSG_PREC_FP64;
     int meshId_ = -1, problemId_ = -1;
+    uint64_t ctxGen_ = 0;         // engine generation the ids belong to
     std::mutex mu_;
     void ensureProblem();
 };

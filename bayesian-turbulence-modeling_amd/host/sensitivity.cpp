@@ -27,11 +27,11 @@ ParameterSensitivity::ParameterSensitivity(const Mesh& mesh, const ObservationOp
 }
 
 ParameterSensitivity::~ParameterSensitivity() {
-    try {
-        rsg_context* c = Engine::get().ctx();
+    // never starts an engine (see ForwardModel::~ForwardModel)
+    if (meshId_ < 0) return;
+    if (rsg_context* c = Engine::get().existing(ctxGen_)) {
         for (const auto& kv : problems_) rsg_problem_release(c, kv.second);
-        if (meshId_ >= 0) rsg_mesh_release(c, meshId_);
-    } catch (...) {
+        rsg_mesh_release(c, meshId_);
     }
 }
 
@@ -46,6 +46,7 @@ int ParameterSensitivity::problemFor(const SolverSettings& s) {
         const int rc = rsg_mesh_upload(c, &mesh_.desc(), &mid);
         if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
         meshId_ = mid;
+        ctxGen_ = Engine::get().generation();
     }
     ResolvedBC R;
     bcs_.resolve(mesh_, R);

@@ -64,6 +64,7 @@ private:
     FlowFields state_;
     bool hasState_ = false;
     int meshId_ = -1;
+    uint64_t ctxGen_ = 0;                               // engine generation the ids belong to
     std::map<std::pair<int, double>, int> problems_;   // (maxIterations, convergenceTol) -> problem id
 
     int problemFor(const SolverSettings& s);

@@ -1,28 +1,38 @@
 """Benchmark: converged RANS forward solves per second on one MI355X.
 
-Workload (one "step"): the C2 batched ensemble -- 512 streamwise-periodic
+Headline workload (C2): the batched ensemble -- 512 streamwise-periodic
 channel solves (4 x ny quads, ny 96-384 from Re_tau log-uniform in
 [180, 2000], betaStar/a1/sigma_k1/sigma_w1/beta1 within +-20 % of their
 defaults), each driven by SIMPLE + SST k-omega to the reference's convergence
-criterion at tol 1e-8, all members resident on the device at once (one
-workgroup per member, persistent launches time-sliced at 200 ms, continuous
-batching).
+criterion at tol 1e-8 (200,000-iteration cap), all members resident on the
+device (one workgroup per member, persistent launches time-sliced at 200 ms,
+continuous batching, largest meshes first).
 
-Prints ONE JSON line (rank 0).  value = converged solves / wall second over the
-timed steps; p50/p99 = per-member latency (submit -> result) inside a step.
-roofline = algorithmic bytes of the SIMPLE chunk kernel / its HIP-event time,
-against the 8 TB/s HBM peak; traffic = the kernel's measured HBM bytes per
-launch (two rocprofv3 counter passes after the timed region, scaled by a copy
-kernel).  cpu_baseline = the reference solver compiled from its own sources
-(oracle/_ref) on one host core: a bounded sample of three members gives the
-cost per outer iteration, applied to this run's iteration counts.
+One timed step = the whole ensemble solved cold (every member submitted at
+once, timed to the last retirement).  A step takes minutes, so --steps and
+--warmup are upper bounds: the warmup is one pass of every member capped at
+100 outer iterations (engine start-up, code-object load), and timed steps
+after the first run only while they fit --step-budget-s; the JSON line
+reports the steps actually timed (and the requested counts beside them).
 
---workload c1 instead times the stand-alone 261,600-cell backward-facing-step
-pressure system (rsg_system_solve, AMG-PCG to 1e-10, FP64 and FP32-inner with
-FP64 refinement) next to the reference's CPU PCG on the same system.
---workload walldist times the device wall-distance kernel on a 2M-cell mesh.
---workload bump times team solves (one member on a team of workgroups) of the
-periodic bump channel at 40k, 160k and 2M cells per outer iteration.
+Prints ONE JSON line.  value = converged solves / wall second over the timed
+steps; p50/p99 = per-member latency (submit -> result).  --precision mixed
+runs the members' linear solves as FP32 inner solves under FP64 refinement
+and then re-solves a sample of converged members in FP64 to measure
+max_rel_l2_vs_fp64.  roofline = algorithmic bytes of the SIMPLE chunk kernel
+(algorithmic_bytes) / its HIP-event time on the engine stream, against the
+8 TB/s HBM peak; traffic = the kernel's measured HBM bytes per launch (two
+rocprofv3 --pmc passes over a capped run, scaled by a copy kernel of known
+size).  cpu_baseline = the reference solver compiled from its own sources
+(oracle/_ref) on this host's cores, P = 1 and P = all, timed on a bounded
+sample and scaled by this run's outer-iteration counts.
+
+Other workloads (--workload): c1 (the stand-alone 261,600-cell
+backward-facing-step pressure system, rsg_system_solve, FP64 and mixed, next
+to the reference's CPU PCG), c4p (the same at 2M cells), c0 (single-solve
+latency of the Re_tau 550 channel and the same mesh batched), walldist (the
+device wall-distance kernel on a 2M-cell mesh), bump (team solves of one
+bump-channel member at 40k, 160k and 2M cells).
 """
 import argparse
 import glob
@@ -47,36 +57,68 @@ METRIC = "converged RANS solves/sec at <=1e-6 rel-L2 vs FP64; p50/p99 solve late
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n, nif, nb, outer, inner):
-    """Algorithmic HBM bytes of one member's solve (FP64 values, int32 indices),
-    each array counted once per pass (SpMV(LDU) = 8(n + 2 f_i) + 8 f_i + 16 n;
-    dot/axpy = 16-24 n per vector pair; face assembly reads geometry + fields).
+def algorithmic_bytes(n, nif, nb, outer, inner, w_inner=8.0):
+    """Algorithmic HBM bytes of one member's solve, by the rule: every logical
+    input and output array counted once per pass at its stored width (FP64 8,
+    FP32 4, int32 4), cell values gathered through face indices once per cell.
+
+    Per pass (w = 8, planar 2-D mesh: two face-normal components):
+      SpMV(LDU, value width v)  v(n + 2 f) + 8 f + 2 v n
+      Green-Gauss of q fields   f(8 + 3w) + f_b(4 + 2w + q w) + n(w + 3 q w)
+      face assembly of one equation, reading c cell fields
+                                f(8 + 4w) + 2w f + 2w n + c w n
+      SST closure               10 w n (+ the gradients it reads, counted as Green-Gauss)
+    One outer iteration: gradients of Ux Uy Uz k omega + closure, grad p,
+    momentum x and y (assembly reading 8 cell fields each, grad Ux refreshed
+    between), p' assembly (7 fields), grad p', corrections (9 w n), k and
+    omega assemblies (8 fields each), clamps (4 w n), wall pin and change
+    norms (5 w n); and for each of the five linear solves one FP64 defect
+    b - A x (SpMV + 2 w n; mixed precision: two, the initial defect and the
+    refinement check).
+    Inner iterations (value width w_inner: 8 FP64, 4 mixed):
+      BiCGSTAB iteration  2 SpMV + 14 w_inner n
+      AMG-PCG iteration   SpMV + 13 w_inner n + V-cycle,
+      V-cycle             4/3 (4 SpMV + 14 w_inner n)   (level 0 plus geometric coarsening)
     outer: outer iterations; inner: cumulative inner iterations
     (Ux, Uy, Uz, p, k, omega)."""
     w = 8.0
-    spmv = w * (n + 2 * nif) + 8.0 * nif + 2 * w * n + 4.0 * (n + 1 + 2 * nif)   # + cell->face lists
-    gg = nif * (8 + w + 3 * w) + nb * (4 + 4 * w) + n * (w + w + 3 * w) + 4.0 * (n + 1 + 2 * nif)
-    face_asm = nif * (8 + 4 * w + 6 * w) + 2 * w * nif + nb * (8 * w)
-    cell_asm = 4.0 * (n + 1 + 2 * nif) + n * (8 * w) + 2 * w * nif
-    assemble = face_asm + cell_asm
-    sst = 16 * w * n + gg * 5 / 1.0
-    per_outer = (3 * (assemble + gg)          # momentum: assembly + grad U refresh
-                 + gg + assemble + 2 * w * n  # grad p, p' assembly
-                 + gg + 6 * w * n             # grad p', corrections
-                 + 2 * assemble + 4 * w * n   # k, omega assembly + clamps
-                 + sst)
-    bicg = 2 * spmv + 14 * w * n
-    # V-cycle: ~4/3 of level-0 work per smoothing pass (geometric coarsening)
-    vcycle = (4.0 / 3.0) * (5 * spmv + 6 * w * n)
-    pcg_amg = spmv + vcycle + 8 * w * n
+
+    def spmv(v):
+        return v * (n + 2 * nif) + 8.0 * nif + 2 * v * n
+
+    def gg(q):
+        return nif * (8 + 3 * w) + nb * (4 + 2 * w + q * w) + n * (w + 3 * q * w)
+
+    def face_asm(c):
+        return nif * (8 + 4 * w) + 2 * w * nif + 2 * w * n + c * w * n
+
+    defects = 2 if w_inner < 8 else 1
+    per_outer = (gg(5) + 10 * w * n                # turbulence update: gradients + closure
+                 + gg(1)                           # grad p
+                 + 2 * face_asm(8) + gg(1)         # momentum x, y; grad Ux between
+                 + face_asm(7) + gg(1) + 9 * w * n  # p' assembly, grad p', corrections
+                 + 2 * face_asm(8) + 9 * w * n     # k, omega assemblies; clamps, pin, change norms
+                 + 5 * defects * (spmv(w) + 2 * w * n))
+    wi = w_inner
+    bicg = 2 * spmv(wi) + 14 * wi * n
+    vcycle = (4.0 / 3.0) * (4 * spmv(wi) + 14 * wi * n)
+    pcg_amg = spmv(wi) + 13 * wi * n + vcycle
     inner_bytes = (inner[0] + inner[1] + inner[2] + inner[4] + inner[5]) * bicg + inner[3] * pcg_amg
     return outer * per_outer + inner_bytes
 
 
-def run_batch(rs, specs, tol, fms_cache=None, max_iterations=200000):
+def configure_engine(rs, args):
+    rs.engine_set_launch(args.iters_per_launch, args.block)
+    rs.engine_set_time_slice(int(args.slice_ms * 1000))
+    rs.engine_set_schedule(args.schedule)
+
+
+def run_batch(rs, specs, tol, fms_cache=None, max_iterations=200000, precision="fp64"):
     import ensembles
     if fms_cache is None:
         meshes, fms = ensembles.build_models(rs, specs, tol=tol, max_iterations=max_iterations)
+        for fm in fms:
+            fm.set_precision(precision)
     else:
         meshes, fms = fms_cache
     for fm in fms:
@@ -106,6 +148,55 @@ def run_batch(rs, specs, tol, fms_cache=None, max_iterations=200000):
     return (meshes, fms), res, wall
 
 
+def ensemble_alg_bytes(cache, res, precision):
+    meshes, fms = cache
+    total = 0.0
+    for fm, mesh, r in zip(fms, meshes, res):
+        n, nif = mesh.n_cells(), mesh.n_internal_faces()
+        total += algorithmic_bytes(n, nif, mesh.n_faces() - nif, r.simple_iters, fm.last_inner_iterations(),
+                                   4.0 if precision == "mixed" else 8.0)
+    return total
+
+
+def field_errors(a, b):
+    """Relative L2 of U, k, omega, nu_t of solve a against solve b; p against
+    the dynamic-pressure scale rms(|U|^2) of b (the periodic channel's p is
+    round-off, so it cannot be held against itself)."""
+    rel = lambda x, y: float(np.linalg.norm(np.ravel(x) - np.ravel(y)) / max(np.linalg.norm(np.ravel(y)), 1e-300))
+    out = {k: rel(a[k], b[k]) for k in ("U", "k", "omega", "nuT")}
+    q = np.sqrt(np.mean(np.sum(np.asarray(b["U"]).reshape(-1, 3) ** 2, axis=1) ** 2))
+    out["p"] = float(np.linalg.norm(np.asarray(a["p"]) - np.asarray(b["p"])) / np.sqrt(len(b["p"])) / q)
+    return out
+
+
+def fp64_check(rs, specs, cache, res, tol, max_iterations, sample=32, frac=0.6):
+    """Each observable's error of the mixed-precision solves against FP64 GPU
+    solves of the same members: a sample of `sample` converged members spread
+    evenly over the fastest `frac` of them by outer-iteration count (the
+    slowest need minutes each), re-solved in FP64 in one batch."""
+    meshes, fms = cache
+    conv = sorted((i for i, r in enumerate(res) if str(r.status).split(".")[-1] == "Converged"),
+                  key=lambda i: res[i].simple_iters)
+    if not conv:
+        return None
+    pool = conv[:max(1, int(frac * len(conv)))]
+    pick = pool[::max(1, len(pool) // sample)][:sample]
+    sub = [specs[i] for i in pick]
+    (m64, f64), r64, wall = run_batch(rs, sub, tol, None, max_iterations, "fp64")
+    worst = {k: 0.0 for k in ("U", "p", "k", "omega", "nuT")}
+    iters = []
+    for j, i in enumerate(pick):
+        e = field_errors(fms[i].last_fields(), f64[j].last_fields())
+        for k in worst:
+            worst[k] = max(worst[k], e[k])
+        iters.append((res[i].simple_iters, r64[j].simple_iters))
+    return {"max_rel_l2": max(worst.values()), "per_field": worst, "members": len(pick),
+            "cells": [4 * specs[i]["ny"] for i in pick], "fp64_wall_s": wall,
+            "outer_iters_mixed_vs_fp64": iters,
+            "sample": "%d converged members spread over the fastest %d%% by outer iterations, re-solved in FP64; "
+                      "U, k, omega, nu_t relative L2, p against rms(|U|^2)" % (len(pick), int(100 * frac))}
+
+
 def _member_mesh_file(rs, sp, path):
     y = rs.Mesh.geometric_channel_nodes(sp["ny"] // 2, sp["ratio"], 1.0)
     mesh = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))
@@ -113,15 +204,88 @@ def _member_mesh_file(rs, sp, path):
     return 4 * sp["ny"]
 
 
-def cpu_baseline(specs, tol, results, sample_iters=1500):
-    """The reference solver compiled from its own sources (oracle/_ref) on one
-    host core.  Sample: `sample_iters` outer iterations of three members (the
-    smallest, the median and the largest mesh of the batch) give the CPU time
-    per outer iteration as a function of the cell count (linear fit); the CPU
-    time of the whole batch is the sum over members of their outer-iteration
-    counts (this run's, the same SIMPLE algorithm) times that cost, and the
-    reported value is the batch's converged solves over that time."""
+def lscpu_info():
+    """Host CPU of this machine: model, sockets, physical cores, threads, and
+    the CPUs this process may run on."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        info["physical_cores"] = int(info["sockets"]) * int(info["cores_per_socket"])
+    except (KeyError, ValueError):
+        pass
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+SOLVE_DEFAULT_COEFFS = [0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]
+
+
+def _cpu_member_run(task):
+    """One reference solve of `iters` outer iterations (tolerance 1e-300, so it
+    never stops early) on one pinned core; returns seconds per iteration.
+    Runs in a spawned worker process."""
     import ctypes as C
+    lib_path, kind, path, nu, theta, iters, cpu = task
+    if cpu is not None:
+        os.sched_setaffinity(0, {cpu})
+    # the solver library reports mesh loads on stdout: keep the worker's
+    # stdout off the benchmark's (which carries only the JSON line)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    os.close(devnull)
+    lib = C.CDLL(lib_path)
+    D = lambda a: (C.c_double * len(a))(*a)
+    coeffs = list(SOLVE_DEFAULT_COEFFS)
+    for idx, v in zip([8, 9, 0, 1, 2], theta):
+        coeffs[idx] = v
+    packed = [iters, 1e-300, 1e6, 0.7, 0.3, 0.5, 0.5, 200, 1e-3, 5, 1, 500, 0, 1.0, 0, 0,
+              1e-10, 1e10, 1e-6, 0, 0.41, 0, 0, 2, 2]
+    info = (C.c_int * 5)()
+    t0 = time.perf_counter()
+    if kind == "reference":
+        lib.ref_simple_solve(path.encode(), 0, D([1.0, 1e-4, 1.0]), C.c_double(nu), D(coeffs), D(packed),
+                             D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+    else:
+        lib.om_mesh_load.restype = C.c_void_p
+        m = lib.om_mesh_load(path.encode())
+        lib.om_simple_solve(C.c_void_p(m), 0, D([1.0, 1e-4, 1.0]), C.c_double(nu), D(coeffs), D(packed),
+                            D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
+    return (time.perf_counter() - t0) / max(info[0], 1)
+
+
+def _lpt_makespan(costs, p):
+    """Longest-processing-time-first schedule of independent solves on p cores."""
+    import heapq
+    loads = [0.0] * p
+    for c in sorted(costs, reverse=True):
+        heapq.heapreplace(loads, loads[0] + c)
+    return max(loads)
+
+
+def cpu_baseline(specs, results, sample_iters=2000, max_workers=16):
+    """The reference solver compiled from its own sources (oracle/_ref; the
+    oracle's C restatement where the reference library is absent) on the host
+    cores, timed on a bounded sample of the same workload.
+
+    Sample: `sample_iters` outer iterations of the smallest, median and largest
+    member of the ensemble -- once one after the other on one pinned core
+    (P = 1), once as P concurrent pinned processes, one per physical core of
+    this process's CPU share (at most `max_workers`), the three sizes assigned
+    round-robin (P = all, so memory-bandwidth sharing is in the figure).  Each
+    gives the CPU seconds per outer iteration against the cell count (linear
+    fit); the ensemble's CPU time is this run's per-member outer-iteration
+    counts (the same SIMPLE algorithm, same counts) times that cost, scheduled
+    longest-first on P cores.  value = converged solves / that time at P = all."""
+    import multiprocessing as mp
     lib_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
     kind = "reference"
     if not os.path.exists(lib_path):
@@ -130,54 +294,73 @@ def cpu_baseline(specs, tol, results, sample_iters=1500):
     if not os.path.exists(lib_path):
         return None
     import rans_sst_py as rs
-    lib = C.CDLL(lib_path)
-    D = lambda a: (C.c_double * len(a))(*a)
     by_size = sorted(specs, key=lambda s: s["ny"])
     picks = [by_size[0], by_size[len(by_size) // 2], by_size[-1]]
-    cells, ms = [], []
+    cells, paths = [], []
     for i, sp in enumerate(picks):
         path = "/tmp/bench_cpu_member_%d.msh" % i
-        n = _member_mesh_file(rs, sp, path)
-        coeffs = [0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]
-        for idx, v in zip([8, 9, 0, 1, 2], sp["theta"]):
-            coeffs[idx] = v
-        packed = [sample_iters, 1e-300, 1e6, 0.7, 0.3, 0.5, 0.5, 200, 1e-3, 5, 1, 500, 0, 1.0, 0, 0,
-                  1e-10, 1e10, 1e-6, 0, 0.41, 0, 0, 2, 2]
-        info = (C.c_int * 5)()
-        t0 = time.perf_counter()
-        if kind == "reference":
-            lib.ref_simple_solve(path.encode(), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs),
-                                 D(packed), D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
-        else:
-            lib.om_mesh_load.restype = C.c_void_p
-            m = lib.om_mesh_load(path.encode())
-            lib.om_simple_solve(C.c_void_p(m), 0, D([1.0, 1e-4, 1.0]), C.c_double(sp["nu"]), D(coeffs),
-                                D(packed), D([1, 0, 0, 0, 1e-4, 1.0]), *([None] * 13), None, 0, info)
-        dt = time.perf_counter() - t0
-        cells.append(n)
-        ms.append(1e3 * dt / max(info[0], 1))
-    a, b = np.polyfit(np.array(cells, dtype=float), np.array(ms), 1)
-    cpu_s = sum(max(r["outer"], 1) * (a * r["cells"] + b) for r in results) / 1e3
+        cells.append(_member_mesh_file(rs, sp, path))
+        paths.append(path)
+    cpu = lscpu_info()
+    avail = sorted(os.sched_getaffinity(0))
+    P = max(1, min(len(avail), max_workers, cpu.get("physical_cores", len(avail))))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(P) as pool:
+        # P = 1: the three members one after the other on one core
+        one = [pool.apply(_cpu_member_run, ((lib_path, kind, paths[i], picks[i]["nu"], picks[i]["theta"],
+                                             sample_iters, avail[0]),)) for i in range(3)]
+        allc = pool.map(_cpu_member_run, [(lib_path, kind, paths[w % 3], picks[w % 3]["nu"], picks[w % 3]["theta"],
+                                           sample_iters, avail[w]) for w in range(P)], chunksize=1)
+    loaded = [np.mean([allc[w] for w in range(P) if w % 3 == i] or [one[i]]) for i in range(3)]
     conv = sum(r["converged"] for r in results)
-    return {"value": conv / cpu_s, "unit": "solves/s", "cores": 1, "kind": kind,
-            "sample": "%d outer iterations of 3 members (%s cells): %s ms/iteration on one core; batch CPU "
-                      "time = sum of this run's outer-iteration counts x that cost = %.0f s"
-                      % (sample_iters, "/".join(str(c) for c in cells), "/".join("%.2f" % v for v in ms), cpu_s)}
+
+    def fit(per_iter):
+        a, b = np.polyfit(np.array(cells, dtype=float), np.array(per_iter), 1)
+        return [max(r["outer"], 1) * (a * r["cells"] + b) for r in results]
+
+    t1 = sum(fit(one))
+    tp = _lpt_makespan(fit(loaded), P)
+    return {"value": conv / tp, "unit": "solves/s", "cores": P, "kind": kind,
+            "p1_value": conv / t1,
+            "cpu": {k: cpu.get(k) for k in ("model", "sockets", "physical_cores", "threads_per_core",
+                                            "affinity_cpus")},
+            "sample": "%d outer iterations of the smallest/median/largest member (%s cells): %s ms per iteration "
+                      "on one pinned core, %s ms with %d pinned processes at once; ensemble CPU time from this "
+                      "run's outer-iteration counts: %.0f s on one core, %.0f s on %d cores (longest first)"
+                      % (sample_iters, "/".join(str(c) for c in cells), "/".join("%.3f" % (1e3 * v) for v in one),
+                         "/".join("%.3f" % (1e3 * v) for v in loaded), P, t1, tp, P)}
 
 
 COPY_BYTES = 256 << 20
 
 
-def copy_calibration():
-    """A plain copy kernel of known size (256 MiB read, 256 MiB written), run
-    under the same counters as the solver to scale FETCH_SIZE / WRITE_SIZE."""
-    import torch
-    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    torch.mul(a, 1.0, out=b)
-    torch.cuda.synchronize()
-    print("copy: %d bytes read, %d bytes written" % (COPY_BYTES, COPY_BYTES), flush=True)
+def _rsg_lib():
+    import ctypes as C
+    lib = C.CDLL(os.path.join(PKG, "librsg.so"))
+    lib.rsg_last_error.restype = C.c_char_p
+    return lib
+
+
+def traffic_child(args):
+    """Child of a rocprofv3 --pmc pass: a plain copy kernel of COPY_BYTES on the
+    engine's library (the counter scale), then the headline ensemble capped at
+    --max-iterations outer iterations; prints the capped run's algorithmic
+    bytes (JSON)."""
+    import ctypes as C
+    import rans_sst_py as rs
+    import ensembles
+    lib = _rsg_lib()
+    ctx = C.c_void_p()
+    if lib.rsg_context_create(0, C.c_uint64(1 << 30), C.byref(ctx)) != 0:
+        raise RuntimeError("rsg_context_create failed")
+    sec = C.c_double()
+    if lib.rsg_copy_calibration(ctx, C.c_uint64(COPY_BYTES), C.byref(sec)) != 0:
+        raise RuntimeError("copy calibration: %s" % lib.rsg_last_error(ctx))
+    lib.rsg_context_destroy(ctx)
+    specs = ensembles.channel_ensemble(args.members, args.seed)
+    configure_engine(rs, args)
+    cache, res, _ = run_batch(rs, specs, args.tol, None, args.max_iterations, args.precision)
+    print(json.dumps({"alg_bytes": ensemble_alg_bytes(cache, res, args.precision)}), flush=True)
 
 
 def _pmc_run(counter, argv, tag, tmp):
@@ -185,7 +368,7 @@ def _pmc_run(counter, argv, tag, tmp):
     ([(kernel name, counter KiB)], stdout)."""
     rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     out = os.path.join(tmp, tag)
-    cmd = ["timeout", "-s", "KILL", "240", rocprof, "--pmc", counter, "-d", out, "-o", "run", "--",
+    cmd = ["timeout", "-s", "KILL", "120", rocprof, "--pmc", counter, "-d", out, "-o", "run", "--",
            sys.executable, "-u"] + argv
     r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True, text=True)
     if r.returncode != 0:
@@ -202,43 +385,39 @@ def _pmc_run(counter, argv, tag, tmp):
     return rows, r.stdout
 
 
-def measure_traffic(args):
+def measure_traffic(args, cap=300):
     """HBM traffic of the chunk kernel relative to its algorithmic bytes.
 
-    Two counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over
-    the same ensemble capped at 1000 outer iterations, each scaled by what the
-    counter reads for a plain copy kernel of known size; the ratio of the
-    scaled bytes to the algorithmic bytes of the same pass."""
+    Two counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
+    over one child process that runs the copy kernel of known size and then the
+    same ensemble capped at `cap` outer iterations; each counter is scaled by
+    what it reads for the copy, and the scaled bytes are divided by the capped
+    run's algorithmic bytes."""
     tmp = tempfile.mkdtemp(prefix="bench_pmc_", dir="/tmp")
     try:
         me = os.path.abspath(__file__)
-        cal = {}
-        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            rows, _ = _pmc_run(counter, [me, "--copy-calibration"], "cal_" + counter, tmp)
-            kib = [v for k, v in rows if "MulFunctor" in k]
-            if not kib:
-                raise RuntimeError("copy kernel not found in the %s calibration" % counter)
-            cal[counter] = COPY_BYTES / (1024.0 * kib[0])
-        work = [me, "--members", str(args.members), "--seed", str(args.seed), "--tol", str(args.tol),
-                "--iters-per-launch", str(args.iters_per_launch), "--block", str(args.block),
-                "--slice-ms", str(args.slice_ms), "--schedule", args.schedule,
-                "--max-iterations", "1000", "--no-cpu", "--no-traffic"]
-        hbm, alg, launches = {}, {}, {}
+        work = [me, "--traffic-child", "--members", str(args.members), "--seed", str(args.seed), "--tol",
+                str(args.tol), "--iters-per-launch", str(args.iters_per_launch), "--block", str(args.block),
+                "--slice-ms", str(args.slice_ms), "--schedule", args.schedule, "--precision", args.precision,
+                "--max-iterations", str(cap)]
+        out = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             rows, stdout = _pmc_run(counter, work, counter, tmp)
+            cal = [v for k, v in rows if "k_copy_calibration" in k]
             kib = [v for k, v in rows if "k_simple_chunk" in k]
-            line = json.loads(stdout.strip().splitlines()[-1])
-            hbm[counter] = 1024.0 * sum(kib) * cal[counter]
-            alg[counter] = line["roofline"]["alg_bytes_total"]
-            launches[counter] = len(kib)
-        ratio = hbm["FETCH_SIZE"] / alg["FETCH_SIZE"] + hbm["WRITE_SIZE"] / alg["WRITE_SIZE"]
-        return {"hbm_over_algorithmic": ratio,
-                "fetch_scale": cal["FETCH_SIZE"], "write_scale": cal["WRITE_SIZE"],
-                "fetch_bytes": hbm["FETCH_SIZE"], "write_bytes": hbm["WRITE_SIZE"],
-                "alg_bytes": alg["FETCH_SIZE"], "launches": launches["FETCH_SIZE"],
-                "sample": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over the same %d-member ensemble "
-                          "capped at 1000 outer iterations; counters scaled by a 256 MiB copy kernel"
-                          % args.members}
+            if not cal or not kib:
+                raise RuntimeError("%s pass: copy or chunk kernel missing from the counters" % counter)
+            scale = COPY_BYTES / (1024.0 * cal[0])
+            alg = json.loads(stdout.strip().splitlines()[-1])["alg_bytes"]
+            out[counter] = dict(scale=scale, bytes=1024.0 * sum(kib) * scale, alg=alg, launches=len(kib))
+        f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+        ratio = f["bytes"] / f["alg"] + w["bytes"] / w["alg"]
+        return {"hbm_over_algorithmic": ratio, "fetch_scale": f["scale"], "write_scale": w["scale"],
+                "fetch_bytes": f["bytes"], "write_bytes": w["bytes"], "alg_bytes": f["alg"],
+                "launches": f["launches"],
+                "sample": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over the same %d-member ensemble capped "
+                          "at %d outer iterations; counters scaled by a %d MiB copy kernel (rsg_copy_calibration)"
+                          % (args.members, cap, COPY_BYTES >> 20)}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
@@ -551,24 +730,139 @@ def bench_c1(args, big=False):
         "roofline": None, "cpu_baseline": cpu}), flush=True)
 
 
+def bench_c2(args):
+    """The headline: the C2 ensemble, one timed step = one full ensemble."""
+    t_begin = time.perf_counter()
+    remaining = lambda: args.budget_s - (time.perf_counter() - t_begin)
+    import rans_sst_py as rs
+    import ensembles
+    specs = ensembles.channel_ensemble(args.members, args.seed)
+    configure_engine(rs, args)
+    # engine start-up, code-object load and one short pass of every member
+    # (100 outer iterations) outside the timed region
+    warm = 0
+    if args.warmup > 0:
+        _, _, wall = run_batch(rs, specs, args.tol, None, 100, args.precision)
+        warm = 1
+        print("bench: warmup pass (100 outer iterations per member) %.2f s" % wall, file=sys.stderr, flush=True)
+    else:
+        _, prime = ensembles.build_models(rs, specs[:1], tol=args.tol, max_iterations=2)
+        rs.evaluate_many(prime, [specs[0]["theta"]])
+    stats0 = rs.engine_stats()
+    walls, lat, conv, outer = [], [], 0, []
+    cache, res = None, None
+    # each step is a whole ensemble (minutes); further steps run only while
+    # they fit the step budget
+    while len(walls) < max(args.steps, 1):
+        cache, res, wall = run_batch(rs, specs, args.tol, cache, args.max_iterations, args.precision)
+        walls.append(wall)
+        print("bench: step %d %.2f s" % (len(walls), wall), file=sys.stderr, flush=True)
+        for r in res:
+            lat.append(r.latency)
+            conv += str(r.status).split(".")[-1] == "Converged"
+            outer.append(r.simple_iters)
+        if sum(walls) + wall > args.step_budget_s:
+            break
+    stats1 = rs.engine_stats()
+    steps = len(walls)
+    total = sum(walls)
+    launches = stats1[0] - stats0[0]
+    kernel_ms = stats1[3] - stats0[3]
+    bytes_alg = ensemble_alg_bytes(cache, res, args.precision) * steps
+    achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    alg_per_launch = bytes_alg / max(launches, 1)
+    check = None
+    if args.precision == "mixed":
+        if remaining() > 150:
+            check = fp64_check(rs, specs, cache, res, args.tol, args.max_iterations)
+            print("bench: FP64 check done", file=sys.stderr, flush=True)
+        else:
+            check = {"skipped": "not enough of the time budget left"}
+    cpu = None
+    if not args.no_cpu and remaining() > 60:
+        cpu = cpu_baseline(specs, [dict(outer=r.simple_iters, cells=m.n_cells(),
+                                        converged=str(r.status).split(".")[-1] == "Converged")
+                                   for r, m in zip(res, cache[0])])
+        print("bench: cpu baseline done", file=sys.stderr, flush=True)
+    traffic, traffic_info = None, None
+    if not args.no_traffic:
+        if remaining() > 150:
+            try:
+                traffic_info = measure_traffic(args)
+                traffic = traffic_info["hbm_over_algorithmic"] * alg_per_launch
+            except Exception as e:   # counters unavailable: report null with the reason
+                traffic_info = {"error": str(e)[:300]}
+            print("bench: traffic passes done", file=sys.stderr, flush=True)
+        else:
+            traffic_info = {"skipped": "not enough of the time budget left"}
+    out = {
+        "metric": METRIC,
+        "value": conv / total,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": 1e3 * total / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-defect" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "C2 batched ensemble: %d periodic channels 4x(96-384), Re_tau logU[180,2000], "
+                               "betaStar/a1/sigma_k1/sigma_w1/beta1 +-20%%, SIMPLE+SST to tol %g, %s"
+                               % (args.members, args.tol,
+                                  "FP32 inner solves with FP64 refinement" if args.precision == "mixed" else "FP64"),
+                   "members": args.members, "tol": args.tol, "seed": args.seed, "precision": args.precision,
+                   "max_iterations": args.max_iterations, "iters_per_launch": args.iters_per_launch,
+                   "block": args.block, "slice_ms": args.slice_ms, "schedule": args.schedule},
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "step_definition": "one step = the whole ensemble solved cold (every member submitted at once, timed to "
+                           "the last retirement); steps beyond the first run only while they fit %.0f s"
+                           % args.step_budget_s,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "converged": int(conv),
+        "solves": len(lat),
+        "outer_iters_median": float(np.median(outer)),
+        "max_rel_l2_vs_fp64": (check or {}).get("max_rel_l2") if args.precision == "mixed" else 0.0,
+        "fp64_check": check if args.precision == "mixed" else "the measured path is the FP64 path itself",
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_simple_chunk", "launches": launches,
+                     "avg_launch_ms": kernel_ms / max(launches, 1),
+                     "alg_bytes_per_launch": alg_per_launch, "alg_bytes_total": bytes_alg,
+                     "traffic_measurement": traffic_info},
+        "cpu_baseline": cpu,
+        "bench_wall_s": time.perf_counter() - t_begin,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="member inner solves: FP64, or FP32 with FP64 defect and refinement")
     ap.add_argument("--iters-per-launch", type=int, default=500)
-    ap.add_argument("--block", type=int, default=0, help="0: engine chooses per launch (256 or 512)")
+    ap.add_argument("--block", type=int, default=0, help="0: engine default (512)")
     ap.add_argument("--slice-ms", type=float, default=200.0,
                     help="time slice per launch in ms (0: fixed --iters-per-launch budget)")
     ap.add_argument("--schedule", choices=("round_robin", "largest_first"), default="largest_first",
                     help="which members a launch runs when more are active than the device holds")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="headline: wall-clock budget of the whole run; optional legs are skipped when it runs out")
+    ap.add_argument("--step-budget-s", type=float, default=300.0,
+                    help="headline: timed steps after the first run only while the total stays within this")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
-    ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump"), default="c2",
                     help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
                          "c0: single-solve latency of the Re_tau 550 channel; walldist: the wall-distance "
@@ -581,112 +875,15 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=200000,
                     help="outer-iteration cap per member (profiling runs shorten it)")
     args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration()
+    if args.traffic_child:
+        traffic_child(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU (--gpus 1): ensemble members would shard across GPUs with no "
               "exchange between them, and runs on several GPUs are not part of this benchmark", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "c1":
-        bench_c1(args)
-        return
-    if args.workload == "c0":
-        bench_c0(args)
-        return
-    if args.workload == "c4p":
-        bench_c1(args, big=True)
-        return
-    if args.workload == "walldist":
-        bench_walldist(args)
-        return
-    if args.workload == "bump":
-        bench_bump(args)
-        return
-    import rans_sst_py as rs
-    import ensembles
-    specs = ensembles.channel_ensemble(args.members, args.seed)
-    rs.engine_set_launch(args.iters_per_launch, args.block)
-    rs.engine_set_time_slice(int(args.slice_ms * 1000))
-    rs.engine_set_schedule(args.schedule)
-    # engine start-up and code-object load, outside the timed region
-    _, prime = ensembles.build_models(rs, specs[:1], tol=args.tol, max_iterations=2)
-    rs.evaluate_many(prime, [specs[0]["theta"]])
-    cache = None
-    for _ in range(args.warmup):
-        cache, _, wall = run_batch(rs, specs, args.tol, cache, args.max_iterations)
-        print("bench: warmup step %.2f s" % wall, file=sys.stderr, flush=True)
-    stats0 = rs.engine_stats()
-    walls, lat, conv, outer = [], [], 0, []
-    res = None
-    for _ in range(args.steps):
-        cache, res, wall = run_batch(rs, specs, args.tol, cache, args.max_iterations)
-        walls.append(wall)
-        print("bench: step %d %.2f s" % (len(walls), wall), file=sys.stderr, flush=True)
-        for r in res:
-            lat.append(r.latency)
-            ok = str(r.status).split(".")[-1] == "Converged"
-            conv += ok
-            outer.append(r.simple_iters)
-    stats1 = rs.engine_stats()
-    total = sum(walls)
-    launches = stats1[0] - stats0[0]
-    kernel_ms = stats1[3] - stats0[3]
-    # algorithmic bytes of the timed steps
-    meshes, fms = cache
-    bytes_alg = 0.0
-    for fm, mesh, r in zip(fms, meshes, res):
-        inner = fm.last_inner_iterations()
-        n, nif = mesh.n_cells(), mesh.n_internal_faces()
-        bytes_alg += algorithmic_bytes(n, nif, mesh.n_faces() - nif, r.simple_iters, inner)
-    bytes_alg *= args.steps
-    achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    alg_per_launch = bytes_alg / max(launches, 1)
-    traffic, traffic_info = None, None
-    if not args.no_traffic:
-        try:
-            traffic_info = measure_traffic(args)
-            traffic = traffic_info["hbm_over_algorithmic"] * alg_per_launch
-        except Exception as e:   # counters unavailable: report null with the reason
-            traffic_info = {"error": str(e)[:300]}
-        print("bench: traffic passes done", file=sys.stderr, flush=True)
-    out = {
-        "metric": METRIC,
-        "value": conv / total,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "C2 batched ensemble: %d periodic channels 4x(96-384), Re_tau logU[180,2000], "
-                               "betaStar/a1/sigma_k1/sigma_w1/beta1 +-20%%, SIMPLE+SST to tol %g, FP64"
-                               % (args.members, args.tol),
-                   "members": args.members, "tol": args.tol, "seed": args.seed,
-                   "iters_per_launch": args.iters_per_launch, "block": args.block,
-                   "slice_ms": args.slice_ms, "schedule": args.schedule},
-        "p50_latency_s": float(np.percentile(lat, 50)),
-        "p99_latency_s": float(np.percentile(lat, 99)),
-        "converged": int(conv),
-        "solves": len(lat),
-        "outer_iters_median": float(np.median(outer)),
-        "max_rel_l2_vs_fp64": 0.0,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_simple_chunk", "launches": launches,
-                     "avg_launch_ms": kernel_ms / max(launches, 1),
-                     "alg_bytes_per_launch": alg_per_launch, "alg_bytes_total": bytes_alg,
-                     "traffic_measurement": traffic_info},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(
-            specs, args.tol, [dict(outer=r.simple_iters, cells=m.n_cells(),
-                                   converged=str(r.status).split(".")[-1] == "Converged")
-                              for r, m in zip(res, meshes)]),
-    }
-    print(json.dumps(out), flush=True)
+    {"c1": bench_c1, "c0": bench_c0, "walldist": bench_walldist, "bump": bench_bump,
+     "c4p": lambda a: bench_c1(a, big=True), "c2": bench_c2}[args.workload](args)
 
 
 if __name__ == "__main__":

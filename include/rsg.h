@@ -71,9 +71,15 @@ extern "C" {
 #define RSG_SOLVER_PCG 1
 #define RSG_SOLVER_BICGSTAB 2
 
-/* precision of a member's inner linear solves */
+/* precision of a member's inner linear solves.  RSG_PREC_MIXED: every linear
+ * solve of the SIMPLE iteration (momentum, p', k, omega) computes the defect
+ * b - A x in FP64, solves A d = r in FP32 (FP32 vectors and matrix values,
+ * FP32-accumulated reductions, the AMG V-cycle in FP32 around the FP64 dense
+ * coarsest inverse), updates x += d in FP64 and repeats until the FP64 defect
+ * is below the inner tolerance (iterative refinement); assembly, the fields,
+ * the near-wall omega and the residual accounting stay FP64. */
 #define RSG_PREC_FP64 0
-#define RSG_PREC_MIXED 1          /* FP32 inner Krylov, FP64 defect/refinement */
+#define RSG_PREC_MIXED 1
 
 /* ticket states */
 #define RSG_TICKET_PENDING 0
@@ -289,6 +295,13 @@ int rsg_amg_aggregate(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
  * Host buffers.  kernel_seconds (nullable) gets the kernel's duration. */
 int rsg_wall_distance(rsg_context* ctx, int32_t n_cells, const double* cell_centers, int32_t n_wall,
                       const double* wall_points, double* wall_dist, double* kernel_seconds);
+
+/* Measurement utility (no reference counterpart): one plain device copy of
+ * `bytes` bytes (a multiple of 16), read once and written once, on the engine
+ * stream; kernel_seconds (nullable) gets its duration.  Run under hardware
+ * byte counters it gives the scale between what FETCH_SIZE / WRITE_SIZE
+ * report and the bytes really moved. */
+int rsg_copy_calibration(rsg_context* ctx, uint64_t bytes, double* kernel_seconds);
 
 #ifdef __cplusplus
 }

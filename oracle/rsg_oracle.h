@@ -50,13 +50,6 @@ int      om_patch_type(const om_mesh* m, int p);
 
 int om_green_gauss(const om_mesh* m, const double* phi, const double* phib, double* grad);
 
-/* SST closure over the mesh (SSTModel::computeFields) and |S|. */
-int om_sst_fields(const om_mesh* m, const double* coeffs, int variant, double nu,
-                  const double* U, const double* Ub, const double* k, const double* kb,
-                  const double* om, const double* omb,
-                  double* nuT, double* F1, double* F2, double* Pk, double* CDkw,
-                  double* Smag);
-
 double om_sst_point(int which, const double* coeffs, int variant, const double* a);
 
 /* One LDU linear solve.  out = {iterations, finalRes, initialRes, converged} */

@@ -102,6 +102,44 @@ def gpu_ctx(rsg_lib):
     rsg_lib.rsg_context_destroy(ctx)
 
 
+ENGINE_DEFAULTS = dict(iters_per_launch=200, block=0, slice_us=0, schedule=0, team=(0, 0, 20))
+
+
+def reset_engine(lib, ctx):
+    """The engine's tuning back to its defaults (rsg_context_create's values)."""
+    d = ENGINE_DEFAULTS
+    lib.rsg_set_launch(ctx, d["iters_per_launch"], d["block"])
+    lib.rsg_set_time_slice(ctx, d["slice_us"])
+    lib.rsg_set_schedule(ctx, d["schedule"])
+    lib.rsg_set_team(ctx, *d["team"])
+
+
+@pytest.fixture(autouse=True)
+def _engine_defaults(request):
+    """Every GPU test starts and ends with the engine at its default tuning, so
+    no test runs whatever launch shape, slice, schedule or team an earlier one
+    left on the session-scoped context (librsg's or rans_sst_py's)."""
+    uses_ctx = "gpu_ctx" in request.fixturenames
+    uses_rs = "rs" in request.fixturenames and request.node.get_closest_marker("gpu") is not None
+    if uses_ctx:
+        reset_engine(request.getfixturevalue("rsg_lib"), request.getfixturevalue("gpu_ctx"))
+    if uses_rs:
+        _reset_rs(request.getfixturevalue("rs"))
+    yield
+    if uses_ctx:
+        reset_engine(request.getfixturevalue("rsg_lib"), request.getfixturevalue("gpu_ctx"))
+    if uses_rs:
+        _reset_rs(request.getfixturevalue("rs"))
+
+
+def _reset_rs(rs):
+    d = ENGINE_DEFAULTS
+    rs.engine_set_launch(d["iters_per_launch"], d["block"])
+    rs.engine_set_time_slice(d["slice_us"])
+    rs.engine_set_schedule("round_robin")
+    rs.engine_set_team(*d["team"])
+
+
 def c0_mesh(rs, tmp_path_factory=None):
     y = rs.Mesh.geometric_channel_nodes(64, 1.06, 1.0)
     mesh = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))

@@ -15,6 +15,8 @@
 // paths can be held against the oracle.
 static int g_resident = 1;
 static int g_stage = 1;      // LDS copies of the coarse AMG images
+static int g_prec = 0;       // RSG_PREC_* of the emulated solves
+extern "C" void emu_set_precision(int p) { g_prec = p; }
 extern "C" void emu_set_stage(int on) { g_stage = on; }
 extern "C" int emu_last_staged_first(void) { return rsg::rsg_emu_staged_first; }
 extern "C" void emu_set_resident(int on) { g_resident = on; }
@@ -66,6 +68,7 @@ static int emu_run(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, double nu,
     for (int i = 0; i < 11; ++i) M.coeffs[i] = coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = init[i];
     M.initFromFields = state != nullptr;
+    M.prec = g_prec;
     M.planarZ = rsg::planar_z(P, P.nB == 0 || rsg::bc_zero_z(bcd, P.nB), s, state ? 0.0 : init[2],
                               state ? state[0] : nullptr, state && P.nB > 0 ? state[1] : nullptr) ? 1 : 0;
     M.iterBudget = iterBudget > 0 ? iterBudget : s->max_iterations;
@@ -141,6 +144,7 @@ extern "C" int emu_ldu_solve(int n, int nif, const int* own, const int* nbr, con
     rsg::Carver cv; cv.base = wblk.data();
     rsg::carve_member(cv, P.nC, P.nIF, P.nB, P.nnz0, 0, M);
     M.mesh = &md;
+    M.prec = g_prec;
     memcpy(M.diag, diag, 8 * (size_t)n); memcpy(M.src, b, 8 * (size_t)n); memcpy(M.pp, x, 8 * (size_t)n);
     if (nif > 0) { memcpy(M.upper, upper, 8 * (size_t)nif); memcpy(M.lower, lower, 8 * (size_t)nif); }
     M.sc->status = RSGM_RUNNING;
