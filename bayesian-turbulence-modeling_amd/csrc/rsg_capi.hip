@@ -40,6 +40,20 @@
 int rsg_team_capacity(int device);
 hipError_t rsg_team_launch(RsgMemberDev* dM, RsgScalars* dOut, double* gred, int blocks, hipStream_t s);
 
+// Workspace of rsg_ldu_solve_noalloc: one stand-alone LDU solve's device block
+// (mesh image, member struct, member workspace) lives here instead of in the
+// context's arena.  Each GPU holds its own copy until the process ends; host
+// code reaches it only through the per-device address that
+// rsg_context_create resolves with hipGetSymbolAddress (ctx->scratchDev).
+#define RSG_NOALLOC_SCRATCH_BYTES (8u << 20)
+static __device__ __align__(256) char scratch[RSG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+// one lock per device: calls through any context of the process on the same
+// GPU take turns on that GPU's scratch array
+std::mutex g_scratch_mu[64];
+}  // namespace
+
 // ------------------------------------------------------------------ kernels
 namespace {
 
@@ -207,6 +221,7 @@ struct rsg_context {
     // stats
     double nLaunch = 0, nAdmitted = 0, nRetired = 0, kernelMs = 0, maxActive = 0;
     int cuCount = 0, warpSize = 64;
+    char* scratchDev = nullptr;   // this device's copy of the no-allocation scratch array
 };
 
 namespace {
@@ -543,6 +558,13 @@ int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** 
     c->warpSize = prop.warpSize;
     if (hipDeviceGetAttribute(&c->clockKHz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->clockKHz = 0;
     c->teamCap = rsg_team_capacity(device);
+    {
+        // the no-allocation solve's workspace: this device's copy of the
+        // scratch array (host code uses this address, never the symbol)
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return RSG_ERR_DEVICE;
+        c->scratchDev = static_cast<char*>(p);
+    }
     c->resident256 = rsg_chunk_occupancy(256) * c->cuCount;
     c->resident512 = rsg_chunk_occupancy(512) * c->cuCount;
     if (hbm_budget_bytes == 0) hbm_budget_bytes = 8ull << 30;
@@ -929,10 +951,14 @@ int rsg_stats(rsg_context* ctx, double* out) {
     return RSG_OK;
 }
 
-int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
-                  const double* diag, const double* upper, const double* lower, const double* b,
-                  double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
-                  double* stats) {
+}  // extern "C"
+
+namespace {
+
+// Argument checks and host-side topology of one stand-alone LDU solve.
+int ldu_prepare(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                const double* diag, const double* upper, const double* lower, const double* b, double* x,
+                int32_t solver, int32_t precision, double* stats, rsg::PrepMesh& pm) {
     if (!ctx || n <= 0 || nif < 0 || !own || (nif > 0 && !nbr) || !diag || (nif > 0 && (!upper || !lower)) ||
         !b || !x || !stats)
         return RSG_ERR_ARG;
@@ -942,18 +968,28 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
         return set_err(ctx, RSG_ERR_ARG, "unknown solver code");
     for (int f = 0; f < nif; ++f)
         if (own[f] < 0 || own[f] >= n || nbr[f] < 0 || nbr[f] >= n) return set_err(ctx, RSG_ERR_ARG, "face index out of range");
-    rsg::PrepMesh pm;
     std::string err;
     if (!rsg::prep_topology(n, nif, own, nbr, pm, err)) return set_err(ctx, RSG_ERR_ARG, err);
-    rsg::MeshOffsets mo = rsg::mesh_offsets(pm);
+    return RSG_OK;
+}
+
+// Device bytes of one stand-alone LDU solve laid out from `base`: the mesh
+// image and its struct, the member struct, the member workspace.
+size_t ldu_block_bytes(const rsg::PrepMesh& pm, const rsg::MeshOffsets& mo) {
     const size_t mbytes = mo.total + kMeshStructSlot;
     const size_t wbytes = rsg::member_bytes(pm.nC, pm.nIF, pm.nB, pm.nnz0, 0);
-    const size_t total = mbytes + 256 + sizeof(RsgMemberDev) + 256 + wbytes + 256;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    size_t off;
-    if (!ctx->arena.alloc(total, off)) return set_err(ctx, RSG_ERR_NOMEM, "system exceeds the HBM budget");
+    return mbytes + 256 + sizeof(RsgMemberDev) + 256 + wbytes + 256;
+}
+
+// Runs one stand-alone LDU solve in the device block at `base` (ctx->mu
+// held): stages the topology and the system, launches the one-workgroup
+// solve and copies x and the statistics back.  Allocates no device memory.
+int ldu_solve_at(rsg_context* ctx, char* base, const rsg::PrepMesh& pm, const rsg::MeshOffsets& mo,
+                 const double* diag, const double* upper, const double* lower, const double* b, double* x,
+                 int32_t solver, int32_t precision, int32_t max_iter, double tol, double* stats) {
+    const int n = pm.nC, nif = pm.nIF;
+    const size_t mbytes = mo.total + kMeshStructSlot;
     hipSetDevice(ctx->device);
-    char* base = ctx->arena.base + off;
     std::vector<char> stage(mbytes, 0);
     rsg::mesh_stage(pm, mo, stage.data());
     RsgMeshDev mview = rsg::mesh_view(pm, mo, base);
@@ -984,10 +1020,53 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
     double st[5] = {0, 0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpy(st, M.gp[0], sizeof st, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(x, M.pp, 8 * (size_t)n, hipMemcpyDeviceToHost);
-    ctx->arena.release(off, total);
     if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
     for (int i = 0; i < 4; ++i) stats[i] = st[i];
     return RSG_OK;
+}
+
+}  // namespace
+
+
+extern "C" {
+
+int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                  const double* diag, const double* upper, const double* lower, const double* b,
+                  double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
+                  double* stats) {
+    rsg::PrepMesh pm;
+    int rc = ldu_prepare(ctx, n, nif, own, nbr, diag, upper, lower, b, x, solver, precision, stats, pm);
+    if (rc != RSG_OK) return rc;
+    rsg::MeshOffsets mo = rsg::mesh_offsets(pm);
+    const size_t total = ldu_block_bytes(pm, mo);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    size_t off;
+    if (!ctx->arena.alloc(total, off)) return set_err(ctx, RSG_ERR_NOMEM, "system exceeds the HBM budget");
+    rc = ldu_solve_at(ctx, ctx->arena.base + off, pm, mo, diag, upper, lower, b, x, solver, precision, max_iter, tol,
+                      stats);
+    ctx->arena.release(off, total);
+    return rc;
+}
+
+int rsg_ldu_solve_noalloc(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                          const double* diag, const double* upper, const double* lower, const double* b,
+                          double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
+                          double* stats) {
+    rsg::PrepMesh pm;
+    int rc = ldu_prepare(ctx, n, nif, own, nbr, diag, upper, lower, b, x, solver, precision, stats, pm);
+    if (rc != RSG_OK) return rc;
+    if (!ctx->scratchDev) return set_err(ctx, RSG_ERR_DEVICE, "the scratch array's device address is unknown");
+    rsg::MeshOffsets mo = rsg::mesh_offsets(pm);
+    const size_t total = ldu_block_bytes(pm, mo);
+    if (total > RSG_NOALLOC_SCRATCH_BYTES)
+        return set_err(ctx, RSG_ERR_ARG, "rsg_ldu_solve_noalloc: the system needs " + std::to_string(total) +
+                                             " bytes of device workspace, more than the " +
+                                             std::to_string(RSG_NOALLOC_SCRATCH_BYTES) +
+                                             "-byte scratch array (rsg_ldu_solve has no such limit)");
+    std::lock_guard<std::mutex> dk(g_scratch_mu[ctx->device & 63]);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return ldu_solve_at(ctx, ctx->scratchDev, pm, mo, diag, upper, lower, b, x, solver, precision, max_iter, tol,
+                        stats);
 }
 
 int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,

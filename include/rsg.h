@@ -258,6 +258,24 @@ int rsg_ldu_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, 
                   double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
                   double* stats);
 
+/* rsg_ldu_solve without any device allocation: the same solve (same
+ * arguments, kernels and results) with its device workspace in a fixed
+ * 8 MiB static device array instead of the context's arena.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc / hipFree.  Systems whose workspace
+ * (mesh image + Krylov vectors + AMG hierarchy, ~1.55 KB per cell on quad
+ * meshes) exceeds 8 MiB -- about 5,400 cells -- are rejected with
+ * RSG_ERR_ARG and a message naming the bytes needed.  Every call on one GPU
+ * uses that GPU's one array (its contents persist from the last call and are
+ * rewritten by the next): overlapping calls on one GPU would race, so the
+ * library serialises them with a lock per device (the array is per process,
+ * so other processes have their own). */
+int rsg_ldu_solve_noalloc(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
+                          const double* diag, const double* upper, const double* lower, const double* b,
+                          double* x, int32_t solver, int32_t precision, int32_t max_iter, double tol,
+                          double* stats);
+
 /* One large LDU linear solve spread over the whole GPU (the stand-alone
  * pressure-type system): PCG from x (in/out, host buffers) preconditioned by
  * the AMG V-cycle (solver RSG_SOLVER_AMG, hierarchy built on the host with the
