@@ -174,6 +174,14 @@ def meshes(rs, tmp_path_factory):
     m = rs.Mesh.load_from_file(str(d / "tri_quad"))
     m.set_patch_type("inlet", "inlet"); m.set_patch_type("outlet", "outlet")
     out["tri_quad"] = str(d / "tri_quad.msh"); m.save_binary(out["tri_quad"])
+    # periodic channel with a Gaussian bump (0.3 x 1 on the lower wall, length
+    # 8, height 2): the bump workload's geometry at 80 x 40 cells
+    x = np.linspace(0.0, 8.0, 81)
+    g = np.exp(-((x - 4.0) / 1.0) ** 2)
+    yb = 0.3 * (g - g[0]); yb[-1] = yb[0]
+    m = rs.Mesh.make_curved_channel_periodic_2d(x, yb, 2.0, 40, 1667.0, 1.0)
+    m.compute_wall_distance()
+    out["bump"] = str(d / "bump.msh"); m.save_binary(out["bump"])
     return out
 
 
@@ -186,6 +194,7 @@ CASES = {
     "plate": (2, (1.0, 1e-4, 10.0), 1e-5, (0.0, 0.0, 0.0)),
     "hill": (0, (1.0, 1e-4, 1.0), 5e-4, (5e-3, 0.0, 0.0)),
     "tri_quad": (0, (1.0, 1e-4, 10.0), 2e-4, (0.0, 0.0, 0.0)),
+    "bump": (0, (1.0, 1e-4, 1.0), 6e-4, (1.0, 0.0, 0.0)),
 }
 
 FIELD_KEYS = ("U", "p", "k", "om", "nuT", "F1", "F2", "Pk", "CDkw")

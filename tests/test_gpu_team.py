@@ -3,7 +3,9 @@ workgroups of one cooperative launch, for meshes beyond one workgroup.
 
 * trajectories against the oracle on the parity meshes, as the member tests
   do (Jacobi-PCG pressure; differences only from reduction order), with a
-  team of 4 workgroups and with the whole device;
+  team of 4 workgroups and with the whole device, and on the bump channel
+  (3,200 cells, the bump workload's geometry) with 16 workgroups and the
+  whole device;
 * the team and the one-workgroup solve of the same member agree to round-off,
   with the AMG pressure solver.
 """
@@ -32,7 +34,8 @@ def team_solve(lib, ctx, path, case, s, blocks, iters_per_launch=25):
 
 
 @pytest.mark.parametrize("case,iters,blocks", [("bfs", 80, 4), ("tri_quad", 80, 4), ("hill", 120, 4),
-                                               ("c0", 200, 4), ("channel_bounded", 150, 0)])
+                                               ("c0", 200, 4), ("channel_bounded", 150, 0), ("bump", 120, 16),
+                                               ("bump", 120, 0)])
 def test_team_trajectory_matches_oracle(rsg_lib, gpu_ctx, oracle, meshes, case, iters, blocks):
     s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=1,
                      nut_floor_iters=min(500, iters // 2), record_history=1)
