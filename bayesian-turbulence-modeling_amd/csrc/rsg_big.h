@@ -1,0 +1,22 @@
+// rsg_big.h -- host interface of the kernel-per-phase path for single large
+// members (rsg_big.hip), used by the engine in rsg_capi.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "rsg_layout.h"
+#include "rsg_prep.h"
+
+struct RsgBig;
+
+// State of one large member whose workspace (hM, device pointers) is already
+// carved and staged; dM: device memory for the member struct.  sc0: its
+// initial scalars.  Returns null (and err) on a device error.
+RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set,
+                       const rsg::PrepMesh& pm, const RsgScalars& sc0, hipStream_t s, std::string& err);
+// Runs up to `iters` outer iterations (fewer if the member finishes); out gets
+// the member's scalars.  A finished member's fields, history, scalars and
+// observables are on the device for collection.
+int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err);
+void rsg_big_destroy(RsgBig* B);

@@ -1,0 +1,677 @@
+// rsg_big.hip -- one large member (10^5..10^7 cells) solved by kernel-per-phase
+// launches over the whole GPU.
+//
+// The ensemble kernels give a member one workgroup; a member far beyond one
+// workgroup's reach instead runs every phase of the outer iteration as its
+// own grid-wide launch: the per-cell and per-face code of rsg_member.h
+// (assembly, Green-Gauss, SST closure, boundary values, corrections, the wall
+// omega pin) with one thread per cell, and the linear solves on the CSR image
+// of each assembled LDU system with the grid kernels of rsg_grid.h
+// (deterministic block-partial dot products).  Between phases the GPU drains
+// one launch and starts the next (a few microseconds), instead of the
+// cooperative team's grid barrier (tens of microseconds), and each phase
+// streams its arrays at HBM rate.
+//
+// The p' AMG hierarchy is aggregated on the host (the reference's greedy
+// pass, rsg::grid::coarsen) from the p' matrix of the outer iteration that
+// needs it, every amg_refresh outer iterations; level 0's values follow the
+// current matrix every iteration and the coarse Galerkin values are refreshed
+// on the device every coarse_refresh iterations (gather lists in the
+// reference's summation order), like the member path's device AMG.
+//
+// Reference restated (as rsg_member.h): SIMPLESolver::solve
+// (incompressible/src/SIMPLESolver.cpp:731-1036) phase by phase, the
+// BiCGSTAB / PCG / AMG-PCG solvers of core/include/LinearSolver.hpp:147-247,
+// 497-656.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rsg.h"
+#include "rsg_layout.h"
+#include "rsg_prep.h"
+#include "rsg_member.h"
+#include "rsg_grid.h"
+#include "rsg_big.h"
+
+namespace rsg {
+namespace big {
+
+using grid::kBlock;
+
+__device__ inline Ctx gctx(RsgMemberDev* M) {
+    Thr t;
+    t.tid = blockIdx.x * blockDim.x + threadIdx.x;
+    t.bs = gridDim.x * blockDim.x;
+    t.red = nullptr; t.lds = nullptr; t.phase = 0;
+    t.xb0 = nullptr; t.xb1 = nullptr; t.xsel = 0; t.cvec = nullptr; t.cvecCap = 0;
+    t.ceVal = nullptr; t.ceCol = nullptr; t.ceCap = 0; t.caCol = nullptr; t.caCap = 0;
+    t.ltid = threadIdx.x; t.lbs = blockDim.x; t.blk = blockIdx.x; t.nblk = gridDim.x; t.gred = nullptr;
+    Ctx C;
+    ctx_init(C, t, M);
+    return C;
+}
+
+#define BIG_CELLS(c) for (int c = C.t.tid; c < C.nC; c += C.t.bs)
+#define BIG_BFACES(b) for (int b = C.t.tid; b < C.nB; b += C.t.bs)
+
+// initUniform's field values (SIMPLESolver.cpp:23-51); the boundary values follow in k_bcs
+__global__ __launch_bounds__(kBlock) void k_init(RsgMemberDev* M) {
+    Ctx C = gctx(M);
+    init_uniform_fields(C);
+}
+
+// boundary values: bit 1 velocity, 2 pressure, 4 k, 8 omega (with beta1)
+__global__ __launch_bounds__(kBlock) void k_bcs(RsgMemberDev* M, int mask, double beta1) {
+    Ctx C = gctx(M);
+    if (mask & 1) vel_bc(C);
+    if (mask & 2) p_bc(C);
+    if (mask & 4) k_bc(C);
+    if (mask & 8) om_bc(C, beta1);
+}
+
+// step 1: SST closure (or the velocity gradients alone) and grad p
+__global__ __launch_bounds__(kBlock) void k_step1(RsgMemberDev* M, int turbUpdate, double nuTMin) {
+    Ctx C = gctx(M);
+    if (turbUpdate) {
+        BIG_CELLS(c) sst_closure_cell(C, c, nuTMin);
+    } else {
+        grad_u(C, 0, 3);
+    }
+    grad_scalar(C, M->p, M->pb, M->gp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_grad_u(RsgMemberDev* M, int c0, int c1) {
+    Ctx C = gctx(M);
+    grad_u(C, c0, c1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_asm_mom(RsgMemberDev* M, int comp) {
+    Ctx C = gctx(M);
+    assemble_momentum(C, comp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_asm_pc(RsgMemberDev* M) {
+    Ctx C = gctx(M);
+    assemble_pcorr(C);
+    BIG_CELLS(c) M->pp[c] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_asm_turb(RsgMemberDev* M, int isOmega) {
+    Ctx C = gctx(M);
+    assemble_turb(C, isOmega != 0);
+}
+
+// p' boundary values (zero at outlets)
+__global__ __launch_bounds__(kBlock) void k_ppb(RsgMemberDev* M) {
+    Ctx C = gctx(M);
+    const RsgMeshDev* m = C.m;
+    BIG_BFACES(b) M->ppb[b] = (m->bFlags[b] & RSG_BF_OUTLET) ? 0.0 : M->pp[m->own[C.nIF + b]];
+}
+
+// step 4: grad p' is current; velocity and pressure corrections
+__global__ __launch_bounds__(kBlock) void k_grad_pp(RsgMemberDev* M) {
+    Ctx C = gctx(M);
+    grad_scalar(C, M->pp, M->ppb, M->gp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_correct(RsgMemberDev* M) {
+    Ctx C = gctx(M);
+    BIG_CELLS(c) correct_cell(C, c);
+}
+
+// k / omega clamps after their solves (SIMPLESolver.cpp:861, :871)
+__global__ __launch_bounds__(kBlock) void k_clamp(RsgMemberDev* M, int isOmega) {
+    Ctx C = gctx(M);
+    const RsgSettingsDev* st = C.s;
+    if (isOmega) { BIG_CELLS(c) M->om[c] = dmax(st->omMin, dmin(1e15, M->om[c])); }
+    else { BIG_CELLS(c) M->k[c] = dmax(st->kMin, dmin(st->kMax, M->k[c])); }
+}
+
+// wall omega pin and the block maxima of the change norms' terms
+__global__ __launch_bounds__(kBlock) void k_pin(RsgMemberDev* M, double* part) {
+    __shared__ double sh[4][kBlock / 64];
+    Ctx C = gctx(M);
+    double mx[4] = {1e-30, 1e-30, 0.0, 0.0};
+    BIG_CELLS(c) pin_cell(C, c, mx);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double v = mx[q];
+        for (int off = 32; off >= 1; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+        if (lane == 0) sh[q][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = sh[threadIdx.x][0];
+        for (int q = 1; q < kBlock / 64; ++q) v = dmax(v, sh[threadIdx.x][q]);
+        part[4 * blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+// out[q] = max over the nb block partials of value q (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_max4(int nb, const double* part, double* out) {
+    __shared__ double sh[4][kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double v = part[q];
+        for (int b = threadIdx.x; b < nb; b += kBlock) v = dmax(v, part[4 * b + q]);
+        for (int off = 32; off >= 1; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+        if (lane == 0) sh[q][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = sh[threadIdx.x][0];
+        for (int q = 1; q < kBlock / 64; ++q) v = dmax(v, sh[threadIdx.x][q]);
+        out[threadIdx.x] = v;
+    }
+}
+
+// CSR values (level-0 pattern of the mesh, lduToCSR order) of the member's
+// current LDU system, and its diagonal
+__global__ __launch_bounds__(kBlock) void k_csr_fill(RsgMemberDev* M, double* v, double* dg) {
+    Ctx C = gctx(M);
+    const RsgMeshDev* m = C.m;
+    const int nnz = m->nnz0, nIF = C.nIF;
+    for (int e = C.t.tid; e < nnz; e += C.t.bs) {
+        const int s = m->src0[e];
+        v[e] = s < 0 ? M->diag[-1 - s] : (s < nIF ? M->upper[s] : M->lower[s - nIF]);
+    }
+    BIG_CELLS(c) dg[c] = M->diag[c];
+}
+
+// coarse Galerkin values of one level: entry E sums its fine entries (gather
+// list, ascending fine entry = the reference's summation order)
+__global__ __launch_bounds__(kBlock) void k_galerkin(int nnzc, const int* gP, const int* gI, const double* fv,
+                                                     double* cv) {
+    const int E = blockIdx.x * kBlock + threadIdx.x;
+    if (E >= nnzc) return;
+    double s = 0.0;
+    for (int q = gP[E]; q < gP[E + 1]; ++q) s += fv[gI[q]];
+    cv[E] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_of(int n, const int* rp, const int* ci, const double* v,
+                                                    double* dg) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+        if (ci[e] == i) d = v[e];
+    dg[i] = d;
+}
+
+// dense Gauss-Jordan inverse of the coarsest level (<= 64 rows), one workgroup
+__global__ __launch_bounds__(kBlock) void k_coarse_inverse(int n, const int* rp, const int* ci, const double* v,
+                                                           double* cinv, int* ok) {
+    __shared__ double G[RSG_COARSE_DENSE * RSG_COARSE_DENSE];
+    const int good = coarse_dense_block(G, rp, ci, v, n, threadIdx.x, blockDim.x);
+    for (int idx = threadIdx.x; idx < n * RSG_COARSE_DENSE; idx += blockDim.x) cinv[idx] = G[idx];
+    if (threadIdx.x == 0) *ok = good;
+}
+
+// BiCGSTAB phases (the recurrence of rsg_member.h's bicgstab)
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int n, double beta, double omega, const double* r,
+                                                   const double* v, const double* dg, double* p, double* ph) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double pc = r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pc;
+    ph[i] = jac<double>(pc, dg[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bicg_s(int n, double alpha, const double* r, const double* v,
+                                                   const double* dg, double* s, double* sh) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double sc = r[i] - alpha * v[i];
+    s[i] = sc;
+    sh[i] = jac<double>(sc, dg[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bicg_x(int n, double alpha, double omega, const double* ph,
+                                                   const double* sh, const double* s, const double* t, double* x,
+                                                   double* r) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    x[i] += alpha * ph[i] + omega * sh[i];
+    r[i] = s[i] - omega * t[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy(int n, double a, const double* p, double* x) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] += a * p[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(int n, const double* a, double* b) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_observe(RsgMemberDev* M) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) observe_member(M);
+}
+
+#undef BIG_CELLS
+#undef BIG_BFACES
+
+inline int cells_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, 1 << 16)); }
+
+}  // namespace big
+}  // namespace rsg
+
+using namespace rsg;
+using namespace rsg::big;
+
+// Host state of one large member.
+struct RsgBig {
+    RsgMemberDev hM{};            // host copy of the member struct (device pointers)
+    RsgMemberDev* dM = nullptr;   // the struct on the device
+    RsgSettingsDev set{};
+    const PrepMesh* pm = nullptr;
+    int nC = 0, nnz0 = 0;
+    const int* rp0 = nullptr;     // device level-0 pattern (the mesh's)
+    const int* ci0 = nullptr;
+    hipStream_t s = nullptr;
+    grid::DevAlloc A;             // work vectors, scratch
+    double *v0 = nullptr, *dg0 = nullptr, *vz = nullptr, *vq = nullptr;
+    grid::Scratch S;
+    // p' AMG: hierarchy and the device gather lists of its coarse levels
+    std::unique_ptr<grid::DevAlloc> HA;
+    grid::Hierarchy<double> H;
+    std::vector<int*> gP, gI;
+    int* dOk = nullptr;
+    bool haveH = false;
+    // outer-iteration state (the device member's Scal, kept on the host)
+    RsgScalars sc{};
+    std::vector<double> hist;
+    std::string err;
+    int initialized = 0;
+};
+
+namespace {
+
+hipError_t dot_read(RsgBig& B, int n, const double* a, const double* b, const double* c, const double* d,
+                    double* out2) {
+    grid::dot2<double>(n, a, b, c, d, B.S, B.S.dots, B.s);
+    hipError_t e = hipMemcpyAsync(B.S.hdots, B.S.dots, 2 * sizeof(double), hipMemcpyDeviceToHost, B.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(B.s);
+    out2[0] = B.S.hdots[0];
+    out2[1] = c ? B.S.hdots[1] : 0.0;
+    return e;
+}
+
+// BiCGSTAB (LinearSolver.hpp:193-245, Jacobi right preconditioning) on the
+// member's current system, CSR values in B.v0 / B.dg0; x in/out.
+hipError_t bicgstab_grid(RsgBig& B, double* x, int maxIter, double tol, SolveRes& res) {
+    const int n = B.nC;
+    RsgMemberDev& M = B.hM;
+    double *r = M.kv[0], *rh = M.kv[1], *p = M.kv[2], *v = M.kv[3], *sv = M.kv[4], *tv = M.kv[5], *ph = M.kv[6],
+           *sh = M.kv[7];
+    const dim3 g(grid::blocks(n)), bl(kBlock);
+    res = SolveRes{0, 0.0, 0.0, 0};
+    hipLaunchKernelGGL(grid::k_residual<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
+                       (const double*)M.src, (const double*)x, r);
+    double d2[2];
+    hipError_t e = dot_read(B, n, r, r, nullptr, nullptr, d2);
+    if (e != hipSuccess) return e;
+    const double r0 = std::sqrt(d2[0]);
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return hipSuccess; }
+    hipLaunchKernelGGL(k_copy, g, bl, 0, B.s, n, (const double*)r, rh);
+    hipLaunchKernelGGL(grid::k_fill<double>, g, bl, 0, B.s, n, p, 0.0);
+    hipLaunchKernelGGL(grid::k_fill<double>, g, bl, 0, B.s, n, v, 0.0);
+    double rho = 1.0, alpha = 1.0, omega = 1.0, rhoN = d2[0];
+    for (int it = 0; it < maxIter; ++it) {
+        if (std::fabs(rhoN) < 1e-30) break;
+        const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
+        hipLaunchKernelGGL(k_bicg_p, g, bl, 0, B.s, n, beta, omega, (const double*)r, (const double*)v,
+                           (const double*)B.dg0, p, ph);
+        hipLaunchKernelGGL(grid::k_spmv<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
+                           (const double*)ph, v);
+        if ((e = dot_read(B, n, rh, v, nullptr, nullptr, d2)) != hipSuccess) return e;
+        const double rv = d2[0];
+        if (std::fabs(rv) < 1e-30) break;
+        alpha = rhoN / rv;
+        hipLaunchKernelGGL(k_bicg_s, g, bl, 0, B.s, n, alpha, (const double*)r, (const double*)v,
+                           (const double*)B.dg0, sv, sh);
+        if ((e = dot_read(B, n, sv, sv, nullptr, nullptr, d2)) != hipSuccess) return e;
+        const double sn = std::sqrt(d2[0]);
+        if (sn / r0 < tol) {
+            hipLaunchKernelGGL(k_axpy, g, bl, 0, B.s, n, alpha, (const double*)ph, x);
+            res.iterations = it + 1; res.finalRes = sn / r0; res.converged = 1;
+            return hipGetLastError();
+        }
+        hipLaunchKernelGGL(grid::k_spmv<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
+                           (const double*)sh, tv);
+        if ((e = dot_read(B, n, tv, tv, tv, sv, d2)) != hipSuccess) return e;
+        omega = (d2[0] > 1e-30) ? d2[1] / d2[0] : 0.0;
+        hipLaunchKernelGGL(k_bicg_x, g, bl, 0, B.s, n, alpha, omega, (const double*)ph, (const double*)sh,
+                           (const double*)sv, (const double*)tv, x, r);
+        if ((e = dot_read(B, n, r, r, rh, r, d2)) != hipSuccess) return e;
+        const double rn = std::sqrt(d2[0]);
+        res.iterations = it + 1;
+        res.finalRes = rn / r0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+        if (std::fabs(omega) < 1e-30) break;
+        rho = rhoN;
+        rhoN = d2[1];
+    }
+    return hipGetLastError();
+}
+
+// Coarse levels and gather lists of the p' hierarchy, aggregated on the host
+// from the current level-0 matrix (values in B.v0 / B.dg0).
+hipError_t build_amg(RsgBig& B) {
+    const PrepMesh& pm = *B.pm;
+    std::vector<grid::Level> H(1);
+    grid::Level& L0 = H[0];
+    L0.n = pm.nC; L0.nnz = pm.nnz0;
+    L0.rp = pm.rp0; L0.ci = pm.ci0;
+    L0.v.resize(pm.nnz0); L0.dg.resize(pm.nC);
+    hipError_t e = hipMemcpyAsync(L0.v.data(), B.v0, 8 * (size_t)pm.nnz0, hipMemcpyDeviceToHost, B.s);
+    if (e == hipSuccess) e = hipMemcpyAsync(L0.dg.data(), B.dg0, 8 * (size_t)pm.nC, hipMemcpyDeviceToHost, B.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(B.s);
+    if (e != hipSuccess) return e;
+    // the reference's greedy aggregation while it coarsens; where it stalls on
+    // a level larger than the dense coarsest size (isotropic Galerkin
+    // stencils: every coupling is at most a quarter of the diagonal, so none
+    // is "strong"), two rounds of pairwise matching instead -- a stalled level
+    // would otherwise end the hierarchy unfactored and the V-cycle would lose
+    // its coarse correction
+    for (int l = 0; l + 1 < RSG_MAX_LEVELS; ++l) {
+        grid::Level& f = H.back();
+        if (f.n <= RSG_MIN_COARSE) break;
+        grid::Level c;
+        const bool ok = grid::coarsen(f, c);
+        if ((!ok || c.n > 0.75 * f.n) && f.n > RSG_COARSE_DENSE) {
+            const std::vector<int> refMap = f.map;
+            grid::Level c2;
+            if (grid::coarsen_pairwise(f, c2)) c = std::move(c2);
+            else if (ok) f.map = refMap;
+            else break;
+        } else if (!ok) {
+            break;
+        }
+        if (c.n <= 0 || c.n >= f.n || (int)f.map.size() != f.n) { B.err = "AMG coarsening failed"; return hipErrorUnknown; }
+        H.push_back(std::move(c));
+    }
+    // gather lists: coarse entry (I, J) <- fine entries of the aggregate rows
+    // of I whose column's aggregate is J, ascending fine entry
+    std::vector<std::vector<int>> gPh(H.size()), gIh(H.size());
+    for (size_t l = 1; l < H.size(); ++l) {
+        const grid::Level& f = H[l - 1];
+        const grid::Level& c = H[l];
+        std::vector<int> f2c(f.nnz, -1);
+        for (int I = 0; I < c.n; ++I)
+            for (int q = c.aP[I]; q < c.aP[I + 1]; ++q) {
+                const int i = c.aI[q];
+                for (int e2 = f.rp[i]; e2 < f.rp[i + 1]; ++e2) {
+                    const int J = f.map[f.ci[e2]];
+                    const int* lo = c.ci.data() + c.rp[I];
+                    const int* hi = c.ci.data() + c.rp[I + 1];
+                    const int* at = std::lower_bound(lo, hi, J);
+                    if (at == hi || *at != J) { B.err = "AMG gather list: coarse entry not found"; return hipErrorUnknown; }
+                    f2c[e2] = (int)(at - c.ci.data());
+                }
+            }
+        for (int e2 = 0; e2 < f.nnz; ++e2)
+            if (f2c[e2] < 0) { B.err = "AMG gather list: fine entry without a coarse entry"; return hipErrorUnknown; }
+        std::vector<int>& P = gPh[l];
+        std::vector<int>& I2 = gIh[l];
+        P.assign(c.nnz + 1, 0);
+        for (int e2 = 0; e2 < f.nnz; ++e2) P[f2c[e2] + 1]++;
+        for (int E = 0; E < c.nnz; ++E) P[E + 1] += P[E];
+        I2.assign(f.nnz, 0);
+        std::vector<int> fill(P.begin(), P.end() - 1);
+        for (int e2 = 0; e2 < f.nnz; ++e2) I2[fill[f2c[e2]]++] = e2;
+    }
+    std::vector<double> G;
+    const bool dense = H.back().n <= RSG_COARSE_DENSE && grid::dense_inverse(H.back(), G);
+    B.HA.reset(new grid::DevAlloc());
+    B.H = grid::Hierarchy<double>();
+    e = grid::upload<double>(H, G, dense, *B.HA, B.H, B.s);
+    if (e != hipSuccess) return e;
+    B.gP.assign(H.size(), nullptr); B.gI.assign(H.size(), nullptr);
+    for (size_t l = 1; l < H.size(); ++l) {
+        B.gP[l] = B.HA->take<int>(gPh[l].size());
+        B.gI[l] = B.HA->take<int>(gIh[l].size());
+        if (B.HA->err == hipSuccess)
+            B.HA->err = hipMemcpyAsync(B.gP[l], gPh[l].data(), 4 * gPh[l].size(), hipMemcpyHostToDevice, B.s);
+        if (B.HA->err == hipSuccess)
+            B.HA->err = hipMemcpyAsync(B.gI[l], gIh[l].data(), 4 * gIh[l].size(), hipMemcpyHostToDevice, B.s);
+    }
+    if (B.HA->err == hipSuccess) B.HA->err = hipStreamSynchronize(B.s);
+    B.haveH = B.HA->err == hipSuccess;
+    return B.HA->err;
+}
+
+// level-0 values of the hierarchy from the current p' matrix; with `coarse`
+// the Galerkin values of every coarser level and the coarsest inverse too
+hipError_t refresh_amg(RsgBig& B, bool coarse) {
+    grid::Hierarchy<double>& H = B.H;
+    hipLaunchKernelGGL(k_csr_fill, dim3(cells_grid(std::max(B.nC, B.nnz0))), dim3(kBlock), 0, B.s, B.dM,
+                       H.lv[0].v, H.lv[0].dg);
+    if (!coarse) return hipGetLastError();
+    const int L = (int)H.lv.size();
+    for (int l = 1; l < L; ++l) {
+        grid::DevLevel<double>& c = H.lv[l];
+        hipLaunchKernelGGL(k_galerkin, dim3(grid::blocks(c.nnz)), dim3(kBlock), 0, B.s, c.nnz, (const int*)B.gP[l],
+                           (const int*)B.gI[l], (const double*)H.lv[l - 1].v, c.v);
+        hipLaunchKernelGGL(k_diag_of, dim3(grid::blocks(c.n)), dim3(kBlock), 0, B.s, c.n, (const int*)c.rp,
+                           (const int*)c.ci, (const double*)c.v, c.dg);
+    }
+    if (H.dense) {
+        const grid::DevLevel<double>& c = H.lv[L - 1];
+        hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(kBlock), 0, B.s, c.n, (const int*)c.rp,
+                           (const int*)c.ci, (const double*)c.v, H.cinv, B.dOk);
+        int ok = 0;
+        hipError_t e = hipMemcpyAsync(&ok, B.dOk, sizeof ok, hipMemcpyDeviceToHost, B.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(B.s);
+        if (e != hipSuccess) return e;
+        if (!ok) H.dense = false;     // zero pivot: the coarsest level is smoothed instead
+    }
+    return hipGetLastError();
+}
+
+// One linear solve of the member's current system into x (device).
+hipError_t solve(RsgBig& B, int code, double* x, bool amgReady, SolveRes& res) {
+    const RsgSettingsDev& st = B.set;
+    const int n = B.nC;
+    RsgMemberDev& M = B.hM;
+    hipLaunchKernelGGL(k_csr_fill, dim3(cells_grid(std::max(n, B.nnz0))), dim3(kBlock), 0, B.s, B.dM, B.v0, B.dg0);
+    if (code == 2) return bicgstab_grid(B, x, st.innerIter, st.innerTol, res);
+    grid::Hierarchy<double> J;                    // Jacobi: level 0 alone
+    const bool amg = code == 0 && amgReady;
+    if (!amg) {
+        J.lv.resize(1);
+        grid::DevLevel<double>& d = J.lv[0];
+        d.n = n; d.nnz = B.nnz0;
+        d.rp = const_cast<int*>(B.rp0); d.ci = const_cast<int*>(B.ci0); d.v = B.v0; d.dg = B.dg0;
+    }
+    grid::PcgResult pr;
+    hipError_t e = grid::pcg<double>(amg ? B.H : J, M.src, x, M.kv[0], B.vz, M.kv[2], B.vq, st.innerIter,
+                                     st.innerTol, amg, B.S, B.s, pr);
+    res = SolveRes{pr.iterations, pr.finalRes, pr.initialRes, pr.converged};
+    return e;
+}
+
+// std::max's comparison, as the device dmax (NaN operands behave alike)
+inline double hmax(double a, double b) { return (a < b) ? b : a; }
+
+// One SIMPLE outer iteration (SIMPLESolver.cpp:731-1036), the phases of
+// rsg_member.h's simple_iteration as grid launches.
+hipError_t outer_iteration(RsgBig& B) {
+    RsgScalars& sc = B.sc;
+    const RsgSettingsDev& st = B.set;
+    RsgMemberDev& M = B.hM;
+    const int iter = sc.iter;
+    const int n = B.nC;
+    const dim3 gc(cells_grid(n)), bl(kBlock);
+    const bool turbActive = iter >= st.turbStart;
+    const bool turbUpdate = turbActive && ((iter - st.turbStart) % st.turbInterval == 0);
+    hipError_t e = hipSuccess;
+    // 1. SST closure + grad p
+    double nuTMin = 0.0;
+    if (turbUpdate) {
+        if (!sc.turbEstablished && iter >= st.turbStart + st.nutFloorIters) sc.turbEstablished = 1;
+        nuTMin = sc.turbEstablished ? 0.0 : 0.1 * M.nu;
+    }
+    hipLaunchKernelGGL(k_step1, gc, bl, 0, B.s, B.dM, turbUpdate ? 1 : 0, nuTMin);
+    // 2. momentum
+    SolveRes rU[3] = {{0, 0.0, 0.0, 0}, {0, 0.0, 0.0, 0}, {0, 0.0, 0.0, 0}};
+    double* ucomp[3] = {M.Ux, M.Uy, M.Uz};
+    for (int comp = 0; comp < 3 && e == hipSuccess; ++comp) {
+        if (comp == 2 && M.planarZ) break;
+        if (comp > 0) hipLaunchKernelGGL(k_grad_u, gc, bl, 0, B.s, B.dM, comp - 1, comp);
+        hipLaunchKernelGGL(k_asm_mom, gc, bl, 0, B.s, B.dM, comp);
+        e = solve(B, st.mSolver, ucomp[comp], false, rU[comp]);
+    }
+    hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, B.dM, 1, 0.0);
+    // 3. pressure correction
+    hipLaunchKernelGGL(k_asm_pc, gc, bl, 0, B.s, B.dM);
+    bool amgReady = false;
+    if (e == hipSuccess && st.pSolver == 0) {
+        const bool rebuild = sc.amgBuiltIter < 0 || iter - sc.amgBuiltIter >= st.amgRefresh || !B.haveH;
+        const bool coarse = rebuild || iter - sc.coarseIter >= st.coarseRefresh;
+        if (rebuild) {
+            hipLaunchKernelGGL(k_csr_fill, dim3(cells_grid(std::max(n, B.nnz0))), bl, 0, B.s, B.dM, B.v0, B.dg0);
+            e = build_amg(B);
+            sc.amgBuiltIter = iter;
+        }
+        if (e == hipSuccess) e = refresh_amg(B, coarse);
+        if (coarse) sc.coarseIter = iter;
+        sc.nLevels = (int)B.H.lv.size();
+        sc.coarseDense = B.H.dense ? 1 : 0;
+        amgReady = true;
+    }
+    SolveRes rP{0, 0.0, 0.0, 0};
+    if (e == hipSuccess) e = solve(B, st.pSolver, M.pp, amgReady, rP);
+    hipLaunchKernelGGL(k_ppb, gc, bl, 0, B.s, B.dM);
+    // 4. corrections
+    hipLaunchKernelGGL(k_grad_pp, gc, bl, 0, B.s, B.dM);
+    hipLaunchKernelGGL(k_correct, gc, bl, 0, B.s, B.dM);
+    hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, B.dM, 1 | 2, 0.0);
+    // 5. turbulence
+    SolveRes rK{0, 0.0, 0.0, 0}, rOm{0, 0.0, 0.0, 0};
+    if (turbUpdate && e == hipSuccess) {
+        hipLaunchKernelGGL(k_asm_turb, gc, bl, 0, B.s, B.dM, 0);
+        e = solve(B, st.tSolver, M.k, false, rK);
+        hipLaunchKernelGGL(k_clamp, gc, bl, 0, B.s, B.dM, 0);
+        hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, B.dM, 4, 0.0);
+        hipLaunchKernelGGL(k_asm_turb, gc, bl, 0, B.s, B.dM, 1);
+        if (e == hipSuccess) e = solve(B, st.tSolver, M.om, false, rOm);
+        hipLaunchKernelGGL(k_clamp, gc, bl, 0, B.s, B.dM, 1);
+        hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, B.dM, 8, M.coeffs[2]);
+        hipLaunchKernelGGL(k_pin, gc, bl, 0, B.s, B.dM, B.S.part);
+        hipLaunchKernelGGL(k_max4, dim3(1), dim3(kBlock), 0, B.s, (int)gc.x, (const double*)B.S.part, B.S.dots);
+        if (e == hipSuccess) e = hipMemcpyAsync(B.S.hdots, B.S.dots, 4 * sizeof(double), hipMemcpyDeviceToHost, B.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(B.s);
+        sc.lastK = B.S.hdots[2] / B.S.hdots[0];
+        sc.lastOm = B.S.hdots[3] / B.S.hdots[1];
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // 6. residual bookkeeping and the convergence / divergence tests (:934-1035)
+    const double eUx = rU[0].initialRes, eUy = rU[1].initialRes, eUz = rU[2].initialRes, eP = rP.initialRes;
+    const double eK = turbActive ? sc.lastK : 0.0, eOm = turbActive ? sc.lastOm : 0.0;
+    sc.innerIters[0] += rU[0].iterations; sc.innerIters[1] += rU[1].iterations; sc.innerIters[2] += rU[2].iterations;
+    sc.innerIters[3] += rP.iterations; sc.innerIters[4] += rK.iterations; sc.innerIters[5] += rOm.iterations;
+    if (st.recordHistory && iter < M.histCap) {
+        const double h[6] = {eUx, eUy, eUz, eP, eK, eOm};
+        B.hist.insert(B.hist.end(), h, h + 6);
+        sc.histLen = iter + 1;
+    }
+    if (iter == 0) {
+        sc.normUx0 = hmax(eUx, 1e-30); sc.normUy0 = hmax(eUy, 1e-30); sc.normP0 = hmax(eP, 1e-30);
+    } else if (iter < 50) {
+        sc.normP0 = hmax(sc.normP0, eP);
+    }
+    const double normMom0 = hmax(sc.normUx0, sc.normUy0);
+    const double nUx = normMom0 < 1e-20 ? 0.0 : eUx / normMom0;
+    const double nUy = normMom0 < 1e-20 ? 0.0 : eUy / normMom0;
+    const double nP = sc.normP0 < 1e-20 ? 0.0 : eP / sc.normP0;
+    double maxRes = hmax(hmax(nUx, nUy), nP);
+    if (turbActive) maxRes = hmax(hmax(maxRes, sc.lastK), sc.lastOm);
+    if (!std::isfinite(eUx) || !std::isfinite(eUy) || !std::isfinite(eUz) || !std::isfinite(eP) ||
+        !std::isfinite(eK) || !std::isfinite(eOm) || !std::isfinite(sc.lastK) || !std::isfinite(sc.lastOm))
+        maxRes = INFINITY;
+    const bool turbScheduled = st.turbStart < st.maxIter;
+    sc.iter = iter + 1;
+    if (maxRes < st.tol && iter > 0 && (!turbScheduled || (turbActive && sc.turbEstablished))) {
+        sc.status = RSGM_CONVERGED; sc.finalIter = iter;
+    } else if (std::isnan(maxRes) || std::isinf(maxRes) || maxRes > st.divLimit) {
+        sc.status = RSGM_DIVERGED; sc.finalIter = iter;
+    } else if (sc.iter >= st.maxIter) {
+        sc.status = RSGM_MAXITER; sc.finalIter = st.maxIter;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set, const PrepMesh& pm,
+                       const RsgScalars& sc0, hipStream_t s, std::string& err) {
+    std::unique_ptr<RsgBig> B(new RsgBig());
+    B->hM = hM; B->dM = dM; B->set = set; B->pm = &pm; B->nC = pm.nC; B->nnz0 = pm.nnz0; B->s = s;
+    B->sc = sc0;
+    RsgMeshDev mesh;
+    hipError_t e = hipMemcpyAsync(&mesh, hM.mesh, sizeof mesh, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { err = std::string("big member: ") + hipGetErrorString(e); return nullptr; }
+    B->rp0 = mesh.rp0; B->ci0 = mesh.ci0;
+    const int nb = std::max(grid::blocks(std::max(pm.nC, pm.nnz0)), cells_grid(pm.nC));
+    B->v0 = B->A.take<double>(pm.nnz0); B->dg0 = B->A.take<double>(pm.nC);
+    B->vz = B->A.take<double>(pm.nC); B->vq = B->A.take<double>(pm.nC);
+    B->S.part = B->A.take<double>(4 * (size_t)nb + 4);
+    B->S.dots = B->A.take<double>(8);
+    B->dOk = B->A.take<int>(1);
+    if (B->A.err == hipSuccess) B->A.err = hipHostMalloc((void**)&B->S.hdots, 8 * sizeof(double), hipHostMallocDefault);
+    if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(dM, &hM, sizeof hM, hipMemcpyHostToDevice, s);
+    if (B->A.err != hipSuccess) { err = std::string("big member: ") + hipGetErrorString(B->A.err); return nullptr; }
+    return B.release();
+}
+
+void rsg_big_destroy(RsgBig* B) {
+    if (!B) return;
+    if (B->S.hdots) (void)hipHostFree(B->S.hdots);
+    delete B;
+}
+
+int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
+    hipError_t e = hipSuccess;
+    if (!B->sc.initialized) {
+        const dim3 gc(cells_grid(B->nC)), bl(kBlock);
+        if (!B->hM.initFromFields) {
+            hipLaunchKernelGGL(k_init, gc, bl, 0, B->s, B->dM);
+            hipLaunchKernelGGL(k_bcs, gc, bl, 0, B->s, B->dM, 15, 0.075);
+        }
+        B->sc.initialized = 1;
+        B->sc.amgBuiltIter = -1;
+        e = hipGetLastError();
+    }
+    for (int k = 0; k < iters && B->sc.status == RSGM_RUNNING && e == hipSuccess; ++k) e = outer_iteration(*B);
+    if (e == hipSuccess && B->sc.status != RSGM_RUNNING) {
+        // the finished member's history and scalars on the device (collected
+        // like any member's), then its observables
+        if (!B->hist.empty())
+            e = hipMemcpyAsync(B->hM.hist, B->hist.data(), 8 * B->hist.size(), hipMemcpyHostToDevice, B->s);
+        if (e == hipSuccess) e = hipMemcpyAsync(B->hM.sc, &B->sc, sizeof B->sc, hipMemcpyHostToDevice, B->s);
+        if (e == hipSuccess && B->hM.obs) {
+            hipLaunchKernelGGL(k_observe, dim3(1), dim3(64), 0, B->s, B->dM);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(B->s);
+    }
+    out = B->sc;
+    if (e != hipSuccess) {
+        err = std::string("big member: ") + (B->err.empty() ? std::string(hipGetErrorString(e)) : B->err);
+        return RSG_ERR_DEVICE;
+    }
+    return RSG_OK;
+}

@@ -35,6 +35,7 @@
 #include "rsg_member.h"
 #include "rsg_grid.h"
 #include "rsg_geom.h"
+#include "rsg_big.h"
 
 // rsg_team.hip
 int rsg_team_capacity(int device);
@@ -170,6 +171,10 @@ struct Ticket {
     // device workspace
     size_t off = 0, bytes = 0;
     RsgMemberDev M{};
+    // kernel-per-phase members (mesh of at least gridMinCells cells)
+    bool grid = false;
+    RsgMemberDev* dM = nullptr;      // the member struct on the device
+    RsgBig* big = nullptr;
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -201,6 +206,9 @@ struct rsg_context {
     // cells run alone on teamBlocks workgroups (0: sized by the cell count),
     // teamIters outer iterations a launch
     int teamMinCells = 0, teamBlocks = 0, teamIters = 20, teamCap = 0;
+    // kernel-per-phase solves (rsg_set_grid): members on meshes of at least
+    // gridMinCells cells (0: never), gridIters outer iterations per engine round
+    int gridMinCells = 0, gridIters = 20;
     double* dGred = nullptr;
     RsgMemberDev* dTeamM = nullptr;
     RsgScalars* dTeamOut = nullptr;
@@ -260,11 +268,13 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     MeshEntry& ME = *ctx->meshes[P.meshId];
     const rsg::PrepMesh& pm = ME.prep;
     const int histCap = P.settings.record_history ? P.settings.max_iterations : 0;
+    T.grid = ctx->gridMinCells > 0 && pm.nC >= ctx->gridMinCells;
     rsg::Carver sz;
     {
         RsgMemberDev tmp;
-        rsg::carve_member(sz, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, tmp);
+        rsg::carve_member(sz, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, tmp, T.grid);
         if (P.nObs > 0) sz.take<double>((size_t)P.nObs + 1);
+        if (T.grid) sz.take<RsgMemberDev>(1);
     }
     size_t bytes = sz.off;
     size_t off;
@@ -273,8 +283,9 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     RsgMemberDev& M = T.M;
     memset(&M, 0, sizeof M);
     rsg::Carver cv; cv.base = ctx->arena.base + off;
-    rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M);
+    rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M, T.grid);
     if (P.nObs > 0) { M.obs = P.obsDev; M.pred = cv.take<double>((size_t)P.nObs + 1); }
+    T.dM = T.grid ? cv.take<RsgMemberDev>(1) : nullptr;
     M.mesh = ME.devStruct; M.bc = P.bcDev; M.set = P.setDev; M.nu = P.nu;
     for (int i = 0; i < 11; ++i) M.coeffs[i] = T.coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = T.init[i];
@@ -371,8 +382,13 @@ void engine_loop(rsg_context* ctx) {
             ctx->nAdmitted += 1;
         }
         if (ctx->active.empty()) continue;
-        const int na = (int)ctx->active.size();
-        ctx->maxActive = std::max(ctx->maxActive, (double)na);
+        ctx->maxActive = std::max(ctx->maxActive, (double)ctx->active.size());
+        // kernel-per-phase members run one after the other on the whole GPU
+        // (gridIters outer iterations each per round); the others share the
+        // member launches
+        std::vector<int64_t> gridIds, rest;
+        for (int64_t id : ctx->active) (ctx->tickets[id]->grid ? gridIds : rest).push_back(id);
+        const int na = (int)rest.size();
         if (na > ctx->dCap) {
             if (ctx->dMembers) hipFree(ctx->dMembers);
             if (ctx->dStatus) hipFree(ctx->dStatus);
@@ -393,11 +409,12 @@ void engine_loop(rsg_context* ctx) {
         const int resident = bs == 256 ? ctx->resident256 : ctx->resident512;
         const int nl = resident > 0 ? std::min(na, resident) : na;
         std::vector<int64_t> batch(nl);
-        if (ctx->schedule == RSG_SCHEDULE_LARGEST_FIRST && nl < na) {
+        if (nl == 0) {
+        } else if (ctx->schedule == RSG_SCHEDULE_LARGEST_FIRST && nl < na) {
             // longest-processing-time first: the members on the largest
             // meshes (the longest solves of an ensemble) hold the device,
             // the others fill the places they free (ties: admission order)
-            std::vector<int64_t> order = ctx->active;
+            std::vector<int64_t> order = rest;
             auto cells = [&](int64_t id) {
                 return ctx->meshes[ctx->problems[ctx->tickets[id]->problemId]->meshId]->prep.nC;
             };
@@ -406,7 +423,7 @@ void engine_loop(rsg_context* ctx) {
             std::copy(order.begin(), order.begin() + nl, batch.begin());
         } else {
             const size_t r0 = ctx->rot % (size_t)na;
-            for (int i = 0; i < nl; ++i) batch[i] = ctx->active[(r0 + i) % na];
+            for (int i = 0; i < nl; ++i) batch[i] = rest[(r0 + i) % na];
             ctx->rot = r0 + nl;
         }
         // members on meshes of at least teamMinCells cells run as teams, one
@@ -455,9 +472,39 @@ void engine_loop(rsg_context* ctx) {
             }
             ctx->hMembers[i] = T.M;
         }
+        std::vector<RsgMemberDev> gridM(gridIds.size());
+        std::vector<RsgSettingsDev> gridSet(gridIds.size());
+        std::vector<const rsg::PrepMesh*> gridMesh(gridIds.size());
+        for (size_t q = 0; q < gridIds.size(); ++q) {
+            Ticket& T = *ctx->tickets[gridIds[q]];
+            ProblemEntry& P = *ctx->problems[T.problemId];
+            gridM[q] = T.M;
+            gridSet[q] = rsg::settings_dev(&P.settings);
+            gridMesh[q] = &ctx->meshes[P.meshId]->prep;
+        }
         lk.unlock();
         hipError_t e = hipSuccess;
         float ms = 0.f;
+        std::vector<int64_t> gridDone;
+        std::string gridErr;
+        for (size_t q = 0; q < gridIds.size() && e == hipSuccess; ++q) {
+            Ticket& T = *ctx->tickets[gridIds[q]];
+            std::string berr;
+            if (!T.big) {
+                RsgScalars s0;
+                member_init_scalars(s0, T.warm ? T.warmTurb : 0);
+                T.big = rsg_big_create(gridM[q], T.dM, gridSet[q], *gridMesh[q], s0, ctx->stream, berr);
+                if (!T.big) { e = hipErrorUnknown; gridErr = berr; break; }
+            }
+            e = hipEventRecord(ctx->ev0, ctx->stream);
+            const int rc = rsg_big_run(T.big, ctx->gridIters, T.sc, berr);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (rc != RSG_OK) { e = hipErrorUnknown; gridErr = berr; break; }
+            float gm = 0.f;
+            if (e == hipSuccess && hipEventElapsedTime(&gm, ctx->ev0, ctx->ev1) == hipSuccess) ms += gm;
+            if (T.sc.status != RSGM_RUNNING) gridDone.push_back(gridIds[q]);
+        }
         if (!team.empty() && ctx->teamCap <= 0) e = hipErrorInvalidConfiguration;
         if (!team.empty() && e == hipSuccess && !ctx->dGred) {
             e = hipMalloc(&ctx->dGred, sizeof(double) * 2 * RSG_TEAM_MAXB * 8);
@@ -494,6 +541,15 @@ void engine_loop(rsg_context* ctx) {
         }
         // retire finished members
         std::vector<int64_t> done;
+        for (int64_t id : gridDone) {
+            if (e != hipSuccess) break;
+            Ticket& T = *ctx->tickets[id];
+            ProblemEntry& P = *ctx->problems[T.problemId];
+            e = collect(T, ctx->meshes[P.meshId]->prep, P.settings, P.nObs);
+            rsg_big_destroy(T.big);
+            T.big = nullptr;
+            done.push_back(id);
+        }
         for (size_t q = 0; q < team.size() && e == hipSuccess; ++q) {
             Ticket& T = *ctx->tickets[team[q]];
             T.sc = teamSt[q];
@@ -516,7 +572,7 @@ void engine_loop(rsg_context* ctx) {
         ctx->nLaunch += 1;
         ctx->kernelMs += ms;
         if (e != hipSuccess) {
-            ctx->lastError = std::string("engine: ") + hipGetErrorString(e);
+            ctx->lastError = gridErr.empty() ? std::string("engine: ") + hipGetErrorString(e) : gridErr;
             ctx->engineError = true;
             for (int64_t id : ctx->active) ctx->tickets[id]->state = RSG_TICKET_FAILED;
             ctx->active.clear();
@@ -590,6 +646,8 @@ int rsg_context_destroy(rsg_context* ctx) {
     if (ctx->engine.joinable()) ctx->engine.join();
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->tickets)
+        if (kv.second->big) { rsg_big_destroy(kv.second->big); kv.second->big = nullptr; }
     if (ctx->dMembers) hipFree(ctx->dMembers);
     if (ctx->dStatus) hipFree(ctx->dStatus);
     if (ctx->dGred) hipFree(ctx->dGred);
@@ -922,6 +980,15 @@ int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t it
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (min_cells > 0 && ctx->teamCap <= 0) return set_err(ctx, RSG_ERR_UNSUPPORTED, "team launches are not available");
     ctx->teamMinCells = min_cells; ctx->teamBlocks = blocks; ctx->teamIters = iters_per_launch;
+    return RSG_OK;
+}
+
+int rsg_set_grid(rsg_context* ctx, int32_t min_cells, int32_t iters_per_launch) {
+    if (!ctx) return RSG_ERR_ARG;
+    if (min_cells < 0) return set_err(ctx, RSG_ERR_ARG, "min_cells must be >= 0");
+    if (iters_per_launch <= 0) return set_err(ctx, RSG_ERR_ARG, "iters_per_launch must be > 0");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->gridMinCells = min_cells; ctx->gridIters = iters_per_launch;
     return RSG_OK;
 }
 

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_dot2(int n, const T* a, const T* b, 
 }
 
 // sums the block partials in block order into out[0] (and out[1] when two)
-__global__ __launch_bounds__(kBlock) void k_finish(int nb, const double* part, double* out, int two) {
+static __global__ __launch_bounds__(kBlock) void k_finish(int nb, const double* part, double* out, int two) {
     __shared__ double sh[2][kBlock / 64];
     double s0 = 0.0, s1 = 0.0;
     // each thread sums a contiguous stripe, stripes combined in thread order
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const double* sc, con
 }
 
 // rz = rzNew (one thread, after every block of k_pcg_dir has read rz)
-__global__ void k_pcg_shift(double* sc) {
+static __global__ void k_pcg_shift(double* sc) {
     if (threadIdx.x == 0 && blockIdx.x == 0) sc[0] = sc[3];
 }
 
@@ -236,11 +236,83 @@ inline void ldu_to_csr(int n, int nif, const int* own, const int* nbr, const dou
     }
 }
 
+// The Galerkin level `c` of `f` under the aggregate map f.map (nc aggregates).
+inline void galerkin(const Level& f, int nc, Level& c);
+
 // One coarsening step: aggregates of `f` and the Galerkin level `c`.
 inline bool coarsen(Level& f, Level& c) {
     f.map.assign(f.n, -1);
     const int nc = amg_aggregate(f.n, f.rp.data(), f.ci.data(), f.v.data(), f.dg.data(), f.map.data());
     if (nc == 0 || nc >= f.n) { f.map.clear(); return false; }
+    galerkin(f, nc, c);
+    return true;
+}
+
+// Pairwise matching (deterministic, row order): each unmatched row joins its
+// unmatched neighbour of largest |a_ij|; a row left without a partner then
+// joins the aggregate of its neighbour of largest |a_ij| (it stays alone only
+// without neighbours).  Aggregates are numbered in order of their first row.
+// Returns the count.
+inline int pair_aggregate(const Level& f, std::vector<int>& map) {
+    std::vector<int> tmp(f.n, -1);
+    std::vector<char> single(f.n, 0);
+    int nt = 0;
+    for (int i = 0; i < f.n; ++i) {
+        if (tmp[i] >= 0) continue;
+        int best = -1;
+        double bv = 0.0;
+        for (int e = f.rp[i]; e < f.rp[i + 1]; ++e) {
+            const int j = f.ci[e];
+            if (j == i || tmp[j] >= 0) continue;
+            const double a = std::fabs(f.v[e]);
+            if (a > bv) { bv = a; best = j; }
+        }
+        tmp[i] = nt;
+        if (best >= 0) tmp[best] = nt; else single[i] = 1;
+        ++nt;
+    }
+    for (int i = 0; i < f.n; ++i) {
+        if (!single[i]) continue;
+        int best = -1;
+        double bv = 0.0;
+        for (int e = f.rp[i]; e < f.rp[i + 1]; ++e) {
+            const int j = f.ci[e];
+            if (j == i) continue;
+            const double a = std::fabs(f.v[e]);
+            if (a > bv) { bv = a; best = j; }
+        }
+        if (best >= 0) tmp[i] = tmp[best];
+    }
+    std::vector<int> id(nt, -1);
+    map.assign(f.n, -1);
+    int nc = 0;
+    for (int i = 0; i < f.n; ++i) {
+        if (id[tmp[i]] < 0) id[tmp[i]] = nc++;
+        map[i] = id[tmp[i]];
+    }
+    return nc;
+}
+
+// Coarsening for levels where the reference's greedy aggregation stalls
+// (fewer than a quarter of the rows merged): two rounds of pairwise matching,
+// the second on the Galerkin image of the first, composed into one map
+// (aggregates of up to four rows).  Not the reference's aggregation: used by
+// the large-member path only, below the levels where the reference's pass
+// still coarsens.
+inline bool coarsen_pairwise(Level& f, Level& c) {
+    std::vector<int> m1, m2;
+    const int n1 = pair_aggregate(f, m1);
+    if (n1 >= f.n) return false;
+    Level mid;
+    f.map = m1;
+    galerkin(f, n1, mid);
+    const int n2 = pair_aggregate(mid, m2);
+    for (int i = 0; i < f.n; ++i) f.map[i] = m2[m1[i]];
+    galerkin(f, n2, c);
+    return true;
+}
+
+inline void galerkin(const Level& f, int nc, Level& c) {
     c.n = nc;
     c.aP.assign(nc + 1, 0);
     for (int i = 0; i < f.n; ++i) c.aP[f.map[i] + 1]++;
@@ -272,7 +344,6 @@ inline bool coarsen(Level& f, Level& c) {
         for (int a = c.rp[I]; a < c.rp[I + 1]; ++a) if (c.ci[a] == I) c.dg[I] = c.v[a];
     }
     c.nnz = c.rp[nc];
-    return true;
 }
 
 // AMGSolver::setup (LinearSolver.hpp:516-550): level 0 from the LDU arrays,

@@ -1430,7 +1430,7 @@ RSG_D void grad_scalar(Ctx& C, const double* phi_, const double* phib_, double**
 
 // --------------------------------------------------------------- init
 // initUniform SIMPLESolver.cpp:23-51 (+ applyAllBCs with beta1 = 0.075)
-RSG_D void init_uniform(Ctx& C) {
+RSG_D void init_uniform_fields(Ctx& C) {
     Thr& t = C.t; const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
     const int nC = C.nC, nB = C.nB;
     const double* I = M->init;
@@ -1448,9 +1448,92 @@ RSG_D void init_uniform(Ctx& C) {
         M->Ubx[b] = I[0]; M->Uby[b] = I[1]; M->Ubz[b] = I[2];
         M->pb[b] = I[3]; M->kb[b] = I[4]; M->omb[b] = I[5];
     }
+}
+RSG_D void init_uniform(Ctx& C) {
+    init_uniform_fields(C);
     RSG_SYNC();
     vel_bc(C); p_bc(C); k_bc(C); om_bc(C, 0.075);
     RSG_SYNC();
+}
+
+// Step 1 of the outer iteration for cell c: Green-Gauss gradients of U, k,
+// omega, |S| and the SST closure (computeFields, SSTModel.cpp:122-172), with
+// the startup eddy-viscosity floor nuTMin.
+RSG_D void sst_closure_cell(Ctx& C, int c, double nuTMin) {
+    const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const RsgSettingsDev* st = C.s;
+    const double nu = C.nu;
+    const double* const phi[5] = {M->Ux, M->Uy, M->Uz, M->k, M->om};
+    const double* const phib[5] = {M->Ubx, M->Uby, M->Ubz, M->kb, M->omb};
+    double g[5][3];
+    gg_cell<5>(m, c, phi, phib, g);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { M->g[q * 3][c] = g[q][0]; M->g[q * 3 + 1][c] = g[q][1]; M->g[q * 3 + 2][c] = g[q][2]; }
+    const double S11 = g[0][0], S22 = g[1][1], S33 = g[2][2];
+    const double S12 = 0.5 * (g[0][1] + g[1][0]);
+    const double S13 = 0.5 * (g[0][2] + g[2][0]);
+    const double S23 = 0.5 * (g[1][2] + g[2][1]);
+    const double Sm = sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33
+                                  + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+    M->Smag[c] = Sm;
+    const double kc = dmax(M->k[c], 0.0);
+    const double wc = dmax(M->om[c], 1e-20);
+    const double y = dmax(m->wd[c], 1e-20);
+    const double dotkw = g[3][0] * g[4][0] + g[3][1] * g[4][1] + g[3][2] * g[4][2];
+    const double CDkw = 2.0 * C.sw2 / wc * dotkw;
+    M->CDkw[c] = CDkw;
+    const double CDpos = dmax(CDkw, 1e-20);
+    const double F1 = (st->variant == 2) ? 1.0 : sst_F1(C, kc, wc, y, nu, CDpos);
+    const double F2 = sst_F2(C, kc, wc, y, nu);
+    M->F1[c] = F1; M->F2[c] = F2;
+    const double nuT = sst_nut(C, kc, wc, Sm, F2);
+    M->Pk[c] = sst_prod(C, nuT, Sm, kc, wc);
+    M->nuT[c] = dmax(nuT, nuTMin);
+}
+
+// Step 4 (correctVelocity / correctPressure, SIMPLESolver.cpp:456-475) for cell c.
+RSG_D void correct_cell(Ctx& C, int c) {
+    const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M;
+    const double rAP = (fabs(M->aPx[c]) > 1e-30) ? 1.0 / M->aPx[c] : 0.0;
+    const double vol = m->vol[c];
+    M->Ux[c] -= rAP * M->gp[0][c] * vol;
+    M->Uy[c] -= rAP * M->gp[1][c] * vol;
+    M->Uz[c] -= rAP * M->gp[2][c] * vol;
+    M->p[c] += C.s->aP * M->pp[c];
+}
+
+// Step 5's wall-cell omega pin (SIMPLESolver.cpp:884-908) for cell c and its
+// terms of the field-change norms (:910-922): mx = max |k|, max |omega|,
+// max |k - kOld|, max |omega - omegaOld| (running).
+RSG_D void pin_cell(Ctx& C, int c, double (&mx)[4]) {
+    const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const RsgSettingsDev* st = C.s;
+    const double nu = C.nu;
+    const double y1 = m->pinY[c];
+    if (y1 > 0.0) {
+        const double omRes = 60.0 * nu / (C.b1 * y1 * y1);
+        if (st->wallFn) {
+            const double sbs = sqrt(C.bs);
+            const double k_p = dmax(M->k[c], 1e-30);
+            const double uTau = sqrt(sqrt(C.bs) * k_p);
+            const double omLog = uTau / (sbs * st->vonK * y1);
+            M->om[c] = sqrt(omRes * omRes + omLog * omLog);
+        } else {
+            M->om[c] = omRes;
+        }
+    }
+    mx[0] = dmax(mx[0], fabs(M->k[c]));
+    mx[1] = dmax(mx[1], fabs(M->om[c]));
+    mx[2] = dmax(mx[2], fabs(M->k[c] - M->kOld[c]));
+    mx[3] = dmax(mx[3], fabs(M->om[c] - M->omOld[c]));
+}
+
+// The per-launch context of member M for thread t: mesh, BCs, settings, nu
+// and the 11 SST coefficients (InferenceParameters.hpp:12-14 order).
+RSG_D void ctx_init(Ctx& C, Thr t, RsgMemberDev* M) {
+    C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
+    C.nC = C.m->nC; C.nIF = C.m->nIF; C.nB = C.m->nB; C.nu = M->nu;
+    C.sk1 = M->coeffs[0]; C.sw1 = M->coeffs[1]; C.b1 = M->coeffs[2]; C.al1 = M->coeffs[3];
+    C.sk2 = M->coeffs[4]; C.sw2 = M->coeffs[5]; C.b2 = M->coeffs[6]; C.al2 = M->coeffs[7];
+    C.bs = M->coeffs[8]; C.a1 = M->coeffs[9]; C.kap = M->coeffs[10];
 }
 
 // ------------------------------------------------------- outer iteration
@@ -1472,34 +1555,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
     if (turbUpdate) {
         if (!S.turbEstablished && iter >= st->turbStart + st->nutFloorIters) S.turbEstablished = 1;
         const double nuTMin = S.turbEstablished ? 0.0 : 0.1 * nu;
-        const double* const phi[5] = {M->Ux, M->Uy, M->Uz, M->k, M->om};
-        const double* const phib[5] = {M->Ubx, M->Uby, M->Ubz, M->kb, M->omb};
-        FOR_CELLS(c) {
-            double g[5][3];
-            gg_cell<5>(m, c, phi, phib, g);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) { M->g[q * 3][c] = g[q][0]; M->g[q * 3 + 1][c] = g[q][1]; M->g[q * 3 + 2][c] = g[q][2]; }
-            const double S11 = g[0][0], S22 = g[1][1], S33 = g[2][2];
-            const double S12 = 0.5 * (g[0][1] + g[1][0]);
-            const double S13 = 0.5 * (g[0][2] + g[2][0]);
-            const double S23 = 0.5 * (g[1][2] + g[2][1]);
-            const double Sm = sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33
-                                          + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
-            M->Smag[c] = Sm;
-            const double kc = dmax(M->k[c], 0.0);
-            const double wc = dmax(M->om[c], 1e-20);
-            const double y = dmax(m->wd[c], 1e-20);
-            const double dotkw = g[3][0] * g[4][0] + g[3][1] * g[4][1] + g[3][2] * g[4][2];
-            const double CDkw = 2.0 * C.sw2 / wc * dotkw;
-            M->CDkw[c] = CDkw;
-            const double CDpos = dmax(CDkw, 1e-20);
-            const double F1 = (st->variant == 2) ? 1.0 : sst_F1(C, kc, wc, y, nu, CDpos);
-            const double F2 = sst_F2(C, kc, wc, y, nu);
-            M->F1[c] = F1; M->F2[c] = F2;
-            const double nuT = sst_nut(C, kc, wc, Sm, F2);
-            M->Pk[c] = sst_prod(C, nuT, Sm, kc, wc);
-            M->nuT[c] = dmax(nuT, nuTMin);
-        }
+        FOR_CELLS(c) sst_closure_cell(C, c, nuTMin);
     } else {
         grad_u(C, 0, 3);
     }
@@ -1563,14 +1619,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
 
     // 4. correctVelocity / correctPressure
     grad_scalar(C, M->pp, M->ppb, M->gp);
-    FOR_CELLS(c) {
-        const double rAP = (fabs(M->aPx[c]) > 1e-30) ? 1.0 / M->aPx[c] : 0.0;
-        const double vol = m->vol[c];
-        M->Ux[c] -= rAP * M->gp[0][c] * vol;
-        M->Uy[c] -= rAP * M->gp[1][c] * vol;
-        M->Uz[c] -= rAP * M->gp[2][c] * vol;
-        M->p[c] += st->aP * M->pp[c];
-    }
+    FOR_CELLS(c) correct_cell(C, c);
     RSG_SYNC();
     vel_bc(C); p_bc(C);
     RSG_SYNC();
@@ -1606,25 +1655,7 @@ RSG_D void simple_iteration(Ctx& C, Scal& S) {
         RSG_SYNC();
         // wall-cell omega pin (:884-908) and field-change norms (:910-922)
         double mx[4] = {1e-30, 1e-30, 0.0, 0.0};
-        const double sbs = sqrt(C.bs);
-        FOR_CELLS(c) {
-            const double y1 = m->pinY[c];
-            if (y1 > 0.0) {
-                const double omRes = 60.0 * nu / (C.b1 * y1 * y1);
-                if (st->wallFn) {
-                    const double k_p = dmax(M->k[c], 1e-30);
-                    const double uTau = sqrt(sqrt(C.bs) * k_p);
-                    const double omLog = uTau / (sbs * st->vonK * y1);
-                    M->om[c] = sqrt(omRes * omRes + omLog * omLog);
-                } else {
-                    M->om[c] = omRes;
-                }
-            }
-            mx[0] = dmax(mx[0], fabs(M->k[c]));
-            mx[1] = dmax(mx[1], fabs(M->om[c]));
-            mx[2] = dmax(mx[2], fabs(M->k[c] - M->kOld[c]));
-            mx[3] = dmax(mx[3], fabs(M->om[c] - M->omOld[c]));
-        }
+        FOR_CELLS(c) pin_cell(C, c, mx);
         bmax(t, mx);
         S.lastK = mx[2] / mx[0];
         S.lastOm = mx[3] / mx[1];
@@ -1767,11 +1798,7 @@ RSG_D void observe_member(const RsgMemberDev* M) {
 template <int CMAX, int PREC = -1>
 RSG_D void run_member(Thr t, RsgMemberDev* M) {
     Ctx C;
-    C.t = t; C.m = M->mesh; C.bc = M->bc; C.s = M->set; C.M = M;
-    C.nC = C.m->nC; C.nIF = C.m->nIF; C.nB = C.m->nB; C.nu = M->nu;
-    C.sk1 = M->coeffs[0]; C.sw1 = M->coeffs[1]; C.b1 = M->coeffs[2]; C.al1 = M->coeffs[3];
-    C.sk2 = M->coeffs[4]; C.sw2 = M->coeffs[5]; C.b2 = M->coeffs[6]; C.al2 = M->coeffs[7];
-    C.bs = M->coeffs[8]; C.a1 = M->coeffs[9]; C.kap = M->coeffs[10];
+    ctx_init(C, t, M);
     RsgScalars* sc = M->sc;
     if (sc->status != RSGM_RUNNING) return;
     if (!sc->initialized) {

@@ -293,7 +293,10 @@ inline RsgBcDev bc_view(const BcOffsets& o, char* base) {
 }
 
 // Member workspace layout.  With cv.base == nullptr only measures.
-inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int histCap, RsgMemberDev& M) {
+// lean: a member solved by the kernel-per-phase path (rsg_big.hip), which
+// keeps its AMG hierarchy outside the workspace: no device-AMG arrays.
+inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int histCap, RsgMemberDev& M,
+                         bool lean = false) {
     double** cellArrays[] = {&M.Ux, &M.Uy, &M.Uz, &M.p, &M.k, &M.om, &M.nuT, &M.F1, &M.F2, &M.Pk,
                              &M.CDkw, &M.Smag, &M.aPx, &M.aPc, &M.pp, &M.kOld, &M.omOld,
                              &M.diag, &M.src};
@@ -306,6 +309,14 @@ inline void carve_member(Carver& cv, int nC, int nIF, int nB, int nnz0, int hist
     M.upper = cv.take<double>(nIF > 0 ? nIF : 1);
     M.lower = cv.take<double>(nIF > 0 ? nIF : 1);
     RsgAmgDev& A = M.amg;
+    if (lean) {
+        A = RsgAmgDev{};
+        M.histCap = histCap;
+        M.hist = histCap > 0 ? cv.take<double>(6 * (size_t)histCap) : nullptr;
+        M.sc = cv.take<RsgScalars>(1);
+        cv.off = (cv.off + 255) & ~size_t(255);
+        return;
+    }
     A.rowCap = 3 * nC + 64;
     A.nnzCap = 3 * nnz0 + 64;
     const int LV = RSG_MAX_LEVELS + 1;

@@ -381,6 +381,11 @@ PYBIND11_MODULE(rans_sst_py, m) {
           "Members on meshes of at least min_cells cells are solved by a team of `blocks` workgroups "
           "(0: 16 sqrt(cells / 40000), at least 16) in cooperative launches of iters_per_launch outer iterations; min_cells 0 turns "
           "team solves off.");
+    m.def("engine_set_grid", [](int minCells, int iters) { Engine::get().setGrid(minCells, iters); },
+          py::arg("min_cells"), py::arg("iters_per_launch") = 20,
+          "Members on meshes of at least min_cells cells (checked at admission) run alone on the whole GPU, "
+          "every phase of the outer iteration a grid-wide launch, iters_per_launch outer iterations per engine "
+          "round; min_cells 0 turns it off.  Takes precedence over team solves.");
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
     m.def("engine_set_schedule", [](const std::string& p) {
               if (p == "round_robin") Engine::get().setSchedule(RSG_SCHEDULE_ROUND_ROBIN);

@@ -67,6 +67,12 @@ void Engine::setTimeSlice(int microseconds) {
     if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
 }
 
+void Engine::setGrid(int minCells, int itersPerLaunch) {
+    rsg_context* c = ctx();
+    int rc = rsg_set_grid(c, minCells, itersPerLaunch);
+    if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
+}
+
 void Engine::setTeam(int minCells, int blocks, int itersPerLaunch) {
     rsg_context* c = ctx();
     int rc = rsg_set_team(c, minCells, blocks, itersPerLaunch);

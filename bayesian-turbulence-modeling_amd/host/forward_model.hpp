@@ -48,6 +48,9 @@ public:
     // members on meshes of at least minCells cells run as teams of `blocks`
     // workgroups (0: sized by the cell count); minCells 0 turns team solves off
     void setTeam(int minCells, int blocks, int itersPerLaunch);
+    // members on meshes of at least minCells cells run kernel-per-phase on the
+    // whole GPU (0 turns it off)
+    void setGrid(int minCells, int itersPerLaunch);
     std::vector<double> stats();
 private:
     std::mutex mu_;

@@ -244,6 +244,18 @@ int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds);
  * reduction-order round-off; time slices do not apply to team solves. */
 int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t iters_per_launch);
 
+/* Kernel-per-phase solves: a member whose mesh has at least min_cells cells
+ * (0: never; checked when the member is admitted) runs alone on the whole GPU,
+ * every phase of its outer iteration a grid-wide launch (assembly, gradients
+ * and closure one thread per cell; BiCGSTAB / PCG / AMG-PCG on the CSR image
+ * of each system with block-partial dot products; the p' AMG hierarchy
+ * aggregated on the host every amg_refresh outer iterations, its coarse
+ * Galerkin values refreshed on the device every coarse_refresh), at most
+ * iters_per_launch outer iterations per engine round.  For meshes of 10^5
+ * cells and more; it takes precedence over team solves.  Its inner solves run
+ * in FP64 whatever the member's precision. */
+int rsg_set_grid(rsg_context* ctx, int32_t min_cells, int32_t iters_per_launch);
+
 #define RSG_SCHEDULE_ROUND_ROBIN 0
 #define RSG_SCHEDULE_LARGEST_FIRST 1
 int rsg_set_schedule(rsg_context* ctx, int32_t policy);
