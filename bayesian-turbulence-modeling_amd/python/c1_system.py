@@ -3,10 +3,19 @@ mixed triangle/quad mesh) for the stand-alone linear-solve path.
 
 Domain: step height h = 1 with expansion ratio 1.2 (upstream channel
 y in [1, 6], downstream y in [0, 6]), 10 h upstream and 30 h downstream of the
-step.  Mesh: wall-clustered quad layers along every wall; the core quads are
-jittered (seeded) and split into two triangles along alternating diagonals,
-so the cell-face graph is unstructured in the core.  Faces are numbered
-internal first; the owner is the lower cell index.
+step.  Mesh: wall-clustered quad layers along the top and bottom walls of both
+blocks (a wall-graded tensor grid); the core of each block is a seeded
+Delaunay triangulation (core="delaunay", the default).  Its points are the
+core's boundary nodes (shared with the quad layers and, at x = 0, with the
+other block) plus one seeded point per interior grid node, displaced uniformly
+within +-0.3 of a grid cell.  The triangulation (scipy.spatial.Delaunay, i.e.
+Qhull) is computed in the grid's index coordinates, where the point spacing is
+isotropic, and mapped node by node through the monotone wall grading to
+physical space; mirrored ghost points outside the core make every boundary
+segment a Gabriel edge, so the triangles conform to the quad layers.
+core="split" gives the older variant: jittered core quads split into two
+triangles along alternating diagonals.  Faces are numbered internal first; the
+owner is the lower cell index.
 
 System (the matrix class of assemblePressureCorrection,
 incompressible/src/SIMPLESolver.cpp:322-450): per-cell coefficient nu_t
@@ -36,8 +45,10 @@ def _clustered(n, length, ratio_lo, ratio_hi=None):
     return x / x[-1] * length
 
 
-def _cells_and_nodes(nx_up, nx_dn, ny_up, ny_low, jitter, seed):
-    """Structured node grid of the two blocks (conformal at x = 0) and their quads."""
+def _cells_and_nodes(nx_up, nx_dn, ny_up, ny_low, jitter, seed, keep_rows=None):
+    """Structured node grid of the two blocks (conformal at x = 0) and their
+    quads.  Interior nodes are jittered (seeded); keep_rows(block, ny) names
+    node rows left straight (the Delaunay core's boundary rows)."""
     rng = np.random.default_rng(seed)
     # streamwise nodes: clustered towards the step on both sides
     xu = -_clustered(nx_up, 10.0, 1.01)[::-1]                     # [-10, 0], fine at x = 0
@@ -62,41 +73,94 @@ def _cells_and_nodes(nx_up, nx_dn, ny_up, ny_low, jitter, seed):
     remap = np.arange(len(xy))
     remap[shared_dn] = up[-1, :]
     # interior jitter (not on walls / inlet / outlet / interface)
-    for ids, xs, ys in ((up, xu, yu), (dn, xd, yd)):
+    for b, (ids, xs, ys) in enumerate(((up, xu, yu), (dn, xd, yd))):
         inner = ids[1:-1, 1:-1]
         hx = np.diff(xs)[:-1][:, None] * np.ones((1, len(ys) - 2))
         hy = np.ones((len(xs) - 2, 1)) * np.diff(ys)[:-1][None, :]
         dx = rng.uniform(-jitter, jitter, inner.shape) * np.minimum(hx, np.diff(xs)[1:][:, None])
         dy = rng.uniform(-jitter, jitter, inner.shape) * np.minimum(hy, np.diff(ys)[1:][None, :])
+        if keep_rows is not None:
+            fixed = np.zeros(len(ys), bool)
+            fixed[list(keep_rows(b, len(ys) - 1))] = True
+            dx[:, fixed[1:-1]] = 0.0
+            dy[:, fixed[1:-1]] = 0.0
         xy[inner.ravel(), 0] += dx.ravel()
         xy[inner.ravel(), 1] += dy.ravel()
     quads = []
     for ids in (up, dn):
         q = np.stack([ids[:-1, :-1], ids[1:, :-1], ids[1:, 1:], ids[:-1, 1:]], axis=-1)   # ccw
         quads.append(q)
-    return xy, remap, quads
+    return xy, remap, quads, [(up, xu, yu), (dn, xd, yd)]
 
 
-def make_system(nx_up=150, nx_dn=450, ny_up=200, ny_low=40, wall_layers=12, jitter=0.15, seed=0):
+def _delaunay_core(ids, xs, ys, wl, jit, rng):
+    """Seeded Delaunay triangulation of one block's core (node rows wl ..
+    ny - wl, all columns).  Returns the new interior points (physical
+    coordinates) and the triangles as rows of (kind, index): kind 0 a
+    structured node id of the block's boundary ring, kind 1 a new point."""
+    from scipy.spatial import Delaunay
+    nx, ny = len(xs) - 1, len(ys) - 1
+    j0, j1 = wl, ny - wl
+    # boundary ring of the core box in index space (integer lattice points)
+    ring = [(i, j0) for i in range(nx + 1)] + [(i, j1) for i in range(nx + 1)] + \
+           [(0, j) for j in range(j0 + 1, j1)] + [(nx, j) for j in range(j0 + 1, j1)]
+    ring = np.array(ring, dtype=float)
+    # one seeded point per interior grid node, within +-jit of it
+    ii, jj = np.meshgrid(np.arange(1, nx), np.arange(j0 + 1, j1), indexing="ij")
+    pts = np.stack([ii.ravel() + rng.uniform(-jit, jit, ii.size),
+                    jj.ravel() + rng.uniform(-jit, jit, ii.size)], axis=1)
+    # ghosts: the first interior row / column mirrored across each side, and the corners
+    g = [pts[np.abs(pts[:, 1] - (j0 + 1)) <= jit] * [1, -1] + [0, 2 * j0],
+         pts[np.abs(pts[:, 1] - (j1 - 1)) <= jit] * [1, -1] + [0, 2 * j1],
+         pts[np.abs(pts[:, 0] - 1) <= jit] * [-1, 1],
+         pts[np.abs(pts[:, 0] - (nx - 1)) <= jit] * [-1, 1] + [2 * nx, 0],
+         np.array([[-1.0, j0 - 1], [nx + 1, j0 - 1], [-1.0, j1 + 1], [nx + 1, j1 + 1]])]
+    allp = np.concatenate([ring, pts] + g)
+    nreal = len(ring) + len(pts)
+    simp = Delaunay(allp).simplices
+    simp = simp[(simp < nreal).all(axis=1)]
+    # physical coordinates through the monotone wall grading
+    X = lambda a: np.interp(a, np.arange(nx + 1), xs)
+    Y = lambda a: np.interp(a, np.arange(ny + 1), ys)
+    phys = np.stack([X(pts[:, 0]), Y(pts[:, 1])], axis=1)
+    ring_ids = ids[ring[:, 0].astype(int), ring[:, 1].astype(int)]
+    # orientation: counter-clockwise in index space, and the same in physical space
+    P = allp[simp]
+    cross = (P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1]) - (P[:, 2, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1])
+    simp[cross < 0] = simp[cross < 0][:, [0, 2, 1]]
+    return phys, simp, ring_ids, len(ring)
+
+
+def make_system(nx_up=150, nx_dn=450, ny_up=200, ny_low=40, wall_layers=12, jitter=0.15, seed=0,
+                core="delaunay"):
     """The C1 system; the defaults give ~250k cells."""
-    xy, remap, quads = _cells_and_nodes(nx_up, nx_dn, ny_up, ny_low, jitter, seed)
+    if core not in ("delaunay", "split"):
+        raise ValueError("core must be 'delaunay' or 'split'")
+    keep = (lambda b, ny: (wall_layers, ny - wall_layers)) if core == "delaunay" else None
+    xy, remap, quads, blocks = _cells_and_nodes(nx_up, nx_dn, ny_up, ny_low, jitter, seed, keep)
     polys = []                                   # list of node-index arrays (ccw)
-    for q in quads:
+    drng = np.random.default_rng(seed + 2)
+    for q, (ids, xs, ys) in zip(quads, blocks):
         nxq, nyq = q.shape[:2]
         ii, jj = np.meshgrid(np.arange(nxq), np.arange(nyq), indexing="ij")
         near_wall = (jj < wall_layers) | (jj >= nyq - wall_layers)
         qq = remap[q]
-        for split, flip in ((False, None), (True, 0), (True, 1)):
-            if not split:
-                sel = near_wall
-                polys.append(("quad", qq[sel]))
-            else:
-                sel = (~near_wall) & (((ii + jj) % 2) == flip)
-                c = qq[sel]
-                if flip == 0:      # diagonal 0-2
-                    polys.append(("tri", np.concatenate([c[:, [0, 1, 2]], c[:, [0, 2, 3]]])))
-                else:              # diagonal 1-3
-                    polys.append(("tri", np.concatenate([c[:, [0, 1, 3]], c[:, [1, 2, 3]]])))
+        polys.append(("quad", qq[near_wall]))
+        if core == "delaunay":
+            phys, simp, ring_ids, nring = _delaunay_core(ids, xs, ys, wall_layers, 0.3, drng)
+            new_ids = len(xy) + np.arange(len(phys))
+            xy = np.concatenate([xy, phys])
+            remap = np.concatenate([remap, new_ids])
+            node_of = np.concatenate([remap[ring_ids], new_ids])
+            polys.append(("tri", node_of[simp]))
+            continue
+        for flip in (0, 1):
+            sel = (~near_wall) & (((ii + jj) % 2) == flip)
+            c = qq[sel]
+            if flip == 0:      # diagonal 0-2
+                polys.append(("tri", np.concatenate([c[:, [0, 1, 2]], c[:, [0, 2, 3]]])))
+            else:              # diagonal 1-3
+                polys.append(("tri", np.concatenate([c[:, [0, 1, 3]], c[:, [1, 2, 3]]])))
     tri = np.concatenate([p for k, p in polys if k == "tri"])
     quad = np.concatenate([p for k, p in polys if k == "quad"])
     n_tri, n_quad = len(tri), len(quad)

@@ -68,10 +68,10 @@ def test_hierarchy_matches_oracle(host_lib, oracle, size):
     np.testing.assert_array_equal(mp, mpo)
     # coarsening stops at <= 50 rows, after 20 coarsenings, or where the
     # greedy aggregation stalls (the full system's weakly coupled rows leave a
-    # coarsest level of ~6k rows, in the reference as here)
+    # coarsest level of ~8k rows, in the reference as here)
     assert np.all(np.diff(sz) < 0) and nl <= 21
     if size == "full":
-        assert nl == 17 and sz[-1] == 6147
+        assert nl == 17 and sz[-1] == 7779
 
 
 def test_hierarchy_matches_reference(host_lib, ref):
