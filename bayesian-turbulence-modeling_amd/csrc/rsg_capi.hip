@@ -1154,6 +1154,10 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
     build_hierarchy(n, nif, own, nbr, diag, upper, lower, solver == RSG_SOLVER_AMG ? RSG_MAX_LEVELS : 0, H);
     std::vector<double> G;
     const bool dense = solver == RSG_SOLVER_AMG && H.back().n <= 64 && dense_inverse(H.back(), G);
+    // a stalled coarsest level (above 64 rows): MFMA inverse on the device
+    const bool bigDense = solver == RSG_SOLVER_AMG && !dense && H.size() > 1 && H.back().n <= RSG_DENSE_MAX;
+    int denseRows = 0;
+    double denseMs = 0.0;
     const auto t1 = std::chrono::steady_clock::now();
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "hipSetDevice");
     hipStream_t s;
@@ -1175,6 +1179,15 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
             std::vector<double> none;
             upload<double>(top, none, false, A, H64, s);
         }
+        const double* dinv = nullptr;
+        int dinvN = 0;
+        if (bigDense && A.err == hipSuccess) {
+            double* inv = nullptr;
+            bool ok = false;
+            A.err = dense_inverse_device(H.back(), A, s, &inv, &dinvN, &ok, &denseMs);
+            if (ok) { dinv = inv; denseRows = H.back().n; }
+        }
+        if (precision == RSG_PREC_FP64) { H64.dinv = dinv; H64.dinvN = dinvN; }
         double* b64 = A.take<double>(n);
         double* x64 = A.take<double>(n);
         if (A.err == hipSuccess) A.err = hipMemcpyAsync(b64, b, 8 * (size_t)n, hipMemcpyHostToDevice, s);
@@ -1190,6 +1203,7 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
             // margin), but never asks FP32 for more than 1e-5
             Hierarchy<float> H32;
             upload<float>(H, G, dense, A, H32, s);
+            H32.dinv = dinv; H32.dinvN = dinvN;
             double* r64 = A.take<double>(n);
             float* b32 = A.take<float>(n); float* e32 = A.take<float>(n);
             float* r = A.take<float>(n); float* z = A.take<float>(n);
@@ -1234,6 +1248,38 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
     stats[4] = (double)H.size(); stats[5] = refinements;
     stats[6] = std::chrono::duration<double>(t1 - t0).count();
     stats[7] = std::chrono::duration<double>(t2 - t1).count();
+    stats[8] = denseRows; stats[9] = 1e-3 * denseMs;
+    return RSG_OK;
+}
+
+int rsg_dense_inverse(rsg_context* ctx, int32_t n, const double* a, double* inv, double* kernel_seconds) {
+    using namespace rsg::grid;
+    if (!ctx || n <= 0 || n > RSG_DENSE_MAX || !a || !inv) return RSG_ERR_ARG;
+    // the matrix as a CSR level (every entry, zeros included, keeps the pattern dense)
+    Level L;
+    L.n = n; L.nnz = n * n;
+    L.rp.resize(n + 1); L.ci.resize((size_t)n * n); L.v.assign(a, a + (size_t)n * n);
+    for (int i = 0; i <= n; ++i) L.rp[i] = i * n;
+    for (size_t q = 0; q < (size_t)n * n; ++q) L.ci[q] = (int)(q % n);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "hipSetDevice");
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, "stream");
+    hipError_t e = hipSuccess;
+    bool ok = false;
+    double ms = 0.0;
+    {
+        DevAlloc A;
+        double* M = nullptr;
+        int N = 0;
+        e = dense_inverse_device(L, A, s, &M, &N, &ok, &ms);
+        if (e == hipSuccess && ok)
+            e = hipMemcpy2DAsync(inv, 8 * (size_t)n, M, 8 * (size_t)N, 8 * (size_t)n, n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return set_err(ctx, RSG_ERR_DEVICE, hipGetErrorString(e));
+    if (!ok) return set_err(ctx, RSG_ERR_ARG, "matrix is not positive definite (a pivot was not positive)");
+    if (kernel_seconds) *kernel_seconds = 1e-3 * ms;
     return RSG_OK;
 }
 

@@ -14,7 +14,10 @@
 //                                 ascending fine row then entry
 //   PCGSolver / AMG     :147-186, :602-653  PCG recurrence, tol relative to r0
 // The V-cycle smooths with damped Jacobi (omega 2/3, V(2,2)) and solves the
-// coarsest level (<= 64 rows) with a dense inverse, as the member V-cycle does.
+// coarsest level (<= 64 rows) with a dense inverse, as the member V-cycle does;
+// a larger coarsest level (where the greedy aggregation stalls, up to
+// RSG_DENSE_MAX rows) is inverted on the FP64 matrix cores (rsg_dense.h) and
+// applied as one matrix-vector product per V-cycle.
 // Kernels stay one phase each, CSR, one row per thread: on a 261,600-row
 // system with 17 levels, folding the residual into the restriction and the
 // prolongation into the next sweep, ELL images, and a single-workgroup kernel
@@ -26,6 +29,8 @@
 #pragma once
 #include <chrono>
 #include <vector>
+
+#include "rsg_dense.h"
 
 namespace rsg {
 namespace grid {
@@ -405,6 +410,8 @@ template <typename T> struct Hierarchy {
     std::vector<DevLevel<T>> lv;
     T* cinv = nullptr;
     bool dense = false;
+    const double* dinv = nullptr;   // FP64 inverse of a large coarsest level (row stride dinvN)
+    int dinvN = 0;
 };
 
 template <typename T>
@@ -443,6 +450,49 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
 
 inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
+// Explicit FP64 inverse of level L on the device (rsg_dense.h), row stride
+// *N = L.n rounded up to 64.  *ok is false when a pivot was not positive (the
+// caller then smooths the level); *ms gets the inversion's device time.
+inline hipError_t dense_inverse_device(const Level& L, DevAlloc& A, hipStream_t s, double** inv, int* N, bool* ok,
+                                       double* ms) {
+    *ok = false; *inv = nullptr; *ms = 0.0;
+    const int n = L.n, Np = dense::padded(n);
+    *N = Np;
+    int* rp = A.take<int>(n + 1); int* ci = A.take<int>(L.nnz); double* v = A.take<double>(L.nnz);
+    double* M = A.take<double>((size_t)Np * Np);
+    double* R = A.take<double>((size_t)dense::kNB * Np); double* C = A.take<double>((size_t)Np * dense::kNB);
+    double* P = A.take<double>(dense::kNB * dense::kNB);
+    int* flag = A.take<int>(1);
+    if (A.err != hipSuccess) return A.err;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemcpyAsync(rp, L.rp.data(), 4 * (size_t)(n + 1), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ci, L.ci.data(), 4 * (size_t)L.nnz, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(v, L.v.data(), 8 * (size_t)L.nnz, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        const size_t total = (size_t)Np * Np;
+        const int g = (int)std::min<size_t>((total + dense::kThreads - 1) / dense::kThreads, 8192);
+        hipLaunchKernelGGL(dense::k_dense_clear, dim3(g), dim3(dense::kThreads), 0, s, n, Np, M);
+        hipLaunchKernelGGL(dense::k_dense_scatter, dim3((n + dense::kThreads - 1) / dense::kThreads),
+                           dim3(dense::kThreads), 0, s, n, Np, (const int*)rp, (const int*)ci, (const double*)v, M);
+        e = hipEventRecord(e0, s);
+        if (e == hipSuccess) dense::invert(Np, M, R, C, P, flag, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        int hflag = 1;
+        if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, sizeof hflag, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        *ms = t;
+        *ok = e == hipSuccess && hflag == 0;
+        if (*ok) *inv = M;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return e;
+}
+
 // z = V-cycle(r) on hierarchy D (level 0 vectors: rin, z; level matrices in D)
 template <typename T>
 void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
@@ -450,6 +500,11 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
     if (L == 1) {
         const DevLevel<T>& d = D.lv[0];
         if (D.dense) { hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, d.n, D.cinv, rin, z); return; }
+        if (D.dinv) {
+            hipLaunchKernelGGL(dense::k_dense_apply<T>, dim3((d.n + 3) / 4), dim3(dense::kThreads), 0, s, d.n,
+                               D.dinvN, D.dinv, rin, z);
+            return;
+        }
         hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), dim3(kBlock), 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin,
                            (const T*)nullptr, z);
         return;
@@ -468,6 +523,9 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
         const DevLevel<T>& c = D.lv[L - 1];
         if (D.dense) {
             hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, c.n, D.cinv, (const T*)c.b, c.x);
+        } else if (D.dinv) {
+            hipLaunchKernelGGL(dense::k_dense_apply<T>, dim3((c.n + 3) / 4), dim3(dense::kThreads), 0, s, c.n,
+                               D.dinvN, D.dinv, (const T*)c.b, c.x);
         } else {
             const dim3 g(blocks(c.n)), bl(kBlock);
             hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)nullptr, c.x);

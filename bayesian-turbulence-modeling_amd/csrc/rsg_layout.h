@@ -16,6 +16,7 @@
 #define RSG_MAX_LEVELS 20     // AMGSolver::maxLevels_ (LinearSolver.hpp:508)
 #define RSG_MIN_COARSE 50     // AMGSolver::minCoarseSize_ (LinearSolver.hpp:509)
 #define RSG_COARSE_DENSE 64   // coarsest levels up to this size are inverted densely
+#define RSG_DENSE_MAX 16384   // larger coarsest levels up to this size: MFMA inverse on the device (2 GiB)
 #define RSG_ELL_K 4           // internal faces per cell in the cell-major (ELL) face slots
 // Resident solves: at most RSG_RES_MAXN cells per member (LDS exchange
 // buffers) and RSG_RES_MAXC coarse AMG rows (LDS coarse-level x).

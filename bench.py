@@ -672,7 +672,7 @@ def bench_c1(args, big=False):
     M = c1_system.csr_matrix(s)
 
     def solve(prec):
-        x = np.zeros(s["n"]); st = np.zeros(8)
+        x = np.zeros(s["n"]); st = np.zeros(10)
         t0 = time.perf_counter()
         rc = lib.rsg_system_solve(ctx, s["n"], s["nif"], ip(s["own"]), ip(s["nbr"]), dp(s["diag"]), dp(s["upper"]),
                                   dp(s["lower"]), dp(s["b"]), dp(x), 0, prec, 20000, C.c_double(1e-10), dp(st))
@@ -729,9 +729,11 @@ def bench_c1(args, big=False):
                                % ("2M-cell (bump-ensemble size)" if big else "C1 stand-alone", s["n"], s["n_tri"],
                                   s["n_quad"]), "cells": s["n"], "tol": 1e-10},
         "fp64": {"iterations": int(f["st"][0]), "levels": int(f["st"][4]), "setup_s": f["st"][6],
-                 "device_solve_s": f["st"][7], "wall_s": tf, "true_rel_res": f["true_res"]},
+                 "device_solve_s": f["st"][7], "wall_s": tf, "true_rel_res": f["true_res"],
+                 "coarsest_rows_inverted": int(f["st"][8]), "coarsest_inverse_s": f["st"][9]},
         "mixed": {"iterations": int(m["st"][0]), "refinements": int(m["st"][5]), "setup_s": m["st"][6],
                   "device_solve_s": m["st"][7], "wall_s": float(np.median(m["walls"])),
+                  "coarsest_rows_inverted": int(m["st"][8]), "coarsest_inverse_s": m["st"][9],
                   "true_rel_res": m["true_res"], "rel_l2_vs_fp64": rel},
         "roofline": None, "cpu_baseline": cpu}), flush=True)
 

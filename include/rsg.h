@@ -297,11 +297,27 @@ int rsg_ldu_solve_noalloc(rsg_context* ctx, int32_t n, int32_t nif, const int32_
  * reduction still missing (0.3 margin), at most 1e-5 relative.
  * Replaces AMGSolver::solve / PCGSolver::solve (core/include/LinearSolver.hpp:
  * 602-653, 147-186) on systems too large for one workgroup.
- * stats = {iterations (inner, summed), finalRes, initialRes, converged, levels,
- *          refinement sweeps, setup seconds, solve seconds}. */
+ * A coarsest level above 64 rows (where the greedy aggregation stalls) of at
+ * most 16,384 rows is inverted on the FP64 matrix cores once per call and
+ * applied as one dense product per V-cycle (the reference: 50 Gauss-Seidel
+ * sweeps, LinearSolver.hpp:508-512, 557-562); if a pivot of that inversion is
+ * not positive the level is smoothed by 51 Jacobi sweeps instead.
+ * stats (10 doubles) = {iterations (inner, summed), finalRes, initialRes,
+ *          converged, levels, refinement sweeps, setup seconds, solve seconds,
+ *          rows of the coarsest level inverted on the device (0: none),
+ *          seconds of that inversion}. */
 int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, const int32_t* nbr,
                      const double* diag, const double* upper, const double* lower, const double* b, double* x,
                      int32_t solver, int32_t precision, int32_t max_iter, double tol, double* stats);
+
+/* The explicit inverse of one dense SPD matrix (n x n, row-major, host
+ * buffers, n <= 16,384) by the device routine that inverts large AMG coarsest
+ * levels: blocked Gauss-Jordan (block 64, no pivoting) whose block products run
+ * on the FP64 matrix cores.  Stands where the reference solves its coarsest
+ * level by Gauss-Seidel sweeps (core/include/LinearSolver.hpp:508-512,
+ * 557-562).  RSG_ERR_ARG if a pivot is not positive (the matrix is not SPD).
+ * kernel_seconds (nullable) gets the device time of the inversion. */
+int rsg_dense_inverse(rsg_context* ctx, int32_t n, const double* a, double* inv, double* kernel_seconds);
 
 /* Host-only: the AMG hierarchy rsg_system_solve builds (AMGSolver::setup,
  * core/include/LinearSolver.hpp:516-550).  level_sizes/level_nnz get up to

@@ -64,11 +64,12 @@ def test_argument_errors_without_device(rsg_lib):
     assert rsg_lib.rsg_set_launch(None, 100, 0) == -1
     assert rsg_lib.rsg_set_time_slice(None, 1000) == -1
     assert rsg_lib.rsg_set_schedule(None, 0) == -1
-    st = np.zeros(8)
+    st = np.zeros(10)
     assert rsg_lib.rsg_system_solve(None, 1, 0, A.iptr(np.zeros(1, np.int32)), None, A.dptr(np.ones(1)), None, None,
                                     A.dptr(np.ones(1)), A.dptr(np.zeros(1)), 0, 0, 10, C.c_double(1e-8),
                                     A.dptr(st)) == -1
     assert rsg_lib.rsg_problem_set_observations(None, 0, 0, None, 0, None, None) == -1
     assert rsg_lib.rsg_set_team(None, 0, 0, 20) == -1
+    assert rsg_lib.rsg_dense_inverse(None, 4, A.dptr(np.eye(4).ravel()), A.dptr(np.zeros(16)), None) == -1
     assert rsg_lib.rsg_wall_distance(None, 1, A.dptr(np.zeros(3)), 1, A.dptr(np.zeros(3)), A.dptr(np.zeros(1)),
                                      None) == -1

@@ -27,7 +27,7 @@ SMALL = dict(nx_up=15, nx_dn=45, ny_up=20, ny_low=4, wall_layers=3)
 
 def solve(lib, ctx, s, solver, prec, tol, x0=None, max_iter=5000):
     x = np.zeros(s["n"]) if x0 is None else x0.copy()
-    st = np.zeros(8)
+    st = np.zeros(10)
     print("system solve n=%d solver=%d precision=%d tol=%g" % (s["n"], solver, prec, tol), flush=True)
     chk(lib, ctx, lib.rsg_system_solve(ctx, s["n"], s["nif"], A.iptr(s["own"]), A.iptr(s["nbr"]),
                                        A.dptr(s["diag"]), A.dptr(s["upper"]), A.dptr(s["lower"]),
@@ -100,7 +100,7 @@ def test_zero_rhs_is_converged_without_iterations(rsg_lib, gpu_ctx, small):
 
 
 def test_rejects_bad_arguments(rsg_lib, gpu_ctx, small):
-    x = np.zeros(small["n"]); st = np.zeros(8)
+    x = np.zeros(small["n"]); st = np.zeros(10)
     s = small
     args = (s["n"], s["nif"], A.iptr(s["own"]), A.iptr(s["nbr"]), A.dptr(s["diag"]), A.dptr(s["upper"]),
             A.dptr(s["lower"]), A.dptr(s["b"]), A.dptr(x))
@@ -119,5 +119,7 @@ def test_full_size_fp64_and_mixed(rsg_lib, gpu_ctx):
     xm, stm = solve(rsg_lib, gpu_ctx, s, AMG, MIXED, 1e-10)
     assert stm[3] == 1 and true_rel_res(s, xm) < 1.5e-10
     assert rel(xm, x64) < 1e-6
+    # the stalled coarsest level (7,779 rows) is inverted on the matrix cores
+    assert st64[8] == 7779 and stm[8] == 7779 and st64[9] > 0
     print("full size: fp64 %d it %.3f s, mixed %d it %d sweeps %.3f s, rel diff %.2e"
           % (st64[0], st64[7], stm[0], stm[5], stm[7], rel(xm, x64)), flush=True)
