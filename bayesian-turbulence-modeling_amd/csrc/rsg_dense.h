@@ -13,15 +13,21 @@
 // Inversion: blocked Gauss-Jordan without pivoting (the coarsest Galerkin
 // level of an SPD M-matrix is SPD, so every pivot block is an SPD Schur
 // complement block), block width 64, on the matrix padded to N = 64 k rows
-// with identity rows.  Step K (pivot block rows/cols [64K, 64K + 64)):
-//   P      = A_KK^-1                         k_gj_pivot  (one workgroup, LDS)
-//   R_J    = P A_KJ (J != K),  R_K = P        k_gj_panel  (MFMA, one workgroup per column tile)
-//   C_I    = A_IK                             (copied by k_gj_panel)
-//   A_IJ  <- [J != K] A_IJ - C_I R_J (I != K),  A_KJ <- R_J
-//                                             k_gj_update (MFMA, one workgroup per 64 x 64 tile)
-// which leaves A^-1 in place after N/64 steps: 2 N^3 flops, all but O(N^2 64)
-// of them on MFMA.  A pivot that is not positive or not finite sets a flag the
-// host reads after the last step (the caller then smooths the level instead).
+// with identity rows.  After the blocks S = {0..K-1} have been eliminated the
+// matrix holds  [[A_SS^-1, A_SS^-1 A_SU], [-A_US A_SS^-1, Schur complement]]:
+// tile (J, I) is +-tile (I, J)^T (minus across the S/U split), so only the
+// tiles I >= J are stored and updated -- N^3 flops instead of 2 N^3.  Step K:
+//   Y_J  = M_KJ    (from tile (K, J) for J < K, tile (J, K)^T for J > K)
+//   C_J  = M_JK    (= -Y_J^T for J < K, Y_J^T for J > K)
+//   R_J  = P Y_J,  R_K = P,  P = M_KK^-1                    k_gj_panel (MFMA)
+//   M_IJ <- [J != K] M_IJ - C_I R_J   (I >= J, I != K),  M_KJ <- R_J (J <= K)
+//                                                           k_gj_update (MFMA)
+// The pivot block of step K + 1 is updated first, by workgroup 0 of step K's
+// update, which then inverts it (Gauss-Jordan in LDS) while
+// the other workgroups update the rest; after the last step k_gj_mirror
+// copies the lower tiles to the upper ones.  A pivot that is not positive or
+// not finite sets a flag the host reads after the last step (the caller then
+// smooths the level instead).
 //
 // MFMA lane layout of v_mfma_f64_16x16x4f64 (one wave): lane l supplies
 // A[l % 16][l / 16] of the 16 x 4 operand and B[l / 16][l % 16] of the 4 x 16
@@ -64,36 +70,43 @@ __global__ __launch_bounds__(kThreads) void k_dense_scatter(int n, int N, const 
 }
 
 // --------------------------------------------------------- pivot block
-// P = A_KK^-1 by Gauss-Jordan in LDS (no pivoting).  flag[0] is set to 1 if a
-// pivot is not positive or not finite.
-__global__ __launch_bounds__(kThreads) void k_gj_pivot(int N, int K, const double* __restrict__ A,
-                                                       double* __restrict__ P, int* __restrict__ flag) {
-    __shared__ double G[kNB][kLd];
-    __shared__ double piv;
-    const int t = threadIdx.x;
-    const size_t base = (size_t)K * kNB * N + (size_t)K * kNB;
-    for (int q = t; q < kNB * kNB; q += kThreads) G[q / kNB][q % kNB] = A[base + (size_t)(q / kNB) * N + q % kNB];
-    __syncthreads();
+// P = G^-1 (64 x 64 in LDS, overwritten; P row-major in global memory) by
+// Gauss-Jordan without pivoting, all threads of the workgroup (kThreads):
+// thread t owns column t % 64 of rows t / 64 + 4 q.  Two barriers per step:
+// every thread reads the pivot and its factors, then updates its entries.
+// flag[0] = 1 if a pivot is not positive or not finite.
+__device__ inline void block_gj(double (*G)[kLd], double* __restrict__ P, int* __restrict__ flag) {
+    constexpr int RPT = kNB / (kThreads / kNB);     // rows per thread
+    const int t = threadIdx.x, j = t % kNB, r0 = t / kNB;
+    bool bad = false;
     for (int k = 0; k < kNB; ++k) {
-        if (t == 0) {
-            piv = G[k][k];
-            if (!(piv > 0.0) || !std::isfinite(piv)) { flag[0] = 1; piv = 1.0; }
-        }
-        __syncthreads();
-        // column k of the other rows, read before row k is scaled
-        double fct[kNB / (kThreads / kNB)];
-        const int j = t % kNB, r0 = t / kNB;          // thread: column j, rows r0, r0 + 4, ...
-        for (int q = 0; q < kNB / (kThreads / kNB); ++q) fct[q] = G[r0 + 4 * q][k];
+        double piv = G[k][k];
+        if (!(piv > 0.0) || !std::isfinite(piv)) { bad = true; piv = 1.0; }
+        double fct[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) fct[q] = G[r0 + 4 * q][k];
         const double rowk = (j == k ? 1.0 : G[k][j]) / piv;
         __syncthreads();
-        for (int q = 0; q < kNB / (kThreads / kNB); ++q) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
             const int i = r0 + 4 * q;
-            if (i == k) G[i][j] = rowk;
-            else G[i][j] = (j == k ? 0.0 : G[i][j]) - fct[q] * rowk;
+            G[i][j] = (i == k) ? rowk : (j == k ? 0.0 : G[i][j]) - fct[q] * rowk;
         }
         __syncthreads();
     }
     for (int q = t; q < kNB * kNB; q += kThreads) P[q] = G[q / kNB][q % kNB];
+    if (bad && t == 0) flag[0] = 1;
+}
+
+// P = inverse of diagonal tile (K, K).
+__global__ __launch_bounds__(kThreads) void k_gj_pivot(int N, int K, const double* __restrict__ A,
+                                                       double* __restrict__ P, int* __restrict__ flag) {
+    __shared__ double G[kNB][kLd];
+    const size_t base = (size_t)K * kNB * N + (size_t)K * kNB;
+    for (int q = threadIdx.x; q < kNB * kNB; q += kThreads)
+        G[q / kNB][q % kNB] = A[base + (size_t)(q / kNB) * N + q % kNB];
+    __syncthreads();
+    block_gj(G, P, flag);
 }
 
 // ------------------------------------------------------------ MFMA tile
@@ -115,30 +128,41 @@ __device__ inline void mfma_tile(const double (*Xt)[kLd], const double (*Y)[kLd]
     }
 }
 
-// ----------------------------------------------------------- row panel
-// Workgroup J (column tile J): R[:, J] = P A_KJ (J != K) or P (J == K), and
-// C[J rows, :] = A[J rows, K cols].  R is 64 x N, C is N x 64, row-major.
+// lower-triangle tile t (row-major over I >= J) -> (I, J)
+__device__ inline void tri_tile(long t, int& I, int& J) {
+    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((long)i * (i + 1) / 2 > t) --i;
+    while ((long)(i + 1) * (i + 2) / 2 <= t) ++i;
+    I = i;
+    J = (int)(t - (long)i * (i + 1) / 2);
+}
+
+// ----------------------------------------------------------- panels
+// Workgroup J: Y_J = M_KJ from the stored lower tile, CT[:, J] = C_J^T
+// (64 x N, CT[k][J 64 + i] = M_JK[i][k]) and R[:, J] = P Y_J (R_K = P).
 __global__ __launch_bounds__(kThreads) void k_gj_panel(int N, int K, const double* __restrict__ A,
                                                        const double* __restrict__ P, double* __restrict__ R,
-                                                       double* __restrict__ C) {
+                                                       double* __restrict__ CT) {
     __shared__ double Pt[kNB][kLd];
     __shared__ double Y[kNB][kLd];
     const int J = blockIdx.x, t = threadIdx.x;
-    // C tile: rows of tile J, columns of the pivot block
-    for (int q = t; q < kNB * kNB; q += kThreads) {
-        const int i = q / kNB, k = q % kNB;
-        C[(size_t)(J * kNB + i) * kNB + k] = A[(size_t)(J * kNB + i) * N + K * kNB + k];
-    }
     if (J == K) {
-        for (int q = t; q < kNB * kNB; q += kThreads) R[(size_t)(q / kNB) * N + K * kNB + q % kNB] = P[q];
+        for (int q = t; q < kNB * kNB; q += kThreads) {
+            R[(size_t)(q / kNB) * N + K * kNB + q % kNB] = P[q];
+            CT[(size_t)(q / kNB) * N + K * kNB + q % kNB] = 0.0;
+        }
         return;
     }
+    const double sign = J < K ? -1.0 : 1.0;
     for (int q = t; q < kNB * kNB; q += kThreads) {
         const int i = q / kNB, k = q % kNB;
-        Pt[k][i] = P[q];                                              // P[i][k]
-        Y[i][k] = A[(size_t)(K * kNB + i) * N + J * kNB + k];         // A_KJ[i][k]
+        Pt[k][i] = P[q];                                                          // P[i][k]
+        if (J < K) Y[i][k] = A[(size_t)(K * kNB + i) * N + J * kNB + k];         // tile (K, J)
+        else Y[k][i] = A[(size_t)(J * kNB + i) * N + K * kNB + k];               // tile (J, K)^T
     }
     __syncthreads();
+    for (int q = t; q < kNB * kNB; q += kThreads)
+        CT[(size_t)(q / kNB) * N + J * kNB + q % kNB] = sign * Y[q / kNB][q % kNB];
     const int w = t / 64, lane = t % 64, wr = w / 2, wc = w % 2;
     v4d acc[2][2];
     for (int a = 0; a < 2; ++a)
@@ -153,39 +177,83 @@ __global__ __launch_bounds__(kThreads) void k_gj_panel(int N, int K, const doubl
 }
 
 // ------------------------------------------------------- trailing update
-// Workgroup (I, J): A_IJ <- [J != K] A_IJ - C_I R_J for I != K; A_KJ <- R_J.
+// One workgroup per stored tile (I >= J).  Tile row K: M_KJ <- R_J.  Others:
+// M_IJ <- [J != K] M_IJ - C_I R_J, operands read straight from CT and R in
+// the MFMA lane layout (both coalesced: 16 consecutive doubles per k).
+// Workgroup 0 takes tile (K + 1, K + 1) and then inverts it into Pn.
 __global__ __launch_bounds__(kThreads) void k_gj_update(int N, int K, double* __restrict__ A,
-                                                        const double* __restrict__ R, const double* __restrict__ C) {
-    __shared__ double Xt[kNB][kLd];
-    __shared__ double Y[kNB][kLd];
+                                                        const double* __restrict__ R, const double* __restrict__ CT,
+                                                        double* __restrict__ Pn, int* __restrict__ flag) {
+    __shared__ double G[kNB][kLd];
     const int nt = N / kNB;
-    const int I = blockIdx.x / nt, J = blockIdx.x % nt, t = threadIdx.x;
+    const bool fuse = K + 1 < nt;
+    int I, J;
+    if (fuse && blockIdx.x == 0) {
+        I = J = K + 1;
+    } else {
+        long t = fuse ? (long)blockIdx.x - 1 : (long)blockIdx.x;
+        if (fuse && t >= (long)(K + 1) * (K + 2) / 2 + (K + 1)) ++t;             // skip tile (K+1, K+1)
+        tri_tile(t, I, J);
+    }
+    const int tid = threadIdx.x;
     if (I == K) {
-        for (int q = t; q < kNB * kNB; q += kThreads) {
+        for (int q = tid; q < kNB * kNB; q += kThreads) {
             const int i = q / kNB, j = q % kNB;
             A[(size_t)(K * kNB + i) * N + J * kNB + j] = R[(size_t)i * N + J * kNB + j];
         }
         return;
     }
-    for (int q = t; q < kNB * kNB; q += kThreads) {
-        const int i = q / kNB, k = q % kNB;
-        Xt[k][i] = C[(size_t)(I * kNB + i) * kNB + k];               // C_I[i][k]
-        Y[i][k] = R[(size_t)i * N + J * kNB + k];                     // R_J[i][k]
-    }
-    __syncthreads();
-    const int w = t / 64, lane = t % 64, wr = w / 2, wc = w % 2;
-    v4d acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-    mfma_tile(Xt, Y, wr, wc, lane, acc);
+    const int w = tid / 64, lane = tid % 64, wr = w / 2, wc = w % 2;
+    const int r = lane % 16, kq = lane / 16;
     const bool keep = J != K;
+    // the old tile values first (their loads overlap the products)
+    double old[2][2][4];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int i = 0; i < 4; ++i) {
-                const int row = wr * 32 + a * 16 + lane / 16 + 4 * i, col = wc * 32 + b * 16 + lane % 16;
-                double* dst = A + (size_t)(I * kNB + row) * N + J * kNB + col;
-                *dst = (keep ? *dst : 0.0) - acc[a][b][i];
+                const int row = wr * 32 + a * 16 + kq + 4 * i, col = wc * 32 + b * 16 + r;
+                old[a][b][i] = keep ? A[(size_t)(I * kNB + row) * N + J * kNB + col] : 0.0;
             }
+    v4d acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    const double* cI = CT + (size_t)I * kNB + wr * 32 + r;
+    const double* rJ = R + (size_t)J * kNB + wc * 32 + r;
+#pragma unroll
+    for (int k0 = 0; k0 < kNB; k0 += 4) {
+        const size_t o = (size_t)(k0 + kq) * N;
+        const double a0 = cI[o], a1 = cI[o + 16], b0 = rJ[o], b1 = rJ[o + 16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    const bool pivot = fuse && blockIdx.x == 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int i = 0; i < 4; ++i) {
+                const int row = wr * 32 + a * 16 + kq + 4 * i, col = wc * 32 + b * 16 + r;
+                const double v = old[a][b][i] - acc[a][b][i];
+                A[(size_t)(I * kNB + row) * N + J * kNB + col] = v;
+                if (pivot) G[row][col] = v;
+            }
+    if (!pivot) return;
+    __syncthreads();
+    block_gj(G, Pn, flag);
+}
+
+// Upper tiles from the lower ones (the inverse is symmetric): tile (I, J),
+// I < J, = tile (J, I)^T.  One workgroup per upper tile.
+__global__ __launch_bounds__(kThreads) void k_gj_mirror(int N, double* __restrict__ A) {
+    __shared__ double T[kNB][kLd];
+    int J, I;
+    tri_tile(blockIdx.x, J, I);          // strictly-lower index space of (J - 1, I)
+    ++J;
+    for (int q = threadIdx.x; q < kNB * kNB; q += kThreads)
+        T[q % kNB][q / kNB] = A[(size_t)(J * kNB + q / kNB) * N + I * kNB + q % kNB];
+    __syncthreads();
+    for (int q = threadIdx.x; q < kNB * kNB; q += kThreads)
+        A[(size_t)(I * kNB + q / kNB) * N + J * kNB + q % kNB] = T[q / kNB][q % kNB];
 }
 
 // ------------------------------------------------------------- apply
@@ -205,18 +273,23 @@ __global__ __launch_bounds__(kThreads) void k_dense_apply(int n, int N, const do
 
 __global__ void k_flag_clear(int* flag) { flag[0] = 0; }
 
-// Inverts the dense SPD image A (N x N, padded) in place on stream s; R
-// (64 x N), C (N x 64), P (64 x 64) and flag (one int, cleared here) are
-// device workspace.  The caller reads flag after the stream has finished.
-inline void invert(int N, double* A, double* R, double* C, double* P, int* flag, hipStream_t s) {
+// Inverts the dense SPD image A (N x N, padded) in place on stream s; R and CT
+// (64 x N each), P (2 x 64 x 64) and flag (one int, cleared here) are device
+// workspace.  The caller reads flag after the stream has finished.
+inline void invert(int N, double* A, double* R, double* CT, double* P, int* flag, hipStream_t s) {
     const int nt = N / kNB;
+    const int lower = nt * (nt + 1) / 2;
     hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag);
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kThreads), 0, s, N, 0, (const double*)A, P, flag);
     for (int K = 0; K < nt; ++K) {
-        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kThreads), 0, s, N, K, (const double*)A, P, flag);
-        hipLaunchKernelGGL(k_gj_panel, dim3(nt), dim3(kThreads), 0, s, N, K, (const double*)A, (const double*)P, R, C);
-        hipLaunchKernelGGL(k_gj_update, dim3(nt * nt), dim3(kThreads), 0, s, N, K, A, (const double*)R,
-                           (const double*)C);
+        double* Pk = P + (K & 1) * kNB * kNB;
+        double* Pn = P + ((K + 1) & 1) * kNB * kNB;
+        hipLaunchKernelGGL(k_gj_panel, dim3(nt), dim3(kThreads), 0, s, N, K, (const double*)A, (const double*)Pk,
+                           R, CT);
+        hipLaunchKernelGGL(k_gj_update, dim3(lower), dim3(kThreads), 0, s, N, K, A, (const double*)R,
+                           (const double*)CT, Pn, flag);
     }
+    if (nt > 1) hipLaunchKernelGGL(k_gj_mirror, dim3(nt * (nt - 1) / 2), dim3(kThreads), 0, s, N, A);
 }
 
 }  // namespace

@@ -461,7 +461,7 @@ inline hipError_t dense_inverse_device(const Level& L, DevAlloc& A, hipStream_t 
     int* rp = A.take<int>(n + 1); int* ci = A.take<int>(L.nnz); double* v = A.take<double>(L.nnz);
     double* M = A.take<double>((size_t)Np * Np);
     double* R = A.take<double>((size_t)dense::kNB * Np); double* C = A.take<double>((size_t)Np * dense::kNB);
-    double* P = A.take<double>(dense::kNB * dense::kNB);
+    double* P = A.take<double>(2 * dense::kNB * dense::kNB);
     int* flag = A.take<int>(1);
     if (A.err != hipSuccess) return A.err;
     hipEvent_t e0 = nullptr, e1 = nullptr;
