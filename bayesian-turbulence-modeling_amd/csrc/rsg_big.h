@@ -12,9 +12,13 @@ struct RsgBig;
 
 // State of one large member whose workspace (hM, device pointers) is already
 // carved and staged; dM: device memory for the member struct.  sc0: its
-// initial scalars.  Returns null (and err) on a device error.
+// initial scalars.  maxLevels: 0 coarsens like the reference (at most 20
+// levels, down to 50 rows); k > 0 stops at k levels once the coarsest level
+// can be inverted on the matrix cores (at most 16,384 rows).  Returns null
+// (and err) on a device error.
 RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set,
-                       const rsg::PrepMesh& pm, const RsgScalars& sc0, hipStream_t s, std::string& err);
+                       const rsg::PrepMesh& pm, const RsgScalars& sc0, hipStream_t s, int maxLevels,
+                       std::string& err);
 // Runs up to `iters` outer iterations (fewer if the member finishes); out gets
 // the member's scalars.  A finished member's fields, history, scalars and
 // observables are on the device for collection.

@@ -287,6 +287,10 @@ struct RsgBig {
     std::vector<int*> gP, gI;
     int* dOk = nullptr;
     bool haveH = false;
+    int maxLevels = 0;            // 0: the reference's cap; else levels before an MFMA-inverted coarsest
+    grid::DenseWork DW;           // the coarsest level's inverse when it is above 64 rows
+    bool bigDense = false;
+    double denseMs = 0.0;         // device time of the inversions so far
     // outer-iteration state (the device member's Scal, kept on the host)
     RsgScalars sc{};
     std::vector<double> hist;
@@ -387,6 +391,8 @@ hipError_t build_amg(RsgBig& B) {
     for (int l = 0; l + 1 < RSG_MAX_LEVELS; ++l) {
         grid::Level& f = H.back();
         if (f.n <= RSG_MIN_COARSE) break;
+        // a capped hierarchy ends as soon as its coarsest level can be inverted
+        if (B.maxLevels > 0 && (int)H.size() >= B.maxLevels && f.n <= RSG_DENSE_MAX) break;
         grid::Level c;
         const bool ok = grid::coarsen(f, c);
         if ((!ok || c.n > 0.75 * f.n) && f.n > RSG_COARSE_DENSE) {
@@ -433,10 +439,19 @@ hipError_t build_amg(RsgBig& B) {
     }
     std::vector<double> G;
     const bool dense = H.back().n <= RSG_COARSE_DENSE && grid::dense_inverse(H.back(), G);
+    B.DW.release();
     B.HA.reset(new grid::DevAlloc());
     B.H = grid::Hierarchy<double>();
     e = grid::upload<double>(H, G, dense, *B.HA, B.H, B.s);
     if (e != hipSuccess) return e;
+    // a coarsest level above 64 rows: explicit inverse on the matrix cores,
+    // recomputed with every refresh of the coarse values
+    B.bigDense = false;
+    B.DW = grid::DenseWork();
+    if (!dense && H.size() > 1 && H.back().n <= RSG_DENSE_MAX) {
+        if (!B.DW.take(H.back().n, *B.HA)) return B.HA->err;
+        B.bigDense = true;
+    }
     B.gP.assign(H.size(), nullptr); B.gI.assign(H.size(), nullptr);
     for (size_t l = 1; l < H.size(); ++l) {
         B.gP[l] = B.HA->take<int>(gPh[l].size());
@@ -475,6 +490,16 @@ hipError_t refresh_amg(RsgBig& B, bool coarse) {
         if (e == hipSuccess) e = hipStreamSynchronize(B.s);
         if (e != hipSuccess) return e;
         if (!ok) H.dense = false;     // zero pivot: the coarsest level is smoothed instead
+    }
+    if (B.bigDense) {
+        const grid::DevLevel<double>& c = H.lv[L - 1];
+        bool ok = false;
+        double ms = 0.0;
+        hipError_t e = grid::dense_invert(B.DW, c.rp, c.ci, c.v, B.s, &ok, &ms);
+        if (e != hipSuccess) return e;
+        B.denseMs += ms;
+        H.dinv = ok ? B.DW.M : nullptr;     // a pivot not positive: smoothed instead
+        H.dinvN = B.DW.N;
     }
     return hipGetLastError();
 }
@@ -547,7 +572,7 @@ hipError_t outer_iteration(RsgBig& B) {
         if (e == hipSuccess) e = refresh_amg(B, coarse);
         if (coarse) sc.coarseIter = iter;
         sc.nLevels = (int)B.H.lv.size();
-        sc.coarseDense = B.H.dense ? 1 : 0;
+        sc.coarseDense = (B.H.dense || B.H.dinv) ? 1 : 0;
         amgReady = true;
     }
     SolveRes rP{0, 0.0, 0.0, 0};
@@ -616,8 +641,9 @@ hipError_t outer_iteration(RsgBig& B) {
 }  // namespace
 
 RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set, const PrepMesh& pm,
-                       const RsgScalars& sc0, hipStream_t s, std::string& err) {
+                       const RsgScalars& sc0, hipStream_t s, int maxLevels, std::string& err) {
     std::unique_ptr<RsgBig> B(new RsgBig());
+    B->maxLevels = maxLevels;
     B->hM = hM; B->dM = dM; B->set = set; B->pm = &pm; B->nC = pm.nC; B->nnz0 = pm.nnz0; B->s = s;
     B->sc = sc0;
     RsgMeshDev mesh;
@@ -639,6 +665,7 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
 
 void rsg_big_destroy(RsgBig* B) {
     if (!B) return;
+    B->DW.release();
     if (B->S.hdots) (void)hipHostFree(B->S.hdots);
     delete B;
 }

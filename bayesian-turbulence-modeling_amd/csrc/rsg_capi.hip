@@ -209,6 +209,7 @@ struct rsg_context {
     // kernel-per-phase solves (rsg_set_grid): members on meshes of at least
     // gridMinCells cells (0: never), gridIters outer iterations per engine round
     int gridMinCells = 0, gridIters = 20;
+    int gridLevels = 5;      // p' AMG levels of kernel-per-phase members (0: the reference's cap)
     double* dGred = nullptr;
     RsgMemberDev* dTeamM = nullptr;
     RsgScalars* dTeamOut = nullptr;
@@ -475,6 +476,7 @@ void engine_loop(rsg_context* ctx) {
         std::vector<RsgMemberDev> gridM(gridIds.size());
         std::vector<RsgSettingsDev> gridSet(gridIds.size());
         std::vector<const rsg::PrepMesh*> gridMesh(gridIds.size());
+        const int gridLevels = ctx->gridLevels;
         for (size_t q = 0; q < gridIds.size(); ++q) {
             Ticket& T = *ctx->tickets[gridIds[q]];
             ProblemEntry& P = *ctx->problems[T.problemId];
@@ -493,7 +495,7 @@ void engine_loop(rsg_context* ctx) {
             if (!T.big) {
                 RsgScalars s0;
                 member_init_scalars(s0, T.warm ? T.warmTurb : 0);
-                T.big = rsg_big_create(gridM[q], T.dM, gridSet[q], *gridMesh[q], s0, ctx->stream, berr);
+                T.big = rsg_big_create(gridM[q], T.dM, gridSet[q], *gridMesh[q], s0, ctx->stream, gridLevels, berr);
                 if (!T.big) { e = hipErrorUnknown; gridErr = berr; break; }
             }
             e = hipEventRecord(ctx->ev0, ctx->stream);
@@ -989,6 +991,15 @@ int rsg_set_grid(rsg_context* ctx, int32_t min_cells, int32_t iters_per_launch) 
     if (iters_per_launch <= 0) return set_err(ctx, RSG_ERR_ARG, "iters_per_launch must be > 0");
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->gridMinCells = min_cells; ctx->gridIters = iters_per_launch;
+    return RSG_OK;
+}
+
+int rsg_set_grid_levels(rsg_context* ctx, int32_t max_levels) {
+    if (!ctx) return RSG_ERR_ARG;
+    if (max_levels < 0 || max_levels == 1 || max_levels > RSG_MAX_LEVELS)
+        return set_err(ctx, RSG_ERR_ARG, "max_levels must be 0 or 2..20");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->gridLevels = max_levels;
     return RSG_OK;
 }
 

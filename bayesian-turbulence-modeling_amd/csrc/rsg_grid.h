@@ -450,46 +450,71 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
 
 inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
-// Explicit FP64 inverse of level L on the device (rsg_dense.h), row stride
-// *N = L.n rounded up to 64.  *ok is false when a pivot was not positive (the
-// caller then smooths the level); *ms gets the inversion's device time.
+// Device workspace of the dense coarsest-level inverse (rsg_dense.h): the
+// padded N x N image (the inverse after dense_invert), the two 64 x N panels,
+// two pivot blocks and the pivot flag.
+struct DenseWork {
+    int n = 0, N = 0;
+    double *M = nullptr, *R = nullptr, *CT = nullptr, *P = nullptr;
+    int* flag = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool take(int rows, DevAlloc& A) {
+        n = rows; N = dense::padded(rows);
+        M = A.take<double>((size_t)N * N);
+        R = A.take<double>((size_t)dense::kNB * N); CT = A.take<double>((size_t)N * dense::kNB);
+        P = A.take<double>(2 * dense::kNB * dense::kNB);
+        flag = A.take<int>(1);
+        if (A.err == hipSuccess) A.err = hipEventCreate(&e0);
+        if (A.err == hipSuccess) A.err = hipEventCreate(&e1);
+        return A.err == hipSuccess;
+    }
+    void release() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        e0 = e1 = nullptr;
+    }
+};
+
+// W.M = inverse of the n-row CSR matrix (rp, ci, v: device arrays) on stream
+// s; *ok is false when a pivot was not positive (the caller then smooths the
+// level); *ms gets the inversion's device time.  Waits for the stream.
+inline hipError_t dense_invert(DenseWork& W, const int* rp, const int* ci, const double* v, hipStream_t s, bool* ok,
+                               double* ms) {
+    *ok = false; *ms = 0.0;
+    const size_t total = (size_t)W.N * W.N;
+    const int g = (int)std::min<size_t>((total + dense::kThreads - 1) / dense::kThreads, 8192);
+    hipLaunchKernelGGL(dense::k_dense_clear, dim3(g), dim3(dense::kThreads), 0, s, W.n, W.N, W.M);
+    hipLaunchKernelGGL(dense::k_dense_scatter, dim3((W.n + dense::kThreads - 1) / dense::kThreads),
+                       dim3(dense::kThreads), 0, s, W.n, W.N, rp, ci, v, W.M);
+    hipError_t e = hipEventRecord(W.e0, s);
+    if (e == hipSuccess) dense::invert(W.N, W.M, W.R, W.CT, W.P, W.flag, s);
+    if (e == hipSuccess) e = hipEventRecord(W.e1, s);
+    int hflag = 1;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hflag, W.flag, sizeof hflag, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, W.e0, W.e1);
+    *ms = t;
+    *ok = e == hipSuccess && hflag == 0;
+    return e;
+}
+
+// Explicit FP64 inverse of level L (host CSR) on the device: *inv (row stride
+// *N = L.n rounded up to 64) lives in A.  *ok as for dense_invert.
 inline hipError_t dense_inverse_device(const Level& L, DevAlloc& A, hipStream_t s, double** inv, int* N, bool* ok,
                                        double* ms) {
     *ok = false; *inv = nullptr; *ms = 0.0;
-    const int n = L.n, Np = dense::padded(n);
-    *N = Np;
+    const int n = L.n;
     int* rp = A.take<int>(n + 1); int* ci = A.take<int>(L.nnz); double* v = A.take<double>(L.nnz);
-    double* M = A.take<double>((size_t)Np * Np);
-    double* R = A.take<double>((size_t)dense::kNB * Np); double* C = A.take<double>((size_t)Np * dense::kNB);
-    double* P = A.take<double>(2 * dense::kNB * dense::kNB);
-    int* flag = A.take<int>(1);
-    if (A.err != hipSuccess) return A.err;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DenseWork W;
+    if (!W.take(n, A)) { W.release(); return A.err; }
+    *N = W.N;
     hipError_t e = hipMemcpyAsync(rp, L.rp.data(), 4 * (size_t)(n + 1), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(ci, L.ci.data(), 4 * (size_t)L.nnz, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(v, L.v.data(), 8 * (size_t)L.nnz, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) {
-        const size_t total = (size_t)Np * Np;
-        const int g = (int)std::min<size_t>((total + dense::kThreads - 1) / dense::kThreads, 8192);
-        hipLaunchKernelGGL(dense::k_dense_clear, dim3(g), dim3(dense::kThreads), 0, s, n, Np, M);
-        hipLaunchKernelGGL(dense::k_dense_scatter, dim3((n + dense::kThreads - 1) / dense::kThreads),
-                           dim3(dense::kThreads), 0, s, n, Np, (const int*)rp, (const int*)ci, (const double*)v, M);
-        e = hipEventRecord(e0, s);
-        if (e == hipSuccess) dense::invert(Np, M, R, C, P, flag, s);
-        if (e == hipSuccess) e = hipEventRecord(e1, s);
-        int hflag = 1;
-        if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, sizeof hflag, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        float t = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-        *ms = t;
-        *ok = e == hipSuccess && hflag == 0;
-        if (*ok) *inv = M;
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
+    if (e == hipSuccess) e = dense_invert(W, rp, ci, v, s, ok, ms);
+    if (*ok) *inv = W.M;
+    W.release();
     return e;
 }
 

@@ -386,6 +386,9 @@ PYBIND11_MODULE(rans_sst_py, m) {
           "Members on meshes of at least min_cells cells (checked at admission) run alone on the whole GPU, "
           "every phase of the outer iteration a grid-wide launch, iters_per_launch outer iterations per engine "
           "round; min_cells 0 turns it off.  Takes precedence over team solves.");
+    m.def("engine_set_grid_levels", [](int levels) { Engine::get().setGridLevels(levels); }, py::arg("max_levels"),
+          "p' AMG levels of kernel-per-phase members (0: the reference's 20-level cap down to 50 rows; k > 1: k "
+          "levels, the coarsest inverted on the FP64 matrix cores).");
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
     m.def("engine_set_schedule", [](const std::string& p) {
               if (p == "round_robin") Engine::get().setSchedule(RSG_SCHEDULE_ROUND_ROBIN);
@@ -405,6 +408,18 @@ PYBIND11_MODULE(rans_sst_py, m) {
           py::arg("tickets"),
           "Engine state of each ticket: 0 pending, 1 running, 2 done, 3 failed (does not block).");
     m.def("engine_stats", []() { return Engine::get().stats(); });
+    m.def("engine_device_info",
+          []() {
+              int32_t out[2] = {0, 0};
+              rsg_context* c = Engine::get().ctx();
+              const int rc = rsg_device_info(c, out);
+              if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+              py::dict d;
+              d["cu_count"] = out[0];
+              d["wavefront_size"] = out[1];
+              return d;
+          },
+          "Compute units and wavefront size of the engine's device.");
     m.def("_shutdown", []() { Engine::get().shutdown(); });
     py::module_::import("atexit").attr("register")(m.attr("_shutdown"));
 

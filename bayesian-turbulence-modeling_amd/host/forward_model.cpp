@@ -73,6 +73,12 @@ void Engine::setGrid(int minCells, int itersPerLaunch) {
     if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
 }
 
+void Engine::setGridLevels(int maxLevels) {
+    rsg_context* c = ctx();
+    int rc = rsg_set_grid_levels(c, maxLevels);
+    if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
+}
+
 void Engine::setTeam(int minCells, int blocks, int itersPerLaunch) {
     rsg_context* c = ctx();
     int rc = rsg_set_team(c, minCells, blocks, itersPerLaunch);

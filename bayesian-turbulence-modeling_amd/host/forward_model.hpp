@@ -51,6 +51,8 @@ public:
     // members on meshes of at least minCells cells run kernel-per-phase on the
     // whole GPU (0 turns it off)
     void setGrid(int minCells, int itersPerLaunch);
+    // p' AMG levels of kernel-per-phase members (0: the reference's cap)
+    void setGridLevels(int maxLevels);
     std::vector<double> stats();
 private:
     std::mutex mu_;

@@ -579,6 +579,8 @@ def bench_bump(args):
     def run(mesh, iters, mode, blocks=0):
         rs.engine_set_team(20000 if mode == "team" else 0, blocks, 20)
         rs.engine_set_grid(20000 if mode == "grid" else 0, 20)
+        if mode == "grid":
+            rs.engine_set_grid_levels(blocks)
         s = rs.SolverSettings()
         s.max_iterations = iters
         s.convergence_tol = 1e-12
@@ -608,7 +610,7 @@ def bench_bump(args):
         paths = args.bump_paths.split(",")
         legs = []
         if "grid" in paths:
-            legs.append(("grid", "grid", 0))
+            legs += [("grid" if lv == 5 else "grid_L%d" % lv, "grid", lv) for lv in map(int, args.grid_levels.split(","))]
         if "team" in paths:
             legs += [("team" if b == 0 else "team_%d" % b, "team", b) for b in map(int, args.team_blocks.split(","))]
         if n <= 40000 and not args.no_member:
@@ -636,7 +638,7 @@ def bench_bump(args):
             row[name] = leg
         rows.append(row)
     big = rows[-1]
-    cands = [v for k, v in big.items() if (k.startswith("team") or k == "grid") and isinstance(v, dict)
+    cands = [v for k, v in big.items() if (k.startswith("team") or k.startswith("grid")) and isinstance(v, dict)
              and v.get("ms_per_outer_iter_kernel")]
     best = min(cands, key=lambda v: v["ms_per_outer_iter_kernel"]) if cands else {}
     t = best.get("ms_per_outer_iter_kernel")
@@ -746,6 +748,17 @@ def bench_c2(args):
     import ensembles
     specs = ensembles.channel_ensemble(args.members, args.seed)
     configure_engine(rs, args)
+    tuned = None
+    if args.autotune:
+        # workgroup size and time slice measured on this device and ensemble
+        # (outside the timed region); the engine keeps the pick
+        import autotune
+        t0 = time.perf_counter()
+        choice, _ = autotune.tune_members(rs, specs, args.precision, cache_path=args.tune_cache or None)
+        args.block, args.slice_ms = choice["block"], choice["slice_ms"]
+        tuned = dict(choice, seconds=time.perf_counter() - t0)
+        print("bench: autotune %.1f s: block %d, slice %g ms" % (tuned["seconds"], args.block, args.slice_ms),
+              file=sys.stderr, flush=True)
     # engine start-up, code-object load and one short pass of every member
     # (100 outer iterations) outside the timed region
     warm = 0
@@ -842,6 +855,7 @@ def bench_c2(args):
                      "alg_bytes_per_launch": alg_per_launch, "alg_bytes_total": bytes_alg,
                      "traffic_measurement": traffic_info},
         "cpu_baseline": cpu,
+        "autotune": tuned,
         "bench_wall_s": time.perf_counter() - t_begin,
     }
     print(json.dumps(out), flush=True)
@@ -867,6 +881,10 @@ def main():
                     help="headline: wall-clock budget of the whole run; optional legs are skipped when it runs out")
     ap.add_argument("--step-budget-s", type=float, default=300.0,
                     help="headline: timed steps after the first run only while the total stays within this")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="headline: pick the workgroup size and time slice by measuring candidates on the ensemble "
+                         "(1, the default) or keep --block / --slice-ms (0)")
+    ap.add_argument("--tune-cache", default="", help="JSON file caching autotuner picks (empty: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
@@ -880,6 +898,9 @@ def main():
     ap.add_argument("--team-blocks", default="0",
                     help="bump workload: comma-separated team sizes in workgroups (0: the engine's choice)")
     ap.add_argument("--no-member", action="store_true", help="bump workload: skip the one-workgroup leg")
+    ap.add_argument("--grid-levels", default="5",
+                    help="bump workload: comma-separated p' AMG level caps of the kernel-per-phase path (5: the "
+                         "default, coarsest inverted on the matrix cores; 0: the reference's 20-level cap)")
     ap.add_argument("--bump-paths", default="grid,team",
                     help="bump workload: comma-separated large-member paths to time (grid: kernel per phase; "
                          "team: cooperative workgroup team)")

@@ -255,6 +255,15 @@ int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t it
  * cells and more; it takes precedence over team solves.  Its inner solves run
  * in FP64 whatever the member's precision. */
 int rsg_set_grid(rsg_context* ctx, int32_t min_cells, int32_t iters_per_launch);
+/* p' AMG hierarchy of kernel-per-phase members: max_levels > 1 (default 5)
+ * stops coarsening at that many levels once the coarsest level has at most
+ * 16,384 rows, and that level is inverted on the FP64 matrix cores at every
+ * refresh of the coarse values (rsg_dense_inverse's routine) and applied as
+ * one dense product per V-cycle; 0 coarsens as the reference does (at most 20
+ * levels, down to 50 rows).  Taken by members admitted afterwards.  The
+ * reference's AMGSolver (core/include/LinearSolver.hpp:508-512, 516-551)
+ * coarsens to 50 rows and sweeps the coarsest level 50 times. */
+int rsg_set_grid_levels(rsg_context* ctx, int32_t max_levels);
 
 #define RSG_SCHEDULE_ROUND_ROBIN 0
 #define RSG_SCHEDULE_LARGEST_FIRST 1

@@ -102,7 +102,7 @@ def gpu_ctx(rsg_lib):
     rsg_lib.rsg_context_destroy(ctx)
 
 
-ENGINE_DEFAULTS = dict(iters_per_launch=200, block=0, slice_us=0, schedule=0, team=(0, 0, 20))
+ENGINE_DEFAULTS = dict(iters_per_launch=200, block=0, slice_us=0, schedule=0, team=(0, 0, 20), grid=(0, 20), grid_levels=5)
 
 
 def reset_engine(lib, ctx):
@@ -112,12 +112,15 @@ def reset_engine(lib, ctx):
     lib.rsg_set_time_slice(ctx, d["slice_us"])
     lib.rsg_set_schedule(ctx, d["schedule"])
     lib.rsg_set_team(ctx, *d["team"])
+    lib.rsg_set_grid(ctx, *d["grid"])
+    lib.rsg_set_grid_levels(ctx, d["grid_levels"])
 
 
 @pytest.fixture(autouse=True)
 def _engine_defaults(request):
     """Every GPU test starts and ends with the engine at its default tuning, so
-    no test runs whatever launch shape, slice, schedule or team an earlier one
+    no test runs whatever launch shape, slice, schedule, team or kernel-per-phase
+    threshold an earlier one
     left on the session-scoped context (librsg's or rans_sst_py's)."""
     uses_ctx = "gpu_ctx" in request.fixturenames
     uses_rs = "rs" in request.fixturenames and request.node.get_closest_marker("gpu") is not None
@@ -138,6 +141,8 @@ def _reset_rs(rs):
     rs.engine_set_time_slice(d["slice_us"])
     rs.engine_set_schedule("round_robin")
     rs.engine_set_team(*d["team"])
+    rs.engine_set_grid(*d["grid"])
+    rs.engine_set_grid_levels(d["grid_levels"])
 
 
 def c0_mesh(rs, tmp_path_factory=None):
