@@ -215,37 +215,72 @@ __global__ __launch_bounds__(kBlock) void k_coarse_inverse(int n, const int* rp,
     if (threadIdx.x == 0) *ok = good;
 }
 
-// BiCGSTAB phases (the recurrence of rsg_member.h's bicgstab)
-__global__ __launch_bounds__(kBlock) void k_bicg_p(int n, double beta, double omega, const double* r,
-                                                   const double* v, const double* dg, double* p, double* ph) {
+// BiCGSTAB phases (the recurrence of rsg_member.h's bicgstab); scalars come
+// in FP64 and are rounded to T once
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int n, double beta, double omega, const T* r, const T* v,
+                                                   const T* dg, T* p, T* ph) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double pc = r[i] + beta * (p[i] - omega * v[i]);
+    const T pc = r[i] + (T)beta * (p[i] - (T)omega * v[i]);
     p[i] = pc;
-    ph[i] = jac<double>(pc, dg[i]);
+    ph[i] = jac<T>(pc, dg[i]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bicg_s(int n, double alpha, const double* r, const double* v,
-                                                   const double* dg, double* s, double* sh) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_s(int n, double alpha, const T* r, const T* v, const T* dg, T* s,
+                                                   T* sh) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double sc = r[i] - alpha * v[i];
+    const T sc = r[i] - (T)alpha * v[i];
     s[i] = sc;
-    sh[i] = jac<double>(sc, dg[i]);
+    sh[i] = jac<T>(sc, dg[i]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bicg_x(int n, double alpha, double omega, const double* ph,
-                                                   const double* sh, const double* s, const double* t, double* x,
-                                                   double* r) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_x(int n, double alpha, double omega, const T* ph, const T* sh,
+                                                   const T* s, const T* t, T* x, T* r) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    x[i] += alpha * ph[i] + omega * sh[i];
-    r[i] = s[i] - omega * t[i];
+    x[i] += (T)alpha * ph[i] + (T)omega * sh[i];
+    r[i] = s[i] - (T)omega * t[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_axpy(int n, double a, const double* p, double* x) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpy_t(int n, double a, const T* p, T* x) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) x[i] += a * p[i];
+    if (i < n) x[i] += (T)a * p[i];
+}
+
+// FP32 image of an FP64 CSR level in row_ax's form (off-diagonal values, the
+// FP64 row sum rounded once, the diagonal)
+__global__ __launch_bounds__(kBlock) void k_to_f32(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const double* __restrict__ v, const double* __restrict__ dg,
+                                                   float* __restrict__ vf, float* __restrict__ exf,
+                                                   float* __restrict__ dgf) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double rs = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        rs += v[e];
+        vf[e] = (ci[e] == i) ? 0.0f : (float)v[e];
+    }
+    exf[i] = (float)rs;
+    dgf[i] = (float)dg[i];
+}
+
+// bf = r / rn (the defect scaled to unit norm), ef = 0
+__global__ __launch_bounds__(kBlock) void k_scale_f32(int n, const double* r, double inv, float* bf, float* ef) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    bf[i] = (float)(r[i] * inv);
+    ef[i] = 0.0f;
+}
+
+// x += rn ef (FP64)
+__global__ __launch_bounds__(kBlock) void k_refine(int n, double rn, const float* ef, double* x) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] += rn * (double)ef[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(int n, const double* a, double* b) {
@@ -288,6 +323,14 @@ struct RsgBig {
     int* dOk = nullptr;
     bool haveH = false;
     int maxLevels = 0;            // 0: the reference's cap; else levels before an MFMA-inverted coarsest
+    // mixed precision (a member submitted with RSG_PREC_MIXED): FP32 images of
+    // the level-0 system and of the p' hierarchy, FP32 Krylov vectors
+    bool mixed = false;
+    double* r64 = nullptr;        // FP64 defect
+    float *v0f = nullptr, *ex0f = nullptr, *dg0f = nullptr;
+    float* fw = nullptr;          // 12 FP32 vectors of nC
+    std::unique_ptr<grid::DevAlloc> HFA;
+    grid::Hierarchy<float> Hf;
     grid::DenseWork DW;           // the coarsest level's inverse when it is above 64 rows
     bool bigDense = false;
     double denseMs = 0.0;         // device time of the inversions so far
@@ -300,9 +343,9 @@ struct RsgBig {
 
 namespace {
 
-hipError_t dot_read(RsgBig& B, int n, const double* a, const double* b, const double* c, const double* d,
-                    double* out2) {
-    grid::dot2<double>(n, a, b, c, d, B.S, B.S.dots, B.s);
+template <typename T>
+hipError_t dot_read(RsgBig& B, int n, const T* a, const T* b, const T* c, const T* d, double* out2) {
+    grid::dot2<T>(n, a, b, c, d, B.S, B.S.dots, B.s);
     hipError_t e = hipMemcpyAsync(B.S.hdots, B.S.dots, 2 * sizeof(double), hipMemcpyDeviceToHost, B.s);
     if (e == hipSuccess) e = hipStreamSynchronize(B.s);
     out2[0] = B.S.hdots[0];
@@ -311,53 +354,49 @@ hipError_t dot_read(RsgBig& B, int n, const double* a, const double* b, const do
 }
 
 // BiCGSTAB (LinearSolver.hpp:193-245, Jacobi right preconditioning) on the
-// member's current system, CSR values in B.v0 / B.dg0; x in/out.
-hipError_t bicgstab_grid(RsgBig& B, double* x, int maxIter, double tol, SolveRes& res) {
+// CSR system (level-0 pattern, values v / ex / dg in precision T), x in/out;
+// w: eight work vectors of n.
+template <typename T>
+hipError_t bicgstab_grid(RsgBig& B, const T* v, const T* ex, const T* dg, const T* b, T* x, T* const* w, int maxIter,
+                         double tol, SolveRes& res) {
     const int n = B.nC;
-    RsgMemberDev& M = B.hM;
-    double *r = M.kv[0], *rh = M.kv[1], *p = M.kv[2], *v = M.kv[3], *sv = M.kv[4], *tv = M.kv[5], *ph = M.kv[6],
-           *sh = M.kv[7];
+    T *r = w[0], *rh = w[1], *p = w[2], *vv = w[3], *sv = w[4], *tv = w[5], *ph = w[6], *sh = w[7];
     const dim3 g(grid::blocks(n)), bl(kBlock);
     res = SolveRes{0, 0.0, 0.0, 0};
-    hipLaunchKernelGGL(grid::k_residual<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
-                       (const double*)M.src, (const double*)x, r);
+    hipLaunchKernelGGL(grid::k_residual<T>, g, bl, 0, B.s, n, B.rp0, B.ci0, v, ex, b, (const T*)x, r);
     double d2[2];
-    hipError_t e = dot_read(B, n, r, r, nullptr, nullptr, d2);
+    hipError_t e = dot_read<T>(B, n, r, r, nullptr, nullptr, d2);
     if (e != hipSuccess) return e;
     const double r0 = std::sqrt(d2[0]);
     res.initialRes = r0;
     if (r0 < 1e-30) { res.converged = 1; return hipSuccess; }
-    hipLaunchKernelGGL(k_copy, g, bl, 0, B.s, n, (const double*)r, rh);
-    hipLaunchKernelGGL(grid::k_fill<double>, g, bl, 0, B.s, n, p, 0.0);
-    hipLaunchKernelGGL(grid::k_fill<double>, g, bl, 0, B.s, n, v, 0.0);
+    hipLaunchKernelGGL((grid::k_convert<T, T>), g, bl, 0, B.s, n, (const T*)r, rh);
+    hipLaunchKernelGGL(grid::k_fill<T>, g, bl, 0, B.s, n, p, T(0));
+    hipLaunchKernelGGL(grid::k_fill<T>, g, bl, 0, B.s, n, vv, T(0));
     double rho = 1.0, alpha = 1.0, omega = 1.0, rhoN = d2[0];
     for (int it = 0; it < maxIter; ++it) {
         if (std::fabs(rhoN) < 1e-30) break;
         const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
-        hipLaunchKernelGGL(k_bicg_p, g, bl, 0, B.s, n, beta, omega, (const double*)r, (const double*)v,
-                           (const double*)B.dg0, p, ph);
-        hipLaunchKernelGGL(grid::k_spmv<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
-                           (const double*)ph, v);
-        if ((e = dot_read(B, n, rh, v, nullptr, nullptr, d2)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_bicg_p<T>, g, bl, 0, B.s, n, beta, omega, (const T*)r, (const T*)vv, dg, p, ph);
+        hipLaunchKernelGGL(grid::k_spmv<T>, g, bl, 0, B.s, n, B.rp0, B.ci0, v, ex, (const T*)ph, vv);
+        if ((e = dot_read<T>(B, n, rh, vv, nullptr, nullptr, d2)) != hipSuccess) return e;
         const double rv = d2[0];
         if (std::fabs(rv) < 1e-30) break;
         alpha = rhoN / rv;
-        hipLaunchKernelGGL(k_bicg_s, g, bl, 0, B.s, n, alpha, (const double*)r, (const double*)v,
-                           (const double*)B.dg0, sv, sh);
-        if ((e = dot_read(B, n, sv, sv, nullptr, nullptr, d2)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_bicg_s<T>, g, bl, 0, B.s, n, alpha, (const T*)r, (const T*)vv, dg, sv, sh);
+        if ((e = dot_read<T>(B, n, sv, sv, nullptr, nullptr, d2)) != hipSuccess) return e;
         const double sn = std::sqrt(d2[0]);
         if (sn / r0 < tol) {
-            hipLaunchKernelGGL(k_axpy, g, bl, 0, B.s, n, alpha, (const double*)ph, x);
+            hipLaunchKernelGGL(k_axpy_t<T>, g, bl, 0, B.s, n, alpha, (const T*)ph, x);
             res.iterations = it + 1; res.finalRes = sn / r0; res.converged = 1;
             return hipGetLastError();
         }
-        hipLaunchKernelGGL(grid::k_spmv<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
-                           (const double*)sh, tv);
-        if ((e = dot_read(B, n, tv, tv, tv, sv, d2)) != hipSuccess) return e;
+        hipLaunchKernelGGL(grid::k_spmv<T>, g, bl, 0, B.s, n, B.rp0, B.ci0, v, ex, (const T*)sh, tv);
+        if ((e = dot_read<T>(B, n, tv, tv, tv, sv, d2)) != hipSuccess) return e;
         omega = (d2[0] > 1e-30) ? d2[1] / d2[0] : 0.0;
-        hipLaunchKernelGGL(k_bicg_x, g, bl, 0, B.s, n, alpha, omega, (const double*)ph, (const double*)sh,
-                           (const double*)sv, (const double*)tv, x, r);
-        if ((e = dot_read(B, n, r, r, rh, r, d2)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_bicg_x<T>, g, bl, 0, B.s, n, alpha, omega, (const T*)ph, (const T*)sh, (const T*)sv,
+                           (const T*)tv, x, r);
+        if ((e = dot_read<T>(B, n, r, r, rh, r, d2)) != hipSuccess) return e;
         const double rn = std::sqrt(d2[0]);
         res.iterations = it + 1;
         res.finalRes = rn / r0;
@@ -462,8 +501,44 @@ hipError_t build_amg(RsgBig& B) {
             B.HA->err = hipMemcpyAsync(B.gI[l], gIh[l].data(), 4 * gIh[l].size(), hipMemcpyHostToDevice, B.s);
     }
     if (B.HA->err == hipSuccess) B.HA->err = hipStreamSynchronize(B.s);
+    if (B.HA->err == hipSuccess && B.mixed) {
+        // FP32 mirror of the hierarchy: the same patterns, FP32 values in
+        // row_ax's form and FP32 level vectors (filled by refresh_amg)
+        B.HFA.reset(new grid::DevAlloc());
+        grid::DevAlloc& F = *B.HFA;
+        B.Hf = grid::Hierarchy<float>();
+        B.Hf.lv.resize(B.H.lv.size());
+        for (size_t l = 0; l < B.H.lv.size(); ++l) {
+            const grid::DevLevel<double>& d = B.H.lv[l];
+            grid::DevLevel<float>& f = B.Hf.lv[l];
+            f.n = d.n; f.nnz = d.nnz; f.rp = d.rp; f.ci = d.ci; f.map = d.map; f.aP = d.aP; f.aI = d.aI;
+            f.v = F.take<float>(d.nnz); f.ex = F.take<float>(d.n); f.dg = F.take<float>(d.n);
+            f.x = F.take<float>(d.n); f.b = F.take<float>(d.n); f.r = F.take<float>(d.n);
+        }
+        if (B.H.dense) { B.Hf.cinv = F.take<float>(RSG_COARSE_DENSE * RSG_COARSE_DENSE); B.Hf.dense = true; }
+        if (F.err != hipSuccess) return F.err;
+    }
     B.haveH = B.HA->err == hipSuccess;
     return B.HA->err;
+}
+
+// FP32 mirror of hierarchy levels [l0, l1) and of the coarsest inverse
+void mirror_f32(RsgBig& B, int l0, int l1) {
+    for (int l = l0; l < l1; ++l) {
+        const grid::DevLevel<double>& d = B.H.lv[l];
+        grid::DevLevel<float>& f = B.Hf.lv[l];
+        hipLaunchKernelGGL(k_to_f32, dim3(grid::blocks(d.n)), dim3(kBlock), 0, B.s, d.n, (const int*)d.rp,
+                           (const int*)d.ci, (const double*)d.v, (const double*)d.dg, f.v, f.ex, f.dg);
+    }
+    if (l1 == (int)B.H.lv.size()) {
+        B.Hf.dense = B.H.dense;
+        if (B.H.dense)
+            hipLaunchKernelGGL((grid::k_convert<float, double>), dim3(grid::blocks(RSG_COARSE_DENSE * RSG_COARSE_DENSE)),
+                               dim3(kBlock), 0, B.s, RSG_COARSE_DENSE * RSG_COARSE_DENSE, (const double*)B.H.cinv,
+                               B.Hf.cinv);
+        B.Hf.dinv = B.H.dinv;          // the large coarsest inverse stays FP64
+        B.Hf.dinvN = B.H.dinvN;
+    }
 }
 
 // level-0 values of the hierarchy from the current p' matrix; with `coarse`
@@ -472,7 +547,10 @@ hipError_t refresh_amg(RsgBig& B, bool coarse) {
     grid::Hierarchy<double>& H = B.H;
     hipLaunchKernelGGL(k_csr_fill, dim3(cells_grid(std::max(B.nC, B.nnz0))), dim3(kBlock), 0, B.s, B.dM,
                        H.lv[0].v, H.lv[0].dg);
-    if (!coarse) return hipGetLastError();
+    if (!coarse) {
+        if (B.mixed) mirror_f32(B, 0, 1);
+        return hipGetLastError();
+    }
     const int L = (int)H.lv.size();
     for (int l = 1; l < L; ++l) {
         grid::DevLevel<double>& c = H.lv[l];
@@ -501,18 +579,90 @@ hipError_t refresh_amg(RsgBig& B, bool coarse) {
         H.dinv = ok ? B.DW.M : nullptr;     // a pivot not positive: smoothed instead
         H.dinvN = B.DW.N;
     }
+    if (B.mixed) mirror_f32(B, 0, L);
     return hipGetLastError();
 }
 
-// One linear solve of the member's current system into x (device).
-hipError_t solve(RsgBig& B, int code, double* x, bool amgReady, SolveRes& res) {
+// FP32 inner solve from zero of the level-0 system (FP32 image) or, for
+// p' with the AMG ready, of the FP32 hierarchy: A ef = bf.
+hipError_t inner_f32(RsgBig& B, int code, bool amg, const float* bf, float* ef, int maxIter, double tol,
+                     SolveRes& res) {
+    const int n = B.nC;
+    float* w[8];
+    for (int j = 0; j < 8; ++j) w[j] = B.fw + (size_t)(j + 2) * n;
+    if (code == 2) return bicgstab_grid<float>(B, B.v0f, B.ex0f, B.dg0f, bf, ef, w, maxIter, tol, res);
+    grid::Hierarchy<float> J;
+    if (!amg) {
+        J.lv.resize(1);
+        grid::DevLevel<float>& d = J.lv[0];
+        d.n = n; d.nnz = B.nnz0;
+        d.rp = const_cast<int*>(B.rp0); d.ci = const_cast<int*>(B.ci0); d.v = B.v0f; d.ex = B.ex0f; d.dg = B.dg0f;
+    }
+    grid::PcgResult pr;
+    hipError_t e = grid::pcg<float>(amg ? B.Hf : J, bf, ef, w[0], w[1], w[2], w[3], maxIter, tol, amg, B.S, B.s, pr);
+    res = SolveRes{pr.iterations, pr.finalRes, pr.initialRes, pr.converged};
+    return e;
+}
+
+// Mixed-precision solve (rsg_member.h's lsolve_mixed_global): FP64 defect of
+// the FP64 system, FP32 inner solve of the defect scaled to unit norm, FP64
+// update; at most RSG_MIXED_SWEEPS sweeps and maxIter inner iterations in all.
+hipError_t solve_mixed(RsgBig& B, int code, double* x, bool amg, int maxIter, double tol, SolveRes& res) {
+    const int n = B.nC;
+    const dim3 g(grid::blocks(n)), bl(kBlock);
+    const double* v = amg ? B.H.lv[0].v : B.v0;
+    float* bf = B.fw;
+    float* ef = B.fw + n;
+    if (!amg)
+        hipLaunchKernelGGL(k_to_f32, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0, (const double*)B.dg0,
+                           B.v0f, B.ex0f, B.dg0f);
+    res = SolveRes{0, 0.0, 0.0, 0};
+    double d2[2];
+    auto defect = [&](double& rn) {
+        hipLaunchKernelGGL(grid::k_residual<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, v, (const double*)nullptr,
+                           (const double*)B.hM.src, (const double*)x, B.r64);
+        hipError_t e = dot_read<double>(B, n, B.r64, B.r64, nullptr, nullptr, d2);
+        rn = std::sqrt(d2[0]);
+        return e;
+    };
+    double rn = 0.0;
+    hipError_t e = defect(rn);
+    if (e != hipSuccess) return e;
+    const double r0 = rn;
+    res.initialRes = r0;
+    if (r0 < 1e-30) { res.converged = 1; return hipSuccess; }
+    for (int sweep = 0; sweep < RSG_MIXED_SWEEPS && res.iterations < maxIter; ++sweep) {
+        hipLaunchKernelGGL(k_scale_f32, g, bl, 0, B.s, n, (const double*)B.r64, 1.0 / rn, bf, ef);
+        SolveRes in{0, 0.0, 0.0, 0};
+        e = inner_f32(B, code, amg, bf, ef, maxIter - res.iterations, tol * r0 / rn, in);
+        if (e != hipSuccess) return e;
+        res.iterations += in.iterations;
+        hipLaunchKernelGGL(k_refine, g, bl, 0, B.s, n, rn, (const float*)ef, x);
+        double rnNew = 0.0;
+        if ((e = defect(rnNew)) != hipSuccess) return e;
+        res.finalRes = rnNew / r0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+        if (!(rnNew < rn) || in.iterations == 0) break;   // the FP32 solve stalls: stop refining
+        rn = rnNew;
+    }
+    return hipGetLastError();
+}
+
+// One linear solve of the member's current system into x (device); a mixed
+// member's solves run solve_mixed unless allowMixed is false (omega).
+hipError_t solve(RsgBig& B, int code, double* x, bool amgReady, SolveRes& res, bool allowMixed = true) {
     const RsgSettingsDev& st = B.set;
     const int n = B.nC;
     RsgMemberDev& M = B.hM;
     hipLaunchKernelGGL(k_csr_fill, dim3(cells_grid(std::max(n, B.nnz0))), dim3(kBlock), 0, B.s, B.dM, B.v0, B.dg0);
-    if (code == 2) return bicgstab_grid(B, x, st.innerIter, st.innerTol, res);
-    grid::Hierarchy<double> J;                    // Jacobi: level 0 alone
     const bool amg = code == 0 && amgReady;
+    if (B.mixed && allowMixed) return solve_mixed(B, code, x, amg, st.innerIter, st.innerTol, res);
+    if (code == 2) {
+        double* w[8];
+        for (int j = 0; j < 8; ++j) w[j] = M.kv[j];
+        return bicgstab_grid<double>(B, B.v0, nullptr, B.dg0, M.src, x, w, st.innerIter, st.innerTol, res);
+    }
+    grid::Hierarchy<double> J;                    // Jacobi: level 0 alone
     if (!amg) {
         J.lv.resize(1);
         grid::DevLevel<double>& d = J.lv[0];
@@ -590,7 +740,7 @@ hipError_t outer_iteration(RsgBig& B) {
         hipLaunchKernelGGL(k_clamp, gc, bl, 0, B.s, B.dM, 0);
         hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, B.dM, 4, 0.0);
         hipLaunchKernelGGL(k_asm_turb, gc, bl, 0, B.s, B.dM, 1);
-        if (e == hipSuccess) e = solve(B, st.tSolver, M.om, false, rOm);
+        if (e == hipSuccess) e = solve(B, st.tSolver, M.om, false, rOm, false);   // omega stays FP64
         hipLaunchKernelGGL(k_clamp, gc, bl, 0, B.s, B.dM, 1);
         hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, B.dM, 8, M.coeffs[2]);
         hipLaunchKernelGGL(k_pin, gc, bl, 0, B.s, B.dM, B.S.part);
@@ -657,6 +807,12 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
     B->S.part = B->A.take<double>(4 * (size_t)nb + 4);
     B->S.dots = B->A.take<double>(8);
     B->dOk = B->A.take<int>(1);
+    B->mixed = hM.prec != 0;
+    if (B->mixed) {
+        B->r64 = B->A.take<double>(pm.nC);
+        B->v0f = B->A.take<float>(pm.nnz0); B->ex0f = B->A.take<float>(pm.nC); B->dg0f = B->A.take<float>(pm.nC);
+        B->fw = B->A.take<float>(10 * (size_t)pm.nC);
+    }
     if (B->A.err == hipSuccess) B->A.err = hipHostMalloc((void**)&B->S.hdots, 8 * sizeof(double), hipHostMallocDefault);
     if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(dM, &hM, sizeof hM, hipMemcpyHostToDevice, s);
     if (B->A.err != hipSuccess) { err = std::string("big member: ") + hipGetErrorString(B->A.err); return nullptr; }

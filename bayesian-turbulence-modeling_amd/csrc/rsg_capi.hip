@@ -1224,7 +1224,7 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
             double r0 = -1.0;
             int inner = 0;
             for (int k = 0; k < 30 && A.err == hipSuccess; ++k) {
-                hipLaunchKernelGGL(k_residual<double>, gr, bl, 0, s, n, A0.rp, A0.ci, A0.v, (const double*)b64,
+                hipLaunchKernelGGL(k_residual<double>, gr, bl, 0, s, n, A0.rp, A0.ci, A0.v, (const double*)nullptr, (const double*)b64,
                                    (const double*)x64, r64);
                 dot2<double>(n, r64, r64, (const double*)nullptr, (const double*)nullptr, S, S.dots + 2, s);
                 A.err = hipMemcpyAsync(S.hdots, S.dots, 4 * sizeof(double), hipMemcpyDeviceToHost, s);

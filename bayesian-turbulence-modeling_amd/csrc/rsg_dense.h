@@ -257,19 +257,50 @@ __global__ __launch_bounds__(kThreads) void k_gj_mirror(int N, double* __restric
 }
 
 // ------------------------------------------------------------- apply
-// x = Ainv b on the first n rows, one wave per row, FP64 sums in a fixed
-// order (lane-strided partial sums, then an xor butterfly).
+// x = Ainv b on the first n rows (Ainv's padding columns are zero for those
+// rows).  One wave per pair of rows: every lane reads two adjacent columns of
+// both rows per 128-column chunk (16-byte loads, two chunks in flight), then
+// the lanes' partial sums are combined by an xor butterfly -- a fixed order,
+// so the product is the same on every call.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_dense_apply(int n, int N, const double* __restrict__ Ainv,
                                                           const T* __restrict__ b, T* __restrict__ x) {
-    const int row = blockIdx.x * (kThreads / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (row >= n) return;
-    const double* a = Ainv + (size_t)row * N;
-    double s = 0.0;
-    for (int j = lane; j < n; j += 64) s += a[j] * (double)b[j];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) x[row] = (T)s;
+    const int wave = blockIdx.x * (kThreads / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int r0 = 2 * wave;
+    if (r0 >= n) return;
+    const int r1 = (r0 + 1 < n) ? r0 + 1 : r0;
+    const double2* a0 = reinterpret_cast<const double2*>(Ainv + (size_t)r0 * N);
+    const double2* a1 = reinterpret_cast<const double2*>(Ainv + (size_t)r1 * N);
+    double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
+    int c = 0;
+    for (; c + 256 <= n; c += 256) {
+        const int j = c + 2 * lane, k = j + 128;
+        const double bj0 = (double)b[j], bj1 = (double)b[j + 1], bk0 = (double)b[k], bk1 = (double)b[k + 1];
+        const double2 p = a0[j / 2], q = a1[j / 2], u = a0[k / 2], v = a1[k / 2];
+        s0 += p.x * bj0 + p.y * bj1;
+        s1 += q.x * bj0 + q.y * bj1;
+        t0 += u.x * bk0 + u.y * bk1;
+        t1 += v.x * bk0 + v.y * bk1;
+    }
+    for (int j = c + lane; j < n; j += 64) {
+        const double bj = (double)b[j];
+        s0 += Ainv[(size_t)r0 * N + j] * bj;
+        s1 += Ainv[(size_t)r1 * N + j] * bj;
+    }
+    s0 += t0;
+    s1 += t1;
+    for (int off = 32; off >= 1; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+    }
+    if (lane == 0) {
+        x[r0] = (T)s0;
+        if (r1 != r0) x[r1] = (T)s1;
+    }
 }
+
+// grid size of k_dense_apply for n rows
+inline int apply_blocks(int n) { return (n + 2 * (kThreads / 64) - 1) / (2 * (kThreads / 64)); }
 
 __global__ void k_flag_clear(int* flag) { flag[0] = 0; }
 

@@ -28,6 +28,7 @@
 // every iteration runs on the device.  Included by rsg_capi.hip.
 #pragma once
 #include <chrono>
+#include <type_traits>
 #include <vector>
 
 #include "rsg_dense.h"
@@ -38,42 +39,64 @@ namespace grid {
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ kernels
+// Row products.  ex == null: y_i = sum_e v[e] x[ci[e]] (FP64 levels).  With
+// ex (FP32 levels): y_i = ex_i x_i + sum_e v[e] (x[ci[e]] - x_i), where ex_i
+// is the FP64 row sum rounded once and the diagonal entry's v is 0 -- the
+// constants (the near-null space of a pinned or periodic pressure Laplacian)
+// stay in the near-null space after the coefficients are rounded to FP32.
+template <typename T>
+__device__ __forceinline__ T row_ax(int i, const int* __restrict__ rp, const int* __restrict__ ci,
+                                    const T* __restrict__ v, const T* __restrict__ ex, const T* x) {
+    T s = 0;
+    if (ex) {
+        const T xi = x[i];
+        s = ex[i] * xi;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * (x[ci[e]] - xi);
+    } else {
+        for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+    }
+    return s;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                  const T* __restrict__ v, const T* __restrict__ x,
-                                                  T* __restrict__ y) {
+                                                  const T* __restrict__ v, const T* __restrict__ ex,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    T s = 0;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
-    y[i] = s;
+    y[i] = row_ax<T>(i, rp, ci, v, ex, x);
 }
 
 // r = b - A x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                      const T* __restrict__ v, const T* __restrict__ b,
-                                                      const T* __restrict__ x, T* __restrict__ r) {
+                                                      const T* __restrict__ v, const T* __restrict__ ex,
+                                                      const T* __restrict__ b, const T* __restrict__ x,
+                                                      T* __restrict__ r) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    T s = 0;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
-    r[i] = b[i] - s;
+    r[i] = b[i] - row_ax<T>(i, rp, ci, v, ex, x);
 }
 
 // xo = xi + w (b - A xi) / d  (xi == null: from zero)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                    const T* __restrict__ v, const T* __restrict__ dg,
-                                                    const T* __restrict__ b, const T* xi, T* xo) {
+                                                    const T* __restrict__ v, const T* __restrict__ ex,
+                                                    const T* __restrict__ dg, const T* __restrict__ b, const T* xi,
+                                                    T* xo) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const T wJ = T(2) / T(3);
     T s = b[i];
     T x0 = 0;
     if (xi) {
-        for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xi[ci[e]];
         x0 = xi[i];
+        if (ex) {
+            s -= ex[i] * x0;
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * (xi[ci[e]] - x0);
+        } else {
+            for (int e = rp[i]; e < rp[i + 1]; ++e) s -= v[e] * xi[ci[e]];
+        }
     }
     const T d = dg[i];
     xo[i] = (fabs((double)d) > 1e-30) ? x0 + wJ * s / d : x0;
@@ -391,6 +414,7 @@ template <typename T> struct DevLevel {
     int n = 0, nnz = 0;
     int *rp = nullptr, *ci = nullptr, *map = nullptr, *aP = nullptr, *aI = nullptr;
     T *v = nullptr, *dg = nullptr;
+    T* ex = nullptr;       // FP32 levels: row sums (row_ax's form); null on FP64 levels
     T *x = nullptr, *b = nullptr, *r = nullptr;
 };
 
@@ -435,7 +459,20 @@ hipError_t upload(const std::vector<Level>& H, const std::vector<double>& G, boo
         d.v = A.take<T>(L.nnz); d.dg = A.take<T>(L.n);
         d.x = A.take<T>(L.n); d.b = A.take<T>(L.n); d.r = A.take<T>(L.n);
         put(d.rp, L.rp.data(), 4 * L.rp.size()); put(d.ci, L.ci.data(), 4 * L.ci.size());
-        putT(d.v, L.v); putT(d.dg, L.dg);
+        if (std::is_same<T, float>::value) {
+            // FP32 level in row_ax's form: off-diagonal values, FP64 row sums
+            std::vector<double> off(L.v), rs(L.n, 0.0);
+            for (int i = 0; i < L.n; ++i)
+                for (int e = L.rp[i]; e < L.rp[i + 1]; ++e) {
+                    rs[i] += L.v[e];
+                    if (L.ci[e] == i) off[e] = 0.0;
+                }
+            d.ex = A.take<T>(L.n);
+            putT(d.v, off); putT(d.ex, rs);
+        } else {
+            putT(d.v, L.v);
+        }
+        putT(d.dg, L.dg);
         if (!L.map.empty()) { d.map = A.take<int>(L.n); put(d.map, L.map.data(), 4 * L.map.size()); }
         if (!L.aP.empty()) {
             d.aP = A.take<int>(L.n + 1); d.aI = A.take<int>(L.aI.size());
@@ -526,11 +563,11 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
         const DevLevel<T>& d = D.lv[0];
         if (D.dense) { hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, d.n, D.cinv, rin, z); return; }
         if (D.dinv) {
-            hipLaunchKernelGGL(dense::k_dense_apply<T>, dim3((d.n + 3) / 4), dim3(dense::kThreads), 0, s, d.n,
+            hipLaunchKernelGGL(dense::k_dense_apply<T>, dim3(dense::apply_blocks(d.n)), dim3(dense::kThreads), 0, s, d.n,
                                D.dinvN, D.dinv, rin, z);
             return;
         }
-        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), dim3(kBlock), 0, s, d.n, d.rp, d.ci, d.v, d.dg, rin,
+        hipLaunchKernelGGL(k_jacobi<T>, dim3(blocks(d.n)), dim3(kBlock), 0, s, d.n, d.rp, d.ci, d.v, d.ex, d.dg, rin,
                            (const T*)nullptr, z);
         return;
     }
@@ -538,9 +575,9 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
         const DevLevel<T>& d = D.lv[l];
         const T* b = (l == 0) ? rin : d.b;
         const dim3 g(blocks(d.n)), bl(kBlock);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)nullptr, d.r);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.r, d.x);
-        hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, b, (const T*)d.x, d.r);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.ex, d.dg, b, (const T*)nullptr, d.r);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.ex, d.dg, b, (const T*)d.r, d.x);
+        hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.ex, b, (const T*)d.x, d.r);
         const DevLevel<T>& c = D.lv[l + 1];
         hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(c.n)), bl, 0, s, c.n, c.aP, c.aI, (const T*)d.r, c.b);
     }
@@ -549,14 +586,14 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
         if (D.dense) {
             hipLaunchKernelGGL(k_dense<T>, dim3(1), dim3(kBlock), 0, s, c.n, D.cinv, (const T*)c.b, c.x);
         } else if (D.dinv) {
-            hipLaunchKernelGGL(dense::k_dense_apply<T>, dim3((c.n + 3) / 4), dim3(dense::kThreads), 0, s, c.n,
+            hipLaunchKernelGGL(dense::k_dense_apply<T>, dim3(dense::apply_blocks(c.n)), dim3(dense::kThreads), 0, s, c.n,
                                D.dinvN, D.dinv, (const T*)c.b, c.x);
         } else {
             const dim3 g(blocks(c.n)), bl(kBlock);
-            hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)nullptr, c.x);
+            hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.ex, c.dg, (const T*)c.b, (const T*)nullptr, c.x);
             for (int q = 0; q < 25; ++q) {
-                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.x, c.r);
-                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.dg, (const T*)c.b, (const T*)c.r, c.x);
+                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.ex, c.dg, (const T*)c.b, (const T*)c.x, c.r);
+                hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, c.n, c.rp, c.ci, c.v, c.ex, c.dg, (const T*)c.b, (const T*)c.r, c.x);
             }
         }
     }
@@ -565,8 +602,8 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
         const T* b = (l == 0) ? rin : d.b;
         const dim3 g(blocks(d.n)), bl(kBlock);
         hipLaunchKernelGGL(k_prolong<T>, g, bl, 0, s, d.n, d.map, (const T*)D.lv[l + 1].x, d.x);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.x, d.r);
-        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.dg, b, (const T*)d.r, l == 0 ? z : d.x);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.ex, d.dg, b, (const T*)d.x, d.r);
+        hipLaunchKernelGGL(k_jacobi<T>, g, bl, 0, s, d.n, d.rp, d.ci, d.v, d.ex, d.dg, b, (const T*)d.r, l == 0 ? z : d.x);
     }
 }
 
@@ -595,7 +632,7 @@ hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, 
     const int n = A0.n;
     const dim3 g(blocks(n)), bl(kBlock);
     double* dv = S.dots;          // [0] rz, [1] pAp, [2] rr, [3] rzNew
-    hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, n, A0.rp, A0.ci, A0.v, b, (const T*)x, r);
+    hipLaunchKernelGGL(k_residual<T>, g, bl, 0, s, n, A0.rp, A0.ci, A0.v, A0.ex, b, (const T*)x, r);
     dot2<T>(n, r, r, (const T*)nullptr, (const T*)nullptr, S, dv + 2, s);
     hipError_t e = hipMemcpyAsync(S.hdots, dv, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -608,7 +645,7 @@ hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, 
     hipLaunchKernelGGL((k_convert<T, T>), g, bl, 0, s, n, (const T*)z, p);
     dot2<T>(n, r, z, (const T*)nullptr, (const T*)nullptr, S, dv + 0, s);          // rz
     for (int it = 0; it < maxIter; ++it) {
-        hipLaunchKernelGGL(k_spmv<T>, g, bl, 0, s, n, A0.rp, A0.ci, A0.v, (const T*)p, q);
+        hipLaunchKernelGGL(k_spmv<T>, g, bl, 0, s, n, A0.rp, A0.ci, A0.v, A0.ex, (const T*)p, q);
         dot2<T>(n, p, q, (const T*)nullptr, (const T*)nullptr, S, dv + 1, s);      // pAp
         hipLaunchKernelGGL(k_pcg_update<T>, g, bl, 0, s, n, (const double*)dv, x, r, (const T*)p, (const T*)q);
         dot2<T>(n, r, r, (const T*)nullptr, (const T*)nullptr, S, dv + 2, s);      // rr

@@ -252,8 +252,11 @@ int rsg_set_team(rsg_context* ctx, int32_t min_cells, int32_t blocks, int32_t it
  * aggregated on the host every amg_refresh outer iterations, its coarse
  * Galerkin values refreshed on the device every coarse_refresh), at most
  * iters_per_launch outer iterations per engine round.  For meshes of 10^5
- * cells and more; it takes precedence over team solves.  Its inner solves run
- * in FP64 whatever the member's precision. */
+ * cells and more; it takes precedence over team solves.  A member submitted
+ * with RSG_PREC_MIXED runs its momentum, p' and k solves as FP32 inner solves
+ * (FP32 CSR images, FP32 hierarchy, FP64-accumulated dot products) under FP64
+ * defects and refinement; its omega solve and the coarsest-level inverse stay
+ * FP64. */
 int rsg_set_grid(rsg_context* ctx, int32_t min_cells, int32_t iters_per_launch);
 /* p' AMG hierarchy of kernel-per-phase members: max_levels > 1 (default 5)
  * stops coarsening at that many levels once the coarsest level has at most

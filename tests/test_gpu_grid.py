@@ -114,3 +114,40 @@ def test_grid_drop_in_matches_member(rs):
         assert rel(a, b) < 1e-3, (k, rel(a, b))
     assert len(out["grid"][3]) == 1
     assert out["grid"][3][0] == pytest.approx(out["member"][3][0], rel=1e-3)
+
+
+@pytest.mark.parametrize("levels", [5, 0])
+def test_grid_mixed_tracks_fp64(rsg_lib, gpu_ctx, meshes, levels):
+    """RSG_PREC_MIXED on the kernel-per-phase path (FP32 BiCGSTAB / AMG-PCG
+    inner solves in row-sum form under FP64 defects and refinement, omega in
+    FP64) against the FP64 grid solve of the same bump member, both with the
+    capped (MFMA-inverted coarsest) and the reference's deep p' hierarchy:
+    the transients agree to the inner solves' tolerance."""
+    s = settings_for("bump", max_iterations=200, convergence_tol=1e-12, pressure_solver=0, amg_refresh=50,
+                     coarse_refresh=10)
+    chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_grid_levels(gpu_ctx, levels))
+    chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_grid(gpu_ctx, 1, 50))
+    P = Problem(rsg_lib, gpu_ctx, meshes["bump"], "bump", s)
+    try:
+        print("grid mixed vs fp64, bump, levels %d" % levels, flush=True)
+        t64 = P.submit(prec=0)
+        t32 = P.submit(prec=1)
+        r64, a64 = P.fetch(t64)
+        r32, a32 = P.fetch(t32)
+    finally:
+        P.close()
+        rsg_lib.rsg_set_grid(gpu_ctx, 0, 20)
+        rsg_lib.rsg_set_grid_levels(gpu_ctx, 5)
+    assert r64.final_iter == r32.final_iter == 200
+    errs = {k: rel(a32[k], a64[k]) for k in ("U", "k", "omega", "nuT")}
+    print("grid mixed vs fp64 (levels %d): %s" % (levels, " ".join("%s %.1e" % kv for kv in errs.items())))
+    for k, e in errs.items():
+        assert np.all(np.isfinite(a32[k]))
+        assert e < 1e-3, (k, e)
+
+
+def test_grid_levels_arguments(rsg_lib, gpu_ctx):
+    assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 1) == -1
+    assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 21) == -1
+    assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 0) == 0
+    assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 5) == 0
