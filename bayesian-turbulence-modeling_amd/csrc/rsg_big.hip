@@ -288,6 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_copy(int n, const double* a, double*
     if (i < n) b[i] = a[i];
 }
 
+// one Gauss-Seidel sweep of the member's LDU system, one thread (sequential)
+__global__ void k_gs_sweep(RsgMemberDev* M, double* x) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) gs_sweep(M->mesh, M, x);
+}
+
 __global__ void k_observe(RsgMemberDev* M) {
     if (blockIdx.x == 0 && threadIdx.x == 0) observe_member(M);
 }
@@ -583,6 +588,31 @@ hipError_t refresh_amg(RsgBig& B, bool coarse) {
     return hipGetLastError();
 }
 
+// Gauss-Seidel (rsg_member.h's gauss_seidel): sequential sweeps on one
+// thread, residual and norms grid-wide.
+hipError_t gs_grid(RsgBig& B, double* x, int maxIter, double tol, SolveRes& res) {
+    const int n = B.nC;
+    const dim3 g(grid::blocks(n)), bl(kBlock);
+    double* r = B.hM.kv[0];
+    double d2[2];
+    res = SolveRes{0, 0.0, 0.0, 0};
+    hipError_t e = dot_read<double>(B, n, B.hM.src, B.hM.src, nullptr, nullptr, d2);
+    if (e != hipSuccess) return e;
+    double b0 = std::sqrt(d2[0]);
+    res.initialRes = b0;
+    if (b0 < 1e-30) b0 = 1.0;
+    for (int it = 0; it < maxIter; ++it) {
+        hipLaunchKernelGGL(k_gs_sweep, dim3(1), dim3(1), 0, B.s, B.dM, x);
+        hipLaunchKernelGGL(grid::k_residual<double>, g, bl, 0, B.s, n, B.rp0, B.ci0, (const double*)B.v0,
+                           (const double*)nullptr, (const double*)B.hM.src, (const double*)x, r);
+        if ((e = dot_read<double>(B, n, r, r, nullptr, nullptr, d2)) != hipSuccess) return e;
+        res.iterations = it + 1;
+        res.finalRes = std::sqrt(d2[0]) / b0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+    }
+    return hipGetLastError();
+}
+
 // FP32 inner solve from zero of the level-0 system (FP32 image) or, for
 // p' with the AMG ready, of the FP32 hierarchy: A ef = bf.
 hipError_t inner_f32(RsgBig& B, int code, bool amg, const float* bf, float* ef, int maxIter, double tol,
@@ -656,6 +686,7 @@ hipError_t solve(RsgBig& B, int code, double* x, bool amgReady, SolveRes& res, b
     RsgMemberDev& M = B.hM;
     hipLaunchKernelGGL(k_csr_fill, dim3(cells_grid(std::max(n, B.nnz0))), dim3(kBlock), 0, B.s, B.dM, B.v0, B.dg0);
     const bool amg = code == 0 && amgReady;
+    if (code == RSG_SOLVER_GAUSS_SEIDEL) return gs_grid(B, x, st.innerIter, st.innerTol, res);
     if (B.mixed && allowMixed) return solve_mixed(B, code, x, amg, st.innerIter, st.innerTol, res);
     if (code == 2) {
         double* w[8];

@@ -1042,7 +1042,8 @@ int ldu_prepare(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* own, co
         return RSG_ERR_ARG;
     if (precision != RSG_PREC_FP64 && precision != RSG_PREC_MIXED)
         return set_err(ctx, RSG_ERR_ARG, "unknown precision code");
-    if (solver != RSG_SOLVER_AMG && solver != RSG_SOLVER_PCG && solver != RSG_SOLVER_BICGSTAB)
+    if (solver != RSG_SOLVER_AMG && solver != RSG_SOLVER_PCG && solver != RSG_SOLVER_BICGSTAB &&
+        solver != RSG_SOLVER_GAUSS_SEIDEL)
         return set_err(ctx, RSG_ERR_ARG, "unknown solver code");
     for (int f = 0; f < nif; ++f)
         if (own[f] < 0 || own[f] >= n || nbr[f] < 0 || nbr[f] >= n) return set_err(ctx, RSG_ERR_ARG, "face index out of range");

@@ -30,7 +30,9 @@
 //   transposeStressSource          core/include/StressOperators.hpp:31-84
 //   apply*BC                       core/include/BoundaryCondition.hpp:196-300
 //   BiCGSTAB / PCG / AMG-PCG       core/include/LinearSolver.hpp:147-247, :497-656
+//   Gauss-Seidel                   core/include/LinearSolver.hpp:258-299
 #pragma once
+#include "rsg.h"
 #include "rsg_layout.h"
 
 #ifdef RSG_EMULATE
@@ -633,6 +635,53 @@ RSG_D double ldu_defect(Ctx& C, const double* x, double* r) {
     return acc[0];
 }
 
+// One in-place Gauss-Seidel sweep of the member's LDU system in cell order,
+// run by one thread (each unknown takes the latest neighbour values:
+// GaussSeidelSolver::solve, LinearSolver.hpp:281-288).  A cell's internal
+// faces are visited in ascending face order -- the order in which the
+// reference builds the cell's adjacency -- so the subtractions round alike.
+RSG_D void gs_sweep(const RsgMeshDev* m, const RsgMemberDev* M, double* x) {
+    const int nC = m->nC, nIF = m->nIF;
+    for (int c = 0; c < nC; ++c) {
+        const double d = M->diag[c];
+        if (fabs(d) <= 1e-30) continue;
+        double s = M->src[c];
+        for (int e = m->cfPtr[c]; e < m->cfPtr[c + 1]; ++e) {
+            const int code = m->cfIdx[e];
+            const int f = code >> 1;
+            if (f >= nIF) break;   // boundary faces sort after every internal face
+            if (code & 1) s -= M->lower[f] * x[m->own[f]];
+            else          s -= M->upper[f] * x[m->nbr[f]];
+        }
+        x[c] = s / d;
+    }
+}
+
+// Gauss-Seidel solver (GaussSeidelSolver, LinearSolver.hpp:258-299): initialRes
+// = ||b||, sweeps until ||b - A x|| / ||b|| < tol (||b|| < 1e-30 counts as 1).
+// The sweep is a sequential recurrence (thread 0); the residual and its norm
+// use the whole workgroup.  FP64 whatever the member's precision.
+RSG_D SolveRes gauss_seidel(Ctx& C, double* x, int maxIter, double tol) {
+    Thr& t = C.t; RsgMemberDev* M = C.M; const int nC = C.nC;
+    double* r = M->kv[0];
+    SolveRes res = {0, 0.0, 0.0, 0};
+    double acc[1] = {0.0};
+    FOR_CELLS(c) acc[0] += M->src[c] * M->src[c];
+    bsum(t, acc);
+    double b0 = sqrt(acc[0]);
+    res.initialRes = b0;
+    if (b0 < 1e-30) b0 = 1.0;
+    for (int it = 0; it < maxIter; ++it) {
+        if (t.tid == 0) gs_sweep(C.m, M, x);
+        RSG_SYNC();
+        const double rr = ldu_defect(C, x, r);
+        res.iterations = it + 1;
+        res.finalRes = sqrt(rr) / b0;
+        if (res.finalRes < tol) { res.converged = 1; break; }
+    }
+    return res;
+}
+
 // BiCGSTAB on the member's system from x (FP64).
 RSG_D SolveRes bicgstab(Ctx& C, double* x, int maxIter, double tol) {
     double* r = C.M->kv[0];
@@ -1176,6 +1225,7 @@ namespace rsg {
 template <int CMAX, int PREC = -1>
 RSG_D SolveRes lsolve(Ctx& C, int code, double* x, bool amgReady, bool allowMixed = true) {
     const RsgSettingsDev* s = C.s;
+    if (code == RSG_SOLVER_GAUSS_SEIDEL) return gauss_seidel(C, x, s->innerIter, s->innerTol);
     const bool amg = code == 0 && amgReady;
     if (res_fits(C, CMAX) && (!amg || res_amg_fits(C)))
         return lsolve_res<CMAX, PREC>(C, code, x, amg, s->innerIter, s->innerTol, allowMixed);
@@ -1869,7 +1919,7 @@ RSG_D void run_member(Thr t, RsgMemberDev* M) {
 
 // One linear solve (ILinearSolver::solve, LinearSolver.hpp:115-120) on the
 // LDU system already stored in M (diag/upper/lower/src), x = M->pp in/out.
-// solver: RSG_SOLVER_* codes (0 AMG-PCG, 1 Jacobi-PCG, 2 BiCGSTAB).
+// solver: RSG_SOLVER_* codes (0 AMG-PCG, 1 Jacobi-PCG, 2 BiCGSTAB, 3 Gauss-Seidel).
 template <int CMAX>
 RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter, double tol, int* nLevels) {
     Ctx C;
@@ -1890,7 +1940,8 @@ RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter,
         amg = true;
     }
     SolveRes r;
-    if (res_fits(C, CMAX) && (!amg || res_amg_fits(C))) r = lsolve_res<CMAX>(C, solver, M->pp, amg, maxIter, tol);
+    if (solver == RSG_SOLVER_GAUSS_SEIDEL) r = gauss_seidel(C, M->pp, maxIter, tol);
+    else if (res_fits(C, CMAX) && (!amg || res_amg_fits(C))) r = lsolve_res<CMAX>(C, solver, M->pp, amg, maxIter, tol);
     else if (is_mixed<-1>(C)) r = lsolve_mixed_global(C, solver, M->pp, amg, maxIter, tol);
     else if (solver == 2) r = bicgstab(C, M->pp, maxIter, tol);
     else r = pcg(C, M->pp, maxIter, tol, amg);

@@ -375,7 +375,9 @@ inline bool check_settings(const rsg_settings* s, std::string& err) {
     if (s->momentum_solver == RSG_SOLVER_AMG || s->turbulence_solver == RSG_SOLVER_AMG) {
         err = "settings: AMG is supported for the pressure solver only"; return false;
     }
-    auto ok = [](int c) { return c == RSG_SOLVER_AMG || c == RSG_SOLVER_PCG || c == RSG_SOLVER_BICGSTAB; };
+    auto ok = [](int c) {
+        return c == RSG_SOLVER_AMG || c == RSG_SOLVER_PCG || c == RSG_SOLVER_BICGSTAB || c == RSG_SOLVER_GAUSS_SEIDEL;
+    };
     if (!ok(s->pressure_solver) || !ok(s->momentum_solver) || !ok(s->turbulence_solver)) {
         err = "settings: unknown linear solver code"; return false;
     }

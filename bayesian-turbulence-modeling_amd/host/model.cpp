@@ -134,7 +134,7 @@ static int32_t solverCode(const std::string& s) {
     if (s == "AMG") return RSG_SOLVER_AMG;
     if (s == "PCG") return RSG_SOLVER_PCG;
     if (s == "BiCGSTAB") return RSG_SOLVER_BICGSTAB;
-    if (s == "GaussSeidel") throw std::invalid_argument("GaussSeidel linear solver is not available on the device");
+    if (s == "GaussSeidel") return RSG_SOLVER_GAUSS_SEIDEL;
     return RSG_SOLVER_BICGSTAB;   // makeSolver fallback (LinearSolver.hpp:664)
 }
 

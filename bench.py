@@ -656,6 +656,102 @@ def bench_bump(args):
         "cpu_baseline": None}), flush=True)
 
 
+def bench_c4(args):
+    """The large-member ensemble: the periodic Gaussian-bump channel at 2M
+    cells (2000 x 1000, bump 0.3 x 1, length 8, height 2), nu = 0.3 / 1e4
+    (bump-height Reynolds number 1e4 at unit bulk velocity; body force 2e-3
+    gives Re_tau ~ 1500 and U_b ~ 1; tanh wall clustering for y+ = 1), 65
+    members: the default SST coefficients and 64 seeded +-20 % perturbations
+    of betaStar, a1, sigma_k1, sigma_w1, beta1.  Every member runs on the
+    kernel-per-phase path with the 5-level p' hierarchy (coarsest inverted on
+    the matrix cores), once with FP32 inner solves under FP64 refinement and
+    once in FP64, each for --c4-iters outer iterations from the same cold
+    start; each member's relative L2 of U, p, k, omega, nu_t between the two
+    is reported.  A 2M-cell member needs ~10^4 outer iterations to reach
+    1e-8, so the comparison is made at a fixed outer iteration."""
+    # 65 resident 2M-cell members: about 1.2 GB of engine arena each
+    os.environ.setdefault("RSG_HBM_BUDGET_GB", "120")
+    import rans_sst_py as rs
+    import ensembles
+    n_mem, K = args.c4_members, args.c4_iters
+    mesh = bump_mesh(rs, 2000, 1000, re=1500.0)
+    t0 = time.perf_counter()
+    mesh.compute_wall_distance(True)
+    print("bench: c4 mesh %d cells, wall distance %.1f s" % (mesh.n_cells(), time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    specs = ensembles.channel_ensemble(n_mem, args.seed)
+    thetas = [list(ensembles.THETA_DEFAULTS)] + [sp["theta"] for sp in specs[1:]]
+    ps = rs.InferenceParameterSet.from_indices("c2_five", ensembles.THETA_INDICES)
+    obs = rs.ObservationOperator()
+    rs.engine_set_time_slice(0)
+    rs.engine_set_grid(100000, 20)
+    rs.engine_set_grid_levels(5)
+
+    def run(precision):
+        s = rs.SolverSettings()
+        s.max_iterations = K
+        s.convergence_tol = 1e-12
+        s.body_force = rs.Vec3(2e-3, 0.0, 0.0)
+        s.verbose = False
+        bcs = rs.FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+        fms = []
+        for _ in thetas:
+            fm = rs.ForwardModel(mesh, ps, obs, bcs, 0.3 / 1e4, s)
+            fm.set_precision(precision)
+            fm.clear_cache()
+            fms.append(fm)
+        st0 = rs.engine_stats()
+        t = time.perf_counter()
+        # members in groups of 13 (each group's members share the GPU in turns;
+        # a progress line per group)
+        res = []
+        for g in range(0, len(fms), 13):
+            res += rs.evaluate_many(fms[g:g + 13], thetas[g:g + 13])
+            print("bench: c4 %s: %d/%d members done at %.1f s" % (precision, len(res), len(fms),
+                                                                  time.perf_counter() - t), file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t
+        st1 = rs.engine_stats()
+        print("bench: c4 %s: %d members x %d outer iterations in %.1f s" % (precision, len(fms), K, wall),
+              file=sys.stderr, flush=True)
+        fields = [fm.last_fields() for fm in fms]
+        inner = [fm.last_inner_iterations() for fm in fms]
+        return dict(wall=wall, kernel_s=(st1[3] - st0[3]) * 1e-3, res=res, fields=fields, inner=inner)
+
+    mixed = run("mixed")
+    fp64 = run("fp64")
+    per = []
+    for i in range(len(thetas)):
+        e = field_errors(mixed["fields"][i], fp64["fields"][i])
+        a, b = np.asarray(mixed["fields"][i]["p"]), np.asarray(fp64["fields"][i]["p"])
+        e["p_rel_l2"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        e["status"] = [str(mixed["res"][i].status).split(".")[-1], str(fp64["res"][i].status).split(".")[-1]]
+        e["outer_iters"] = [int(mixed["res"][i].simple_iters), int(fp64["res"][i].simple_iters)]
+        e["inner_iters_mixed"] = list(mixed["inner"][i])
+        e["inner_iters_fp64"] = list(fp64["inner"][i])
+        per.append(e)
+    worst = {k: max(p[k] for p in per) for k in ("U", "p", "k", "omega", "nuT", "p_rel_l2")}
+    its = len(thetas) * K
+    print(json.dumps({
+        "metric": "member outer SIMPLE+SST iterations/s of the 65-member 2M-cell bump ensemble (mixed precision), "
+                  "and each member's rel-L2 vs its FP64 GPU solve",
+        "value": its / mixed["wall"], "unit": "member_outer_iters/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * mixed["wall"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-inner/f64-defect", "data": "synthetic",
+        "config": {"workload": "C4: periodic Gaussian-bump channel 2000x1000 = 2M cells, nu 3e-5, body force 2e-3, "
+                               "%d members (defaults + seeded +-20%% on 5 SST coefficients), kernel-per-phase path, "
+                               "5-level p' AMG with the coarsest inverted on the matrix cores, %d outer iterations "
+                               "per member, mixed vs FP64" % (len(thetas), K),
+                   "cells": mesh.n_cells(), "members": len(thetas), "outer_iters": K},
+        "mixed": {"wall_s": mixed["wall"], "kernel_s": mixed["kernel_s"],
+                  "ms_per_member_outer_iter": 1e3 * mixed["wall"] / its},
+        "fp64": {"wall_s": fp64["wall"], "kernel_s": fp64["kernel_s"],
+                 "ms_per_member_outer_iter": 1e3 * fp64["wall"] / its,
+                 "member_outer_iters_per_s": its / fp64["wall"]},
+        "max_rel_l2_vs_fp64": worst,
+        "per_member": per,
+        "roofline": None, "cpu_baseline": None}), flush=True)
+
+
 def bench_c1(args, big=False):
     """The large single system: solves/s of AMG-PCG to 1e-10 (host hierarchy
     setup + device solve + host<->device copies, wall clock), FP64 and mixed.
@@ -889,11 +985,14 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump"), default="c2",
+    ap.add_argument("--c4-members", type=int, default=65, help="c4 workload: ensemble members")
+    ap.add_argument("--c4-iters", type=int, default=40, help="c4 workload: outer iterations per member and precision")
+    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump", "c4"), default="c2",
                     help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
                          "c0: single-solve latency of the Re_tau 550 channel; walldist: the wall-distance "
                          "kernel on a 2M-cell mesh; c4p: the pressure system at 2M cells; bump: team solves "
-                         "of one bump-channel member at 40k, 160k and 2M cells")
+                         "of one bump-channel member at 40k, 160k and 2M cells; c4: the 65-member 2M-cell bump "
+                         "ensemble, mixed precision against FP64 per member")
     ap.add_argument("--bump-sizes", default="", help="comma-separated cell counts of the bump workload to run")
     ap.add_argument("--team-blocks", default="0",
                     help="bump workload: comma-separated team sizes in workgroups (0: the engine's choice)")
@@ -915,7 +1014,7 @@ def main():
               "exchange between them, and runs on several GPUs are not part of this benchmark", file=sys.stderr)
         sys.exit(2)
     {"c1": bench_c1, "c0": bench_c0, "walldist": bench_walldist, "bump": bench_bump,
-     "c4p": lambda a: bench_c1(a, big=True), "c2": bench_c2}[args.workload](args)
+     "c4p": lambda a: bench_c1(a, big=True), "c2": bench_c2, "c4": bench_c4}[args.workload](args)
 
 
 if __name__ == "__main__":

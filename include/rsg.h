@@ -70,6 +70,10 @@ extern "C" {
 #define RSG_SOLVER_AMG 0
 #define RSG_SOLVER_PCG 1
 #define RSG_SOLVER_BICGSTAB 2
+/* GaussSeidelSolver (core/include/LinearSolver.hpp:258-299): in-place sweeps
+ * in cell order, a sequential recurrence run by one GPU thread; for
+ * momentum / turbulence / pressure solves and rsg_ldu_solve, always FP64. */
+#define RSG_SOLVER_GAUSS_SEIDEL 3
 
 /* precision of a member's inner linear solves.  RSG_PREC_MIXED: every linear
  * solve of the SIMPLE iteration (momentum, p', k, omega) computes the defect

@@ -48,6 +48,21 @@ def test_emulated_device_trajectory_bit_exact(oracle, emu, meshes, case, iters, 
         assert np.array_equal(a[k_emu], o[k_or]), k_emu
 
 
+@pytest.mark.parametrize("case,iters", [("channel_bounded", 120), ("c0", 300), ("tri_quad", 60)])
+def test_emulated_gauss_seidel_trajectory_bit_exact(oracle, emu, meshes, case, iters):
+    """Gauss-Seidel (LinearSolver.hpp:258-299) as the momentum and turbulence
+    solver, Jacobi-PCG pressure: the whole trajectory equals the oracle's."""
+    s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=1, momentum_solver=3,
+                     turbulence_solver=3, nut_floor_iters=min(500, iters // 2), record_history=1)
+    r, a = emu_solve(emu, meshes[case], case, s)
+    o, ho, io = oracle_solve(oracle, meshes[case], case, s, hist_cap=iters)
+    assert r.final_iter == io[0] and r.converged == io[1] and r.diverged == io[2]
+    he = a["history"][:6 * r.history_len].reshape(-1, 6)
+    assert np.array_equal(he, ho)
+    for k_emu, k_or in (("U", "U"), ("p", "p"), ("k", "k"), ("omega", "om"), ("nuT", "nuT")):
+        assert np.array_equal(a[k_emu], o[k_or]), k_emu
+
+
 @pytest.mark.parametrize("resident", [1, 0])
 def test_emulated_amg_tracks_oracle(oracle, emu, meshes, resident):
     # 700 iterations of C0 with the device AMG vs the oracle's reference AMG:
@@ -80,11 +95,11 @@ def test_emulated_warm_restart_is_a_continuation(oracle, emu, meshes):
     assert np.array_equal(a1["U"], a2["U"]) and np.array_equal(a1["k"], a2["k"])
 
 
-@pytest.mark.parametrize("solver", [0, 1, 2])
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
 def test_emulated_standalone_linear_solve(oracle, emu, meshes, solver):
     """The stand-alone solve path (topology-only mesh with no patches, as
-    rsg_ldu_solve builds it): Jacobi-PCG and BiCGSTAB equal the oracle bit for
-    bit; AMG-PCG solves to the requested tolerance."""
+    rsg_ldu_solve builds it): Jacobi-PCG, BiCGSTAB and Gauss-Seidel equal the
+    oracle bit for bit; AMG-PCG solves to the requested tolerance."""
     msh = A.Msh(meshes["hill"])
     n, nif = msh.nC, msh.nIF
     rng = np.random.default_rng(solver)
@@ -92,6 +107,8 @@ def test_emulated_standalone_linear_solve(oracle, emu, meshes, solver):
     cf = np.exp(rng.normal(size=nif))
     up = -cf.copy(); lo = -(cf + rng.uniform(0, 0.5, nif)) if solver == 2 else -cf.copy()
     dg = np.zeros(n); np.add.at(dg, own, -lo); np.add.at(dg, nbr, -up); dg[0] *= 2.0
+    if solver == 3:       # Gauss-Seidel converges slowly on the pinned Laplacian: a dominant diagonal
+        dg = dg * 1.5
     b = rng.normal(size=n)
     x = np.zeros(n); st = np.zeros(5)
     assert emu.emu_ldu_solve(n, nif, A.iptr(own), A.iptr(nbr), A.dptr(dg), A.dptr(up), A.dptr(lo), A.dptr(b),

@@ -338,3 +338,52 @@ def test_forward_model_end_to_end(rs, oracle):
     assert r2.simple_iters == 32000
     f2 = fm.last_fields()
     assert 0.0 < rel(f2["U"].ravel(), f["U"].ravel()) < 0.05
+
+
+@pytest.mark.parametrize("path", ["member", "grid"])
+def test_gauss_seidel_trajectory_vs_oracle(rsg_lib, gpu_ctx, oracle, meshes, path):
+    """Gauss-Seidel (LinearSolver.hpp:258-299) as the momentum and turbulence
+    solver, on the one-workgroup member path and on the kernel-per-phase
+    path: the trajectory tracks the oracle's (the sweeps are the reference's
+    sequential recurrence; only the residual norms are reduced in another
+    order)."""
+    case, iters = "channel_bounded", 120
+    s = settings_for(case, max_iterations=iters, convergence_tol=1e-12, pressure_solver=1, momentum_solver=3,
+                     turbulence_solver=3, nut_floor_iters=60)
+    if path == "grid":
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_grid(gpu_ctx, 1, 40))
+    P = Problem(rsg_lib, gpu_ctx, meshes[case], case, s)
+    try:
+        print("gauss-seidel trajectory", path, flush=True)
+        r, a = P.solve()
+    finally:
+        P.close()
+        rsg_lib.rsg_set_grid(gpu_ctx, 0, 20)
+    o, _, io = oracle_solve(oracle, meshes[case], case, s)
+    assert r.final_iter == io[0]
+    for k_gpu, k_or in (("U", "U"), ("k", "k"), ("omega", "om"), ("nuT", "nuT")):
+        assert rel(a[k_gpu], o[k_or]) < 1e-4, (path, k_gpu, rel(a[k_gpu], o[k_or]))
+
+
+def test_gauss_seidel_standalone_solve(rsg_lib, gpu_ctx, oracle, meshes):
+    """rsg_ldu_solve with RSG_SOLVER_GAUSS_SEIDEL against a dense solve and
+    the oracle's sweep count."""
+    msh = A.Msh(meshes["hill"])
+    n, nif = msh.nC, msh.nIF
+    rng = np.random.default_rng(3)
+    own = np.ascontiguousarray(msh.owner[:nif], np.int32); nbr = np.ascontiguousarray(msh.neighbor, np.int32)
+    cf = np.exp(rng.normal(size=nif))
+    up, lo = -cf.copy(), -cf.copy()
+    dg = np.zeros(n); np.add.at(dg, own, cf); np.add.at(dg, nbr, cf); dg *= 1.5
+    b = rng.normal(size=n)
+    x = np.zeros(n); st = np.zeros(4)
+    chk(rsg_lib, gpu_ctx, rsg_lib.rsg_ldu_solve(gpu_ctx, n, nif, A.iptr(own), A.iptr(nbr), A.dptr(dg), A.dptr(up),
+                                                A.dptr(lo), A.dptr(b), A.dptr(x), 3, 0, 1000, C.c_double(1e-10),
+                                                A.dptr(st)))
+    Ad = np.diag(dg); Ad[own, nbr] += up; Ad[nbr, own] += lo
+    assert st[3] == 1 and np.linalg.norm(b - Ad @ x) / np.linalg.norm(b) < 1e-9
+    xo = np.zeros(n); so = np.zeros(4)
+    oracle.om_ldu_solve(3, n, nif, A.iptr(own), A.iptr(nbr), A.dptr(dg), A.dptr(up), A.dptr(lo), A.dptr(b),
+                        A.dptr(xo), 1000, C.c_double(1e-10), A.dptr(so))
+    assert st[0] == so[0]
+    assert rel(x, xo) < 1e-12
