@@ -11,16 +11,19 @@
 struct RsgBig;
 
 // State of one large member whose workspace (hM, device pointers) is already
-// carved and staged; dM: device memory for the member struct.  sc0: its
-// initial scalars.  maxLevels: 0 coarsens like the reference (at most 20
+// carved and staged on stream s; dM: device memory for the member struct.
+// sc0: its initial scalars.  The member runs on a stream of its own, so the
+// engine may run several members concurrently from separate host threads
+// (each thread must have the device current).  maxLevels: 0 coarsens like the reference (at most 20
 // levels, down to 50 rows); k > 0 stops at k levels once the coarsest level
-// can be inverted on the matrix cores (at most 16,384 rows).  Returns null
-// (and err) on a device error.
+// can be inverted on the matrix cores (at most 16,384 rows).  graphs: replay
+// the p' V-cycles from hipGraphs.  Returns null (and err) on a device error.
 RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set,
-                       const rsg::PrepMesh& pm, const RsgScalars& sc0, hipStream_t s, int maxLevels,
+                       const rsg::PrepMesh& pm, const RsgScalars& sc0, hipStream_t s, int maxLevels, bool graphs,
                        std::string& err);
 // Runs up to `iters` outer iterations (fewer if the member finishes); out gets
-// the member's scalars.  A finished member's fields, history, scalars and
-// observables are on the device for collection.
+// the member's scalars.  Returns with the member's stream drained: a finished
+// member's fields, history, scalars and observables are on the device for
+// collection.
 int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err);
 void rsg_big_destroy(RsgBig* B);

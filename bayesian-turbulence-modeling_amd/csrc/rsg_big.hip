@@ -318,6 +318,7 @@ struct RsgBig {
     const int* rp0 = nullptr;     // device level-0 pattern (the mesh's)
     const int* ci0 = nullptr;
     hipStream_t s = nullptr;
+    bool ownStream = false;
     grid::DevAlloc A;             // work vectors, scratch
     double *v0 = nullptr, *dg0 = nullptr, *vz = nullptr, *vq = nullptr;
     grid::Scratch S;
@@ -336,6 +337,7 @@ struct RsgBig {
     float* fw = nullptr;          // 12 FP32 vectors of nC
     std::unique_ptr<grid::DevAlloc> HFA;
     grid::Hierarchy<float> Hf;
+    grid::VcycleGraphs VG, VGf;   // p' V-cycles as hipGraphs (FP64, FP32 mirror)
     grid::DenseWork DW;           // the coarsest level's inverse when it is above 64 rows
     bool bigDense = false;
     double denseMs = 0.0;         // device time of the inversions so far
@@ -484,6 +486,7 @@ hipError_t build_amg(RsgBig& B) {
     std::vector<double> G;
     const bool dense = H.back().n <= RSG_COARSE_DENSE && grid::dense_inverse(H.back(), G);
     B.DW.release();
+    B.VG.clear(); B.VGf.clear();      // the graphs hold the old hierarchy's addresses
     B.HA.reset(new grid::DevAlloc());
     B.H = grid::Hierarchy<double>();
     e = grid::upload<double>(H, G, dense, *B.HA, B.H, B.s);
@@ -629,7 +632,8 @@ hipError_t inner_f32(RsgBig& B, int code, bool amg, const float* bf, float* ef, 
         d.rp = const_cast<int*>(B.rp0); d.ci = const_cast<int*>(B.ci0); d.v = B.v0f; d.ex = B.ex0f; d.dg = B.dg0f;
     }
     grid::PcgResult pr;
-    hipError_t e = grid::pcg<float>(amg ? B.Hf : J, bf, ef, w[0], w[1], w[2], w[3], maxIter, tol, amg, B.S, B.s, pr);
+    hipError_t e = grid::pcg<float>(amg ? B.Hf : J, bf, ef, w[0], w[1], w[2], w[3], maxIter, tol, amg, B.S, B.s, pr,
+                                    &B.VGf);
     res = SolveRes{pr.iterations, pr.finalRes, pr.initialRes, pr.converged};
     return e;
 }
@@ -702,7 +706,7 @@ hipError_t solve(RsgBig& B, int code, double* x, bool amgReady, SolveRes& res, b
     }
     grid::PcgResult pr;
     hipError_t e = grid::pcg<double>(amg ? B.H : J, M.src, x, M.kv[0], B.vz, M.kv[2], B.vq, st.innerIter,
-                                     st.innerTol, amg, B.S, B.s, pr);
+                                     st.innerTol, amg, B.S, B.s, pr, &B.VG);
     res = SolveRes{pr.iterations, pr.finalRes, pr.initialRes, pr.converged};
     return e;
 }
@@ -822,15 +826,22 @@ hipError_t outer_iteration(RsgBig& B) {
 }  // namespace
 
 RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set, const PrepMesh& pm,
-                       const RsgScalars& sc0, hipStream_t s, int maxLevels, std::string& err) {
+                       const RsgScalars& sc0, hipStream_t s, int maxLevels, bool graphs, std::string& err) {
     std::unique_ptr<RsgBig> B(new RsgBig());
     B->maxLevels = maxLevels;
-    B->hM = hM; B->dM = dM; B->set = set; B->pm = &pm; B->nC = pm.nC; B->nnz0 = pm.nnz0; B->s = s;
+    B->VG.enabled = B->VGf.enabled = graphs;
+    B->hM = hM; B->dM = dM; B->set = set; B->pm = &pm; B->nC = pm.nC; B->nnz0 = pm.nnz0;
     B->sc = sc0;
-    RsgMeshDev mesh;
-    hipError_t e = hipMemcpyAsync(&mesh, hM.mesh, sizeof mesh, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    // the member's own stream (default priority): members run concurrently,
+    // one host thread each; the engine stream's staging must be done first
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking);
     if (e != hipSuccess) { err = std::string("big member: ") + hipGetErrorString(e); return nullptr; }
+    B->ownStream = true;
+    RsgMeshDev mesh;
+    e = hipMemcpyAsync(&mesh, hM.mesh, sizeof mesh, hipMemcpyDeviceToHost, B->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(B->s);
+    if (e != hipSuccess) { err = std::string("big member: ") + hipGetErrorString(e); rsg_big_destroy(B.release()); return nullptr; }
     B->rp0 = mesh.rp0; B->ci0 = mesh.ci0;
     const int nb = std::max(grid::blocks(std::max(pm.nC, pm.nnz0)), cells_grid(pm.nC));
     B->v0 = B->A.take<double>(pm.nnz0); B->dg0 = B->A.take<double>(pm.nC);
@@ -845,16 +856,25 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
         B->fw = B->A.take<float>(10 * (size_t)pm.nC);
     }
     if (B->A.err == hipSuccess) B->A.err = hipHostMalloc((void**)&B->S.hdots, 8 * sizeof(double), hipHostMallocDefault);
-    if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(dM, &hM, sizeof hM, hipMemcpyHostToDevice, s);
-    if (B->A.err != hipSuccess) { err = std::string("big member: ") + hipGetErrorString(B->A.err); return nullptr; }
+    if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(dM, &hM, sizeof hM, hipMemcpyHostToDevice, B->s);
+    if (B->A.err == hipSuccess) B->A.err = hipStreamSynchronize(B->s);
+    if (B->A.err != hipSuccess) {
+        err = std::string("big member: ") + hipGetErrorString(B->A.err);
+        rsg_big_destroy(B.release());
+        return nullptr;
+    }
     return B.release();
 }
 
 void rsg_big_destroy(RsgBig* B) {
     if (!B) return;
+    if (B->ownStream) (void)hipStreamSynchronize(B->s);
+    B->VG.clear(); B->VGf.clear();
     B->DW.release();
     if (B->S.hdots) (void)hipHostFree(B->S.hdots);
-    delete B;
+    hipStream_t s = B->ownStream ? B->s : nullptr;
+    delete B;                     // frees the member's work arrays
+    if (s) (void)hipStreamDestroy(s);
 }
 
 int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
@@ -880,8 +900,8 @@ int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
             hipLaunchKernelGGL(k_observe, dim3(1), dim3(64), 0, B->s, B->dM);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(B->s);
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(B->s);
     out = B->sc;
     if (e != hipSuccess) {
         err = std::string("big member: ") + (B->err.empty() ? std::string(hipGetErrorString(e)) : B->err);

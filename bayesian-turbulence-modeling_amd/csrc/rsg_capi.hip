@@ -183,6 +183,9 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 }  // namespace
 
+// kernel-per-phase members run concurrently, at most this many at once
+constexpr size_t kGridParallel = 4;
+
 struct rsg_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -210,6 +213,7 @@ struct rsg_context {
     // gridMinCells cells (0: never), gridIters outer iterations per engine round
     int gridMinCells = 0, gridIters = 20;
     int gridLevels = 5;      // p' AMG levels of kernel-per-phase members (0: the reference's cap)
+    int useGraphs = 1;       // V-cycles replayed from hipGraphs (rsg_set_graphs)
     double* dGred = nullptr;
     RsgMemberDev* dTeamM = nullptr;
     RsgScalars* dTeamOut = nullptr;
@@ -477,6 +481,7 @@ void engine_loop(rsg_context* ctx) {
         std::vector<RsgSettingsDev> gridSet(gridIds.size());
         std::vector<const rsg::PrepMesh*> gridMesh(gridIds.size());
         const int gridLevels = ctx->gridLevels;
+        const bool useGraphs = ctx->useGraphs != 0;
         for (size_t q = 0; q < gridIds.size(); ++q) {
             Ticket& T = *ctx->tickets[gridIds[q]];
             ProblemEntry& P = *ctx->problems[T.problemId];
@@ -495,17 +500,36 @@ void engine_loop(rsg_context* ctx) {
             if (!T.big) {
                 RsgScalars s0;
                 member_init_scalars(s0, T.warm ? T.warmTurb : 0);
-                T.big = rsg_big_create(gridM[q], T.dM, gridSet[q], *gridMesh[q], s0, ctx->stream, gridLevels, berr);
+                T.big = rsg_big_create(gridM[q], T.dM, gridSet[q], *gridMesh[q], s0, ctx->stream, gridLevels,
+                                       useGraphs, berr);
                 if (!T.big) { e = hipErrorUnknown; gridErr = berr; break; }
             }
-            e = hipEventRecord(ctx->ev0, ctx->stream);
-            const int rc = rsg_big_run(T.big, ctx->gridIters, T.sc, berr);
-            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            if (rc != RSG_OK) { e = hipErrorUnknown; gridErr = berr; break; }
-            float gm = 0.f;
-            if (e == hipSuccess && hipEventElapsedTime(&gm, ctx->ev0, ctx->ev1) == hipSuccess) ms += gm;
-            if (T.sc.status != RSGM_RUNNING) gridDone.push_back(gridIds[q]);
+        }
+        // kernel-per-phase members: up to kGridParallel at once, each on its
+        // own stream from its own host thread (a small member's phases leave
+        // most CUs idle, and every Krylov iteration waits on the host)
+        for (size_t q0 = 0; q0 < gridIds.size() && e == hipSuccess; q0 += kGridParallel) {
+            const size_t q1 = std::min(gridIds.size(), q0 + kGridParallel);
+            std::vector<int> rcs(q1 - q0, RSG_OK);
+            std::vector<std::string> errs(q1 - q0);
+            std::vector<Ticket*> tk;
+            for (size_t q = q0; q < q1; ++q) tk.push_back(ctx->tickets[gridIds[q]].get());
+            const int dev = ctx->device, iters = ctx->gridIters;
+            const auto g0 = std::chrono::steady_clock::now();
+            auto runOne = [&tk, &rcs, &errs, q0, dev, iters](size_t q) {
+                Ticket& T = *tk[q - q0];
+                if (hipSetDevice(dev) != hipSuccess) { rcs[q - q0] = RSG_ERR_DEVICE; errs[q - q0] = "hipSetDevice"; return; }
+                rcs[q - q0] = rsg_big_run(T.big, iters, T.sc, errs[q - q0]);
+            };
+            std::vector<std::thread> th;
+            for (size_t q = q0 + 1; q < q1; ++q) th.emplace_back(runOne, q);
+            runOne(q0);
+            for (std::thread& t : th) t.join();
+            ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
+            for (size_t q = q0; q < q1; ++q) {
+                if (rcs[q - q0] != RSG_OK) { e = hipErrorUnknown; gridErr = errs[q - q0]; break; }
+                if (tk[q - q0]->sc.status != RSGM_RUNNING) gridDone.push_back(gridIds[q]);
+            }
         }
         if (!team.empty() && ctx->teamCap <= 0) e = hipErrorInvalidConfiguration;
         if (!team.empty() && e == hipSuccess && !ctx->dGred) {
@@ -1003,6 +1027,13 @@ int rsg_set_grid_levels(rsg_context* ctx, int32_t max_levels) {
     return RSG_OK;
 }
 
+int rsg_set_graphs(rsg_context* ctx, int32_t enable) {
+    if (!ctx) return RSG_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->useGraphs = enable ? 1 : 0;
+    return RSG_OK;
+}
+
 int rsg_set_schedule(rsg_context* ctx, int32_t policy) {
     if (!ctx) return RSG_ERR_ARG;
     if (policy != RSG_SCHEDULE_ROUND_ROBIN && policy != RSG_SCHEDULE_LARGEST_FIRST)
@@ -1184,6 +1215,8 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
         S.dots = A.take<double>(8);
         if (A.err == hipSuccess) A.err = hipHostMalloc((void**)&S.hdots, 8 * sizeof(double), hipHostMallocDefault);
         Hierarchy<double> H64;
+        VcycleGraphs VG64, VG32;
+        VG64.enabled = VG32.enabled = ctx->useGraphs != 0;
         if (precision == RSG_PREC_FP64) {
             upload<double>(H, G, dense, A, H64, s);
         } else {
@@ -1208,7 +1241,8 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
             double* r = A.take<double>(n); double* z = A.take<double>(n);
             double* p = A.take<double>(n); double* q = A.take<double>(n);
             if (A.err == hipSuccess)
-                A.err = pcg<double>(H64, b64, x64, r, z, p, q, max_iter, tol, solver == RSG_SOLVER_AMG, S, s, res);
+                A.err = pcg<double>(H64, b64, x64, r, z, p, q, max_iter, tol, solver == RSG_SOLVER_AMG, S, s, res,
+                                    &VG64);
         } else {
             // FP64 defect / refinement around FP32 inner PCG solves; each inner
             // solve reduces its defect by what is still missing (with a 0.3
@@ -1241,7 +1275,7 @@ int rsg_system_solve(rsg_context* ctx, int32_t n, int32_t nif, const int32_t* ow
                 PcgResult ir;
                 const double innerTol = std::max(1e-5, 0.3 * tol / res.finalRes);
                 A.err = pcg<float>(H32, b32, e32, r, z, p, q, max_iter - inner, innerTol, solver == RSG_SOLVER_AMG,
-                                   S, s, ir);
+                                   S, s, ir, &VG32);
                 inner += ir.iterations;
                 hipLaunchKernelGGL(k_add<float>, gr, bl, 0, s, n, (const float*)e32, x64);
                 ++refinements;

@@ -607,6 +607,53 @@ void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s) {
     }
 }
 
+// hipGraphs of whole V-cycles.  A V-cycle's launch sequence and every
+// argument are fixed by the hierarchy (level arrays, coarsest inverse) and the
+// two level-0 vectors, so it is captured once on the stream and replayed with
+// one hipGraphLaunch afterwards (no memset nodes: every kernel writes its
+// outputs).  The owner clears the cache whenever the hierarchy is rebuilt
+// (new allocations may reuse old addresses).
+struct VcycleGraphs {
+    struct Entry {
+        const void *l0, *rin, *z, *dinv;
+        bool dense;
+        size_t levels;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> e;
+    bool enabled = true;
+    void clear() {
+        for (Entry& x : e) (void)hipGraphExecDestroy(x.exec);
+        e.clear();
+    }
+    ~VcycleGraphs() { clear(); }
+};
+
+template <typename T> void vcycle(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s);
+
+// z = V-cycle(r): replayed from G's graph of this (hierarchy, rin, z), captured
+// on first use; launched directly when G is null or disabled.
+template <typename T>
+hipError_t vcycle_run(const Hierarchy<T>& D, const T* rin, T* z, hipStream_t s, VcycleGraphs* G) {
+    if (!G || !G->enabled) { vcycle<T>(D, rin, z, s); return hipSuccess; }
+    const void* l0 = D.lv.empty() ? nullptr : (const void*)D.lv[0].v;
+    for (const VcycleGraphs::Entry& x : G->e)
+        if (x.l0 == l0 && x.rin == rin && x.z == z && x.dinv == D.dinv && x.dense == D.dense &&
+            x.levels == D.lv.size())
+            return hipGraphLaunch(x.exec, s);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    vcycle<T>(D, rin, z, s);
+    e = hipStreamEndCapture(s, &g);
+    if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return e;
+    G->e.push_back({l0, rin, z, D.dinv, D.dense, D.lv.size(), ex});
+    return hipGraphLaunch(ex, s);
+}
+
 struct Scratch {
     double* part = nullptr;   // [2 * blocks]
     double* dots = nullptr;   // [8] device scalars
@@ -627,7 +674,7 @@ struct PcgResult { int iterations = 0; double finalRes = 0, initialRes = 0; int 
 // x in/out, tol relative to ||b - A x0||.  M: V-cycle (amg) or Jacobi.
 template <typename T>
 hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, int maxIter, double tol, bool amg,
-               const Scratch& S, hipStream_t s, PcgResult& res) {
+               const Scratch& S, hipStream_t s, PcgResult& res, VcycleGraphs* G = nullptr) {
     const DevLevel<T>& A0 = H.lv[0];
     const int n = A0.n;
     const dim3 g(blocks(n)), bl(kBlock);
@@ -640,7 +687,9 @@ hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, 
     const double r0 = std::sqrt(S.hdots[2]);
     res.initialRes = r0;
     if (r0 < 1e-30) { res.converged = 1; return hipSuccess; }
-    if (amg) vcycle<T>(H, r, z, s);
+    if (amg) {
+        if ((e = vcycle_run<T>(H, r, z, s, G)) != hipSuccess) return e;
+    }
     else hipLaunchKernelGGL(k_jac<T>, g, bl, 0, s, n, (const T*)A0.dg, (const T*)r, z);
     hipLaunchKernelGGL((k_convert<T, T>), g, bl, 0, s, n, (const T*)z, p);
     dot2<T>(n, r, z, (const T*)nullptr, (const T*)nullptr, S, dv + 0, s);          // rz
@@ -656,7 +705,9 @@ hipError_t pcg(const Hierarchy<T>& H, const T* b, T* x, T* r, T* z, T* p, T* q, 
         res.iterations = it + 1;
         res.finalRes = std::sqrt(S.hdots[2]) / r0;
         if (res.finalRes < tol) { res.converged = 1; break; }
-        if (amg) vcycle<T>(H, r, z, s);
+        if (amg) {
+            if ((e = vcycle_run<T>(H, r, z, s, G)) != hipSuccess) return e;
+        }
         else hipLaunchKernelGGL(k_jac<T>, g, bl, 0, s, n, (const T*)A0.dg, (const T*)r, z);
         dot2<T>(n, r, z, (const T*)nullptr, (const T*)nullptr, S, dv + 3, s);      // rzNew
         hipLaunchKernelGGL(k_pcg_dir<T>, g, bl, 0, s, n, (const double*)dv, (const T*)z, p);

@@ -389,6 +389,9 @@ PYBIND11_MODULE(rans_sst_py, m) {
     m.def("engine_set_grid_levels", [](int levels) { Engine::get().setGridLevels(levels); }, py::arg("max_levels"),
           "p' AMG levels of kernel-per-phase members (0: the reference's 20-level cap down to 50 rows; k > 1: k "
           "levels, the coarsest inverted on the FP64 matrix cores).");
+    m.def("engine_set_graphs", [](bool enable) { Engine::get().setGraphs(enable); }, py::arg("enable"),
+          "Replay AMG V-cycles of kernel-per-phase members from hipGraphs (default on); results are identical "
+          "either way.");
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
     m.def("engine_set_schedule", [](const std::string& p) {
               if (p == "round_robin") Engine::get().setSchedule(RSG_SCHEDULE_ROUND_ROBIN);

@@ -79,6 +79,12 @@ void Engine::setGridLevels(int maxLevels) {
     if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
 }
 
+void Engine::setGraphs(bool enable) {
+    rsg_context* c = ctx();
+    int rc = rsg_set_graphs(c, enable ? 1 : 0);
+    if (rc != RSG_OK) throw std::invalid_argument(engineError(c, rc));
+}
+
 void Engine::setTeam(int minCells, int blocks, int itersPerLaunch) {
     rsg_context* c = ctx();
     int rc = rsg_set_team(c, minCells, blocks, itersPerLaunch);

@@ -53,6 +53,8 @@ public:
     void setGrid(int minCells, int itersPerLaunch);
     // p' AMG levels of kernel-per-phase members (0: the reference's cap)
     void setGridLevels(int maxLevels);
+    // hipGraph replay of AMG V-cycles on the whole-GPU paths
+    void setGraphs(bool enable);
     std::vector<double> stats();
 private:
     std::mutex mu_;

@@ -579,6 +579,8 @@ def bench_bump(args):
     rs.engine_set_launch(20, 0)
     rs.engine_set_time_slice(0)
 
+    rs.engine_set_graphs(bool(args.graphs))
+
     def run(mesh, iters, mode, blocks=0):
         rs.engine_set_team(20000 if mode == "team" else 0, blocks, 20)
         rs.engine_set_grid(20000 if mode == "grid" else 0, 20)
@@ -640,6 +642,37 @@ def bench_bump(args):
                            setup_s=a["wall"] - k1 * (b["wall"] - a["wall"]) / di)
             row[name] = leg
         rows.append(row)
+    if args.bump_members > 1:
+        # several members of the smallest size at once (kernel-per-phase
+        # members run concurrently on their own streams)
+        nx, ny, _ = sizes[0]
+        mesh = bump_mesh(rs, nx, ny)
+        mesh.compute_wall_distance(True)
+        rs.engine_set_team(0, 0, 20)
+        rs.engine_set_grid(20000, 20)
+        rs.engine_set_grid_levels(5)
+        K = 100
+        fms, th = [], []
+        rng = np.random.default_rng(3)
+        for i in range(args.bump_members):
+            s = rs.SolverSettings()
+            s.max_iterations = K; s.convergence_tol = 1e-12; s.verbose = False
+            s.body_force = rs.Vec3(1.0, 0.0, 0.0)
+            bcs = rs.FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+            fms.append(rs.ForwardModel(mesh, ps, obs, bcs, nu, s))
+            th.append([v * rng.uniform(0.8, 1.2) for v in ensembles.THETA_DEFAULTS])
+        t0 = time.perf_counter()
+        rs.evaluate_many(fms[:1], th[:1])
+        one = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rs.evaluate_many(fms, th)
+        many = time.perf_counter() - t0
+        print("bench: bump %d members x %d iterations: %.2f s (one member alone %.2f s)"
+              % (len(fms), K, many, one), file=sys.stderr, flush=True)
+        rows[0]["concurrent_members"] = {"members": len(fms), "outer_iters": K, "wall_s": many,
+                                         "one_member_wall_s": one,
+                                         "ms_per_member_outer_iter": 1e3 * many / (len(fms) * K),
+                                         "ms_per_outer_iter_alone": 1e3 * one / K}
     big = rows[-1]
     cands = [v for k, v in big.items() if (k.startswith("team") or k.startswith("grid")) and isinstance(v, dict)
              and v.get("ms_per_outer_iter_kernel")]
@@ -689,6 +722,7 @@ def bench_c4(args):
     rs.engine_set_time_slice(0)
     rs.engine_set_grid(100000, 20)
     rs.engine_set_grid_levels(5)
+    rs.engine_set_graphs(bool(args.graphs))
 
     def run(precision):
         s = rs.SolverSettings()
@@ -767,6 +801,7 @@ def bench_c1(args, big=False):
     ctx = C.c_void_p()
     if lib.rsg_context_create(0, C.c_uint64(8 << 30), C.byref(ctx)) != 0:
         raise RuntimeError("rsg_context_create failed: no usable HIP device")
+    lib.rsg_set_graphs(ctx, int(args.graphs))
     s = c1_system.make_system(**(C4P_GRID if big else {}))
     ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
@@ -983,6 +1018,8 @@ def main():
                     help="headline: pick the workgroup size and time slice by measuring candidates on the ensemble "
                          "(1, the default) or keep --block / --slice-ms (0)")
     ap.add_argument("--tune-cache", default="", help="JSON file caching autotuner picks (empty: none)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="c1/c4p/bump/c4: replay AMG V-cycles from hipGraphs (1, the default) or launch directly (0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
@@ -996,6 +1033,8 @@ def main():
                          "of one bump-channel member at 40k, 160k and 2M cells; c4: the 65-member 2M-cell bump "
                          "ensemble, mixed precision against FP64 per member")
     ap.add_argument("--bump-sizes", default="", help="comma-separated cell counts of the bump workload to run")
+    ap.add_argument("--bump-members", type=int, default=1,
+                    help="bump workload: also run this many members of the smallest size at once")
     ap.add_argument("--team-blocks", default="0",
                     help="bump workload: comma-separated team sizes in workgroups (0: the engine's choice)")
     ap.add_argument("--no-member", action="store_true", help="bump workload: skip the one-workgroup leg")

@@ -271,6 +271,11 @@ int rsg_set_grid(rsg_context* ctx, int32_t min_cells, int32_t iters_per_launch);
  * reference's AMGSolver (core/include/LinearSolver.hpp:508-512, 516-551)
  * coarsens to 50 rows and sweeps the coarsest level 50 times. */
 int rsg_set_grid_levels(rsg_context* ctx, int32_t max_levels);
+/* hipGraph replay of AMG V-cycles on the whole-GPU paths (rsg_system_solve and
+ * kernel-per-phase members): enable != 0 (the default) captures each V-cycle's
+ * launch sequence once per hierarchy and replays it with one graph launch;
+ * 0 launches every phase directly.  Results are identical either way. */
+int rsg_set_graphs(rsg_context* ctx, int32_t enable);
 
 #define RSG_SCHEDULE_ROUND_ROBIN 0
 #define RSG_SCHEDULE_LARGEST_FIRST 1

@@ -102,7 +102,7 @@ def gpu_ctx(rsg_lib):
     rsg_lib.rsg_context_destroy(ctx)
 
 
-ENGINE_DEFAULTS = dict(iters_per_launch=200, block=0, slice_us=0, schedule=0, team=(0, 0, 20), grid=(0, 20), grid_levels=5)
+ENGINE_DEFAULTS = dict(iters_per_launch=200, block=0, slice_us=0, schedule=0, team=(0, 0, 20), grid=(0, 20), grid_levels=5, graphs=1)
 
 
 def reset_engine(lib, ctx):
@@ -114,6 +114,7 @@ def reset_engine(lib, ctx):
     lib.rsg_set_team(ctx, *d["team"])
     lib.rsg_set_grid(ctx, *d["grid"])
     lib.rsg_set_grid_levels(ctx, d["grid_levels"])
+    lib.rsg_set_graphs(ctx, d["graphs"])
 
 
 @pytest.fixture(autouse=True)

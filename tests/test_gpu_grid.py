@@ -16,6 +16,7 @@ aggregated on the host), for meshes too large for one workgroup.
 import numpy as np
 import pytest
 
+import rsg_abi as A
 from conftest import oracle_solve, settings_for
 from test_gpu_parity import Problem, chk, rel
 
@@ -151,3 +152,53 @@ def test_grid_levels_arguments(rsg_lib, gpu_ctx):
     assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 21) == -1
     assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 0) == 0
     assert rsg_lib.rsg_set_grid_levels(gpu_ctx, 5) == 0
+
+
+def test_grid_vcycle_graphs_identical(rsg_lib, gpu_ctx, meshes):
+    """Kernel-per-phase member with AMG pressure: V-cycles replayed from
+    hipGraphs (captured once per hierarchy, invalidated when it is rebuilt
+    every amg_refresh iterations) reproduce the directly launched solve bit for
+    bit over 150 outer iterations (three hierarchy rebuilds)."""
+    s = settings_for("bump", max_iterations=150, convergence_tol=1e-12, pressure_solver=0, amg_refresh=50,
+                     coarse_refresh=10)
+    out = {}
+    try:
+        for graphs in (0, 1):
+            chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_graphs(gpu_ctx, graphs))
+            out[graphs] = grid_solve(rsg_lib, gpu_ctx, meshes["bump"], "bump", s)
+    finally:
+        rsg_lib.rsg_set_graphs(gpu_ctx, 1)
+    (r0, a0), (r1, a1) = out[0], out[1]
+    assert r0.final_iter == r1.final_iter == 150
+    for k in ("U", "p", "k", "omega", "nuT"):
+        assert np.array_equal(a0[k], a1[k]), k
+
+
+def test_grid_members_concurrent_equal_alone(rsg_lib, gpu_ctx, meshes):
+    """Six kernel-per-phase members submitted together run concurrently (their
+    own streams and host threads, four at a time); each one's fields equal its
+    solve alone bit for bit."""
+    s = settings_for("bump", max_iterations=60, convergence_tol=1e-12, pressure_solver=0, amg_refresh=50,
+                     coarse_refresh=10)
+    rng = np.random.default_rng(5)
+    coeffs = []
+    for _ in range(6):
+        c = list(A.DEFAULT_COEFFS)
+        for i in (8, 9, 0, 1, 2):
+            c[i] *= rng.uniform(0.8, 1.2)
+        coeffs.append(c)
+    chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_grid(gpu_ctx, 1, 20))
+    P = Problem(rsg_lib, gpu_ctx, meshes["bump"], "bump", s)
+    try:
+        print("grid members concurrently", flush=True)
+        tickets = [P.submit(coeffs=c) for c in coeffs]
+        together = [P.fetch(t) for t in tickets]
+        print("grid members alone", flush=True)
+        alone = [P.fetch(P.submit(coeffs=c)) for c in coeffs]
+    finally:
+        P.close()
+        rsg_lib.rsg_set_grid(gpu_ctx, 0, 20)
+    for (r1, a1), (r2, a2) in zip(together, alone):
+        assert r1.final_iter == r2.final_iter == 60
+        for k in ("U", "p", "k", "omega", "nuT"):
+            assert np.array_equal(a1[k], a2[k]), k

@@ -123,3 +123,20 @@ def test_full_size_fp64_and_mixed(rsg_lib, gpu_ctx):
     assert st64[8] == 7779 and stm[8] == 7779 and st64[9] > 0
     print("full size: fp64 %d it %.3f s, mixed %d it %d sweeps %.3f s, rel diff %.2e"
           % (st64[0], st64[7], stm[0], stm[5], stm[7], rel(xm, x64)), flush=True)
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+def test_vcycle_graphs_replay_identically(rsg_lib, gpu_ctx, small, prec):
+    """V-cycles replayed from hipGraphs (rsg_set_graphs(1), the default) give
+    the same solve bit for bit as direct launches: each solve replays its
+    captured V-cycle once per PCG iteration with the solver's own launches and
+    host synchronisations in between; the solve is repeated twice with graphs."""
+    chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_graphs(gpu_ctx, 0))
+    try:
+        x0, st0 = solve(rsg_lib, gpu_ctx, small, AMG, prec, 1e-10)
+    finally:
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_graphs(gpu_ctx, 1))
+    for _ in range(2):
+        x1, st1 = solve(rsg_lib, gpu_ctx, small, AMG, prec, 1e-10)
+        assert st1[0] == st0[0] and st1[3] == st0[3] == 1
+        assert np.array_equal(x1, x0)
