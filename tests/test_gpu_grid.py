@@ -202,3 +202,44 @@ def test_grid_members_concurrent_equal_alone(rsg_lib, gpu_ctx, meshes):
         assert r1.final_iter == r2.final_iter == 60
         for k in ("U", "p", "k", "omega", "nuT"):
             assert np.array_equal(a1[k], a2[k]), k
+
+
+def test_c4_shape_members_mixed_vs_fp64(rs):
+    """The large-member ensemble (C4) at a tenth of its size: three members of
+    a 160,000-cell bump channel (defaults and two +-20 % perturbations of the
+    five SST coefficients), kernel-per-phase path with the 5-level p'
+    hierarchy (coarsest inverted on the matrix cores), 30 outer iterations in
+    mixed precision and in FP64; each member's fields track its FP64 solve."""
+    from test_gpu_team import _bump_mesh
+    import ensembles
+    mesh = _bump_mesh(rs, 400, 400)
+    mesh.compute_wall_distance(True)
+    ps = rs.InferenceParameterSet.from_indices("c2_five", ensembles.THETA_INDICES)
+    obs = rs.ObservationOperator()
+    rng = np.random.default_rng(11)
+    thetas = [list(ensembles.THETA_DEFAULTS)] + [[v * rng.uniform(0.8, 1.2) for v in ensembles.THETA_DEFAULTS]
+                                                  for _ in range(2)]
+    rs.engine_set_grid(100000, 10)
+    rs.engine_set_grid_levels(5)
+    out = {}
+    try:
+        for prec in ("mixed", "fp64"):
+            s = rs.SolverSettings()
+            s.max_iterations = 30; s.convergence_tol = 1e-12; s.verbose = False
+            s.body_force = rs.Vec3(1.0, 0.0, 0.0)
+            bcs = rs.FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+            fms = [rs.ForwardModel(mesh, ps, obs, bcs, 6e-4, s) for _ in thetas]
+            for fm in fms:
+                fm.set_precision(prec)
+            print("c4 shape: 3 members x 160k cells,", prec, flush=True)
+            res = rs.evaluate_many(fms, thetas)
+            assert all(r.simple_iters == 30 for r in res)
+            out[prec] = [fm.last_fields() for fm in fms]
+    finally:
+        rs.engine_set_grid(0, 20)
+    for i in range(len(thetas)):
+        a, b = out["mixed"][i], out["fp64"][i]
+        errs = {k: rel(np.asarray(a[k]).ravel(), np.asarray(b[k]).ravel()) for k in ("U", "k", "omega", "nuT")}
+        print("member %d mixed vs fp64: %s" % (i, " ".join("%s %.1e" % kv for kv in errs.items())))
+        for k, e in errs.items():
+            assert e < 1e-4, (i, k, e)
