@@ -388,11 +388,15 @@ void engine_loop(rsg_context* ctx) {
         }
         if (ctx->active.empty()) continue;
         ctx->maxActive = std::max(ctx->maxActive, (double)ctx->active.size());
-        // kernel-per-phase members run one after the other on the whole GPU
-        // (gridIters outer iterations each per round); the others share the
-        // member launches
+        // kernel-per-phase members: the first kGridParallel in admission
+        // order run gridIters outer iterations each this round, concurrently
+        // with one another and with the member launch below (own streams and
+        // host threads); the others share the member launches
         std::vector<int64_t> gridIds, rest;
-        for (int64_t id : ctx->active) (ctx->tickets[id]->grid ? gridIds : rest).push_back(id);
+        for (int64_t id : ctx->active) {
+            if (!ctx->tickets[id]->grid) rest.push_back(id);
+            else if (gridIds.size() < kGridParallel) gridIds.push_back(id);
+        }
         const int na = (int)rest.size();
         if (na > ctx->dCap) {
             if (ctx->dMembers) hipFree(ctx->dMembers);
@@ -505,31 +509,23 @@ void engine_loop(rsg_context* ctx) {
                 if (!T.big) { e = hipErrorUnknown; gridErr = berr; break; }
             }
         }
-        // kernel-per-phase members: up to kGridParallel at once, each on its
-        // own stream from its own host thread (a small member's phases leave
-        // most CUs idle, and every Krylov iteration waits on the host)
-        for (size_t q0 = 0; q0 < gridIds.size() && e == hipSuccess; q0 += kGridParallel) {
-            const size_t q1 = std::min(gridIds.size(), q0 + kGridParallel);
-            std::vector<int> rcs(q1 - q0, RSG_OK);
-            std::vector<std::string> errs(q1 - q0);
-            std::vector<Ticket*> tk;
-            for (size_t q = q0; q < q1; ++q) tk.push_back(ctx->tickets[gridIds[q]].get());
+        // kernel-per-phase members: each on its own stream from its own host
+        // thread (a small member's phases leave most CUs idle, and every
+        // Krylov iteration waits on the host), joined after the member launch
+        const auto round0 = std::chrono::steady_clock::now();
+        std::vector<int> gridRc(gridIds.size(), RSG_OK);
+        std::vector<std::string> gridErrs(gridIds.size());
+        std::vector<Ticket*> gridTk;
+        for (int64_t id : gridIds) gridTk.push_back(ctx->tickets[id].get());
+        std::vector<std::thread> gridTh;
+        if (e == hipSuccess) {
             const int dev = ctx->device, iters = ctx->gridIters;
-            const auto g0 = std::chrono::steady_clock::now();
-            auto runOne = [&tk, &rcs, &errs, q0, dev, iters](size_t q) {
-                Ticket& T = *tk[q - q0];
-                if (hipSetDevice(dev) != hipSuccess) { rcs[q - q0] = RSG_ERR_DEVICE; errs[q - q0] = "hipSetDevice"; return; }
-                rcs[q - q0] = rsg_big_run(T.big, iters, T.sc, errs[q - q0]);
-            };
-            std::vector<std::thread> th;
-            for (size_t q = q0 + 1; q < q1; ++q) th.emplace_back(runOne, q);
-            runOne(q0);
-            for (std::thread& t : th) t.join();
-            ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
-            for (size_t q = q0; q < q1; ++q) {
-                if (rcs[q - q0] != RSG_OK) { e = hipErrorUnknown; gridErr = errs[q - q0]; break; }
-                if (tk[q - q0]->sc.status != RSGM_RUNNING) gridDone.push_back(gridIds[q]);
-            }
+            for (size_t q = 0; q < gridIds.size(); ++q)
+                gridTh.emplace_back([&gridTk, &gridRc, &gridErrs, q, dev, iters]() {
+                    Ticket& T = *gridTk[q];
+                    if (hipSetDevice(dev) != hipSuccess) { gridRc[q] = RSG_ERR_DEVICE; gridErrs[q] = "hipSetDevice"; return; }
+                    gridRc[q] = rsg_big_run(T.big, iters, T.sc, gridErrs[q]);
+                });
         }
         if (!team.empty() && ctx->teamCap <= 0) e = hipErrorInvalidConfiguration;
         if (!team.empty() && e == hipSuccess && !ctx->dGred) {
@@ -564,6 +560,13 @@ void engine_loop(rsg_context* ctx) {
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             float cm = 0.f;
             if (e == hipSuccess && hipEventElapsedTime(&cm, ctx->ev0, ctx->ev1) == hipSuccess) ms += cm;
+        }
+        for (std::thread& t : gridTh) t.join();
+        if (!gridTh.empty())   // the round's wall time: the launches above overlap the grid members
+            ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - round0).count();
+        for (size_t q = 0; q < gridIds.size() && e == hipSuccess; ++q) {
+            if (gridRc[q] != RSG_OK) { e = hipErrorUnknown; gridErr = gridErrs[q]; break; }
+            if (gridTk[q]->sc.status != RSGM_RUNNING) gridDone.push_back(gridIds[q]);
         }
         // retire finished members
         std::vector<int64_t> done;
