@@ -37,7 +37,7 @@ BUILD = os.path.join(HERE, "_build")
 HIP_UNITS = [("rsg_capi.hip", "rsg_capi", []), ("rsg_team.hip", "rsg_team", []), ("rsg_big.hip", "rsg_big", [])] + \
     [("rsg_chunk.hip", "rsg_chunk_%d_%d" % (bs, prec), ["-DRSG_CHUNK_BS=%d" % bs, "-DRSG_CHUNK_PREC=%d" % prec])
      for bs in (512, 256) for prec in (0, 1)]
-HOST_SOURCES = ("bindings.cpp", "forward_model.cpp", "sensitivity.cpp", "model.cpp", "mesh.cpp")
+HOST_SOURCES = ("bindings.cpp", "forward_model.cpp", "sensitivity.cpp", "model.cpp", "mesh.cpp", "compressible.cpp")
 
 
 def _compile_hip(unit):

@@ -341,6 +341,12 @@ struct RsgBig {
     grid::DenseWork DW;           // the coarsest level's inverse when it is above 64 rows
     bool bigDense = false;
     double denseMs = 0.0;         // device time of the inversions so far
+    // compressible member (rsg_comp.h): its extra fields and per-iteration maxima
+    bool comp = false;
+    RsgCompDev hQ{};
+    RsgCompDev* dQ = nullptr;
+    double* compPart = nullptr;
+    double compTIters = 0.0;      // energy-equation inner iterations so far
     // outer-iteration state (the device member's Scal, kept on the host)
     RsgScalars sc{};
     std::vector<double> hist;
@@ -823,10 +829,13 @@ hipError_t outer_iteration(RsgBig& B) {
     return hipSuccess;
 }
 
+#include "rsg_comp.h"
+
 }  // namespace
 
 RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettingsDev& set, const PrepMesh& pm,
-                       const RsgScalars& sc0, hipStream_t s, int maxLevels, bool graphs, std::string& err) {
+                       const RsgScalars& sc0, hipStream_t s, int maxLevels, bool graphs, const RsgCompDev* comp,
+                       std::string& err) {
     std::unique_ptr<RsgBig> B(new RsgBig());
     B->maxLevels = maxLevels;
     B->VG.enabled = B->VGf.enabled = graphs;
@@ -847,7 +856,7 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
     B->v0 = B->A.take<double>(pm.nnz0); B->dg0 = B->A.take<double>(pm.nC);
     B->vz = B->A.take<double>(pm.nC); B->vq = B->A.take<double>(pm.nC);
     B->S.part = B->A.take<double>(4 * (size_t)nb + 4);
-    B->S.dots = B->A.take<double>(8);
+    B->S.dots = B->A.take<double>(16);
     B->dOk = B->A.take<int>(1);
     B->mixed = hM.prec != 0;
     if (B->mixed) {
@@ -855,7 +864,27 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
         B->v0f = B->A.take<float>(pm.nnz0); B->ex0f = B->A.take<float>(pm.nC); B->dg0f = B->A.take<float>(pm.nC);
         B->fw = B->A.take<float>(10 * (size_t)pm.nC);
     }
-    if (B->A.err == hipSuccess) B->A.err = hipHostMalloc((void**)&B->S.hdots, 8 * sizeof(double), hipHostMallocDefault);
+    if (comp) {
+        // the compressible fields; the problem's data (temperature BCs, gas) as given
+        B->comp = true;
+        RsgCompDev& Q = B->hQ;
+        Q = *comp;
+        const size_t nC = pm.nC, nB = std::max(pm.nB, 1);
+        Q.T = B->A.take<double>(nC); Q.Tb = B->A.take<double>(nB); Q.rho = B->A.take<double>(nC);
+        Q.mu = B->A.take<double>(nC); Q.Told = B->A.take<double>(nC);
+        Q.Ux0 = B->A.take<double>(nC); Q.Uy0 = B->A.take<double>(nC); Q.Uz0 = B->A.take<double>(nC);
+        Q.mdu = B->A.take<double>(nC);
+        for (int q = 0; q < 3; ++q) Q.gt[q] = B->A.take<double>(nC);
+        Q.zeroB = B->A.take<double>(nB);
+        Q.pMech = B->A.take<double>(nC);
+        B->compPart = B->A.take<double>((size_t)CQ_COUNT * cells_grid(pm.nC));
+        B->dQ = B->A.take<RsgCompDev>(1);
+        if (B->A.err == hipSuccess) B->A.err = hipMemsetAsync(Q.zeroB, 0, 8 * nB, B->s);
+        if (B->A.err == hipSuccess)
+            B->A.err = hipMemsetAsync(B->compPart, 0, 8 * (size_t)CQ_COUNT * cells_grid(pm.nC), B->s);
+        if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(B->dQ, &Q, sizeof Q, hipMemcpyHostToDevice, B->s);
+    }
+    if (B->A.err == hipSuccess) B->A.err = hipHostMalloc((void**)&B->S.hdots, 16 * sizeof(double), hipHostMallocDefault);
     if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(dM, &hM, sizeof hM, hipMemcpyHostToDevice, B->s);
     if (B->A.err == hipSuccess) B->A.err = hipStreamSynchronize(B->s);
     if (B->A.err != hipSuccess) {
@@ -881,7 +910,13 @@ int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
     hipError_t e = hipSuccess;
     if (!B->sc.initialized) {
         const dim3 gc(cells_grid(B->nC)), bl(kBlock);
-        if (!B->hM.initFromFields) {
+        if (B->comp) {
+            // initUniform (CompressibleSIMPLESolver.cpp:27-72): fields, then
+            // applyAllCompressibleBCs with the initial viscosity and beta1 = 0.075
+            hipLaunchKernelGGL(kc_init, gc, bl, 0, B->s, B->dM, B->dQ);
+            hipLaunchKernelGGL(k_bcs, gc, bl, 0, B->s, B->dM, 15, 0.075);
+            hipLaunchKernelGGL(kc_bnd, gc, bl, 0, B->s, B->dM, B->dQ, 1, 1);
+        } else if (!B->hM.initFromFields) {
             hipLaunchKernelGGL(k_init, gc, bl, 0, B->s, B->dM);
             hipLaunchKernelGGL(k_bcs, gc, bl, 0, B->s, B->dM, 15, 0.075);
         }
@@ -889,13 +924,18 @@ int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
         B->sc.amgBuiltIter = -1;
         e = hipGetLastError();
     }
-    for (int k = 0; k < iters && B->sc.status == RSGM_RUNNING && e == hipSuccess; ++k) e = outer_iteration(*B);
+    for (int k = 0; k < iters && B->sc.status == RSGM_RUNNING && e == hipSuccess; ++k)
+        e = B->comp ? comp_outer_iteration(*B) : outer_iteration(*B);
     if (e == hipSuccess && B->sc.status != RSGM_RUNNING) {
         // the finished member's history and scalars on the device (collected
         // like any member's), then its observables
         if (!B->hist.empty())
             e = hipMemcpyAsync(B->hM.hist, B->hist.data(), 8 * B->hist.size(), hipMemcpyHostToDevice, B->s);
         if (e == hipSuccess) e = hipMemcpyAsync(B->hM.sc, &B->sc, sizeof B->sc, hipMemcpyHostToDevice, B->s);
+        if (e == hipSuccess && B->comp) {
+            hipLaunchKernelGGL(kc_export, dim3(cells_grid(B->nC)), dim3(kBlock), 0, B->s, B->dM, B->dQ, B->nC);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess && B->hM.obs) {
             hipLaunchKernelGGL(k_observe, dim3(1), dim3(64), 0, B->s, B->dM);
             e = hipGetLastError();
@@ -907,5 +947,21 @@ int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
         err = std::string("big member: ") + (B->err.empty() ? std::string(hipGetErrorString(e)) : B->err);
         return RSG_ERR_DEVICE;
     }
+    return RSG_OK;
+}
+
+int rsg_big_comp_fields(RsgBig* B, double* T, double* rho, double* pMech, double* Tb, double* energyIters,
+                        std::string& err) {
+    if (!B->comp) { err = "not a compressible member"; return RSG_ERR_STATE; }
+    const RsgCompDev& Q = B->hQ;
+    const size_t n = B->nC, nB = B->pm->nB;
+    hipError_t e = hipSuccess;
+    if (T) e = hipMemcpyAsync(T, Q.T, 8 * n, hipMemcpyDeviceToHost, B->s);
+    if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, Q.rho, 8 * n, hipMemcpyDeviceToHost, B->s);
+    if (e == hipSuccess && pMech) e = hipMemcpyAsync(pMech, Q.pMech, 8 * n, hipMemcpyDeviceToHost, B->s);
+    if (e == hipSuccess && Tb && nB > 0) e = hipMemcpyAsync(Tb, Q.Tb, 8 * nB, hipMemcpyDeviceToHost, B->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(B->s);
+    if (energyIters) *energyIters = B->compTIters;
+    if (e != hipSuccess) { err = std::string("compressible member: ") + hipGetErrorString(e); return RSG_ERR_DEVICE; }
     return RSG_OK;
 }

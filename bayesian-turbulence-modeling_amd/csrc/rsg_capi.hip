@@ -151,6 +151,11 @@ struct ProblemEntry {
     int nObs = 0;                   // observables (rsg_problem_set_observations)
     size_t obsOff = 0, obsBytes = 0;
     RsgObsDev* obsDev = nullptr;
+    // compressible problem (rsg_comp_problem_create): gas, alphaT, the
+    // temperature BC arrays (arena block compOff)
+    bool comp = false;
+    RsgCompDev compDev{};
+    size_t compOff = 0, compBytes = 0;
 };
 
 struct Ticket {
@@ -175,6 +180,11 @@ struct Ticket {
     bool grid = false;
     RsgMemberDev* dM = nullptr;      // the member struct on the device
     RsgBig* big = nullptr;
+    // compressible solve: initial T / rho / mu, results T, rho, p_mech (cells), T (boundary)
+    bool comp = false;
+    double compT0 = 0, compRho0 = 0, compMu0 = 0;
+    std::vector<double> compCells, compTb;
+    double compEnergyIters = 0;
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -273,7 +283,7 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     MeshEntry& ME = *ctx->meshes[P.meshId];
     const rsg::PrepMesh& pm = ME.prep;
     const int histCap = P.settings.record_history ? P.settings.max_iterations : 0;
-    T.grid = ctx->gridMinCells > 0 && pm.nC >= ctx->gridMinCells;
+    T.grid = T.comp || (ctx->gridMinCells > 0 && pm.nC >= ctx->gridMinCells);
     rsg::Carver sz;
     {
         RsgMemberDev tmp;
@@ -291,7 +301,8 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M, T.grid);
     if (P.nObs > 0) { M.obs = P.obsDev; M.pred = cv.take<double>((size_t)P.nObs + 1); }
     T.dM = T.grid ? cv.take<RsgMemberDev>(1) : nullptr;
-    M.mesh = ME.devStruct; M.bc = P.bcDev; M.set = P.setDev; M.nu = P.nu;
+    M.mesh = ME.devStruct; M.bc = P.bcDev; M.set = P.setDev;
+    M.nu = T.comp ? T.compMu0 / T.compRho0 : P.nu;   // compressible: the initial mu / rho
     for (int i = 0; i < 11; ++i) M.coeffs[i] = T.coeffs[i];
     for (int i = 0; i < 6; ++i) M.init[i] = T.init[i];
     M.initFromFields = T.warm;
@@ -484,6 +495,7 @@ void engine_loop(rsg_context* ctx) {
         std::vector<RsgMemberDev> gridM(gridIds.size());
         std::vector<RsgSettingsDev> gridSet(gridIds.size());
         std::vector<const rsg::PrepMesh*> gridMesh(gridIds.size());
+        std::vector<RsgCompDev> gridComp(gridIds.size());
         const int gridLevels = ctx->gridLevels;
         const bool useGraphs = ctx->useGraphs != 0;
         for (size_t q = 0; q < gridIds.size(); ++q) {
@@ -492,6 +504,10 @@ void engine_loop(rsg_context* ctx) {
             gridM[q] = T.M;
             gridSet[q] = rsg::settings_dev(&P.settings);
             gridMesh[q] = &ctx->meshes[P.meshId]->prep;
+            if (T.comp) {
+                gridComp[q] = P.compDev;
+                gridComp[q].Tinit = T.compT0; gridComp[q].rho0 = T.compRho0; gridComp[q].mu0 = T.compMu0;
+            }
         }
         lk.unlock();
         hipError_t e = hipSuccess;
@@ -505,7 +521,7 @@ void engine_loop(rsg_context* ctx) {
                 RsgScalars s0;
                 member_init_scalars(s0, T.warm ? T.warmTurb : 0);
                 T.big = rsg_big_create(gridM[q], T.dM, gridSet[q], *gridMesh[q], s0, ctx->stream, gridLevels,
-                                       useGraphs, berr);
+                                       useGraphs, T.comp ? &gridComp[q] : nullptr, berr);
                 if (!T.big) { e = hipErrorUnknown; gridErr = berr; break; }
             }
         }
@@ -575,6 +591,16 @@ void engine_loop(rsg_context* ctx) {
             Ticket& T = *ctx->tickets[id];
             ProblemEntry& P = *ctx->problems[T.problemId];
             e = collect(T, ctx->meshes[P.meshId]->prep, P.settings, P.nObs);
+            if (e == hipSuccess && T.comp) {
+                const rsg::PrepMesh& pm = ctx->meshes[P.meshId]->prep;
+                T.compCells.assign(3 * (size_t)pm.nC, 0.0);
+                T.compTb.assign((size_t)std::max(pm.nB, 1), 0.0);
+                const size_t nC = pm.nC;
+                if (rsg_big_comp_fields(T.big, T.compCells.data(), T.compCells.data() + nC,
+                                        T.compCells.data() + 2 * nC, T.compTb.data(), &T.compEnergyIters,
+                                        gridErr) != RSG_OK)
+                    e = hipErrorUnknown;
+            }
             rsg_big_destroy(T.big);
             T.big = nullptr;
             done.push_back(id);
@@ -789,6 +815,7 @@ int rsg_problem_release(rsg_context* ctx, int32_t problem_id) {
     ctx->meshes[it->second->meshId]->users--;
     ctx->arena.release(it->second->off, it->second->bytes);
     if (it->second->obsBytes) ctx->arena.release(it->second->obsOff, it->second->obsBytes);
+    if (it->second->compBytes) ctx->arena.release(it->second->compOff, it->second->compBytes);
     ctx->problems.erase(it);
     return RSG_OK;
 }
@@ -869,6 +896,7 @@ int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const
         if (ctx->engineError) return set_err(ctx, RSG_ERR_DEVICE, ctx->lastError);
         auto it = ctx->problems.find(problem_id);
         if (it == ctx->problems.end()) return set_err(ctx, RSG_ERR_STATE, "unknown problem id");
+        if (it->second->comp) return set_err(ctx, RSG_ERR_ARG, "compressible problem: submit with rsg_comp_submit");
         const rsg::PrepMesh& pm = ctx->meshes[it->second->meshId]->prep;
         T->problemId = problem_id;
         for (int i = 0; i < 11; ++i) T->coeffs[i] = coeffs[i];
@@ -905,6 +933,120 @@ int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const
         ctx->tickets[T->id] = std::move(T);
     }
     ctx->cvEngine.notify_all();
+    return RSG_OK;
+}
+
+int rsg_comp_problem_create(rsg_context* ctx, int32_t mesh_id, const rsg_bc_desc* bc, const rsg_comp_desc* comp,
+                            const rsg_settings* settings, int32_t* problem_id) {
+    if (!ctx || !bc || !comp || !settings || !problem_id) return RSG_ERR_ARG;
+    if (!(comp->gamma > 1.0) || !(comp->R > 0.0) || !(comp->Pr > 0.0) || !(comp->Pr_T > 0.0) ||
+        !(comp->mu_ref > 0.0) || !(comp->T_ref > 0.0) || !(comp->S_suth >= 0.0) || !(comp->alpha_t > 0.0) ||
+        !(comp->alpha_t <= 1.0))
+        return set_err(ctx, RSG_ERR_ARG, "compressible problem: gas constants or alpha_t out of range");
+    int nB = 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->meshes.find(mesh_id);
+        if (it == ctx->meshes.end()) return set_err(ctx, RSG_ERR_STATE, "unknown mesh id");
+        nB = it->second->prep.nB;
+    }
+    if (nB > 0 && (!comp->t_type || !comp->t_value))
+        return set_err(ctx, RSG_ERR_ARG, "compressible problem: missing temperature BC array");
+    for (int b = 0; b < nB; ++b)
+        if (comp->t_type[b] < RSG_TBC_DIRICHLET || comp->t_type[b] > RSG_TBC_KEEP)
+            return set_err(ctx, RSG_ERR_ARG, "compressible problem: unknown temperature BC type");
+    int32_t pid = -1;
+    const int rc = rsg_problem_create(ctx, mesh_id, bc, 0.0, settings, &pid);
+    if (rc != RSG_OK) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ProblemEntry& P = *ctx->problems[pid];
+    const size_t nb1 = (size_t)std::max(nB, 1);
+    const size_t oV = (4 * nb1 + 255) & ~size_t(255);
+    const size_t bytes = oV + 8 * nb1;
+    std::vector<char> stage(bytes, 0);
+    for (int b = 0; b < nB; ++b) {
+        const int32_t t = comp->t_type[b] == RSG_TBC_DIRICHLET ? RSGT_DIRICHLET
+                          : comp->t_type[b] == RSG_TBC_ZERO_GRADIENT ? RSGT_ZERO_GRADIENT : 2;
+        memcpy(stage.data() + 4 * b, &t, 4);
+    }
+    if (nB > 0) memcpy(stage.data() + oV, comp->t_value, 8 * (size_t)nB);
+    size_t off;
+    hipError_t e = hipErrorOutOfMemory;
+    if (ctx->arena.alloc(bytes, off)) {
+        e = hipMemcpy(ctx->arena.base + off, stage.data(), bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) ctx->arena.release(off, bytes);
+    }
+    if (e != hipSuccess) {
+        ctx->meshes[P.meshId]->users--;
+        ctx->arena.release(P.off, P.bytes);
+        ctx->problems.erase(pid);
+        return set_err(ctx, e == hipErrorOutOfMemory ? RSG_ERR_NOMEM : RSG_ERR_DEVICE,
+                       "compressible problem: temperature BCs do not fit");
+    }
+    P.comp = true;
+    P.compOff = off; P.compBytes = bytes;
+    RsgCompDev& Q = P.compDev;
+    Q.tT = reinterpret_cast<const int*>(ctx->arena.base + off);
+    Q.tV = reinterpret_cast<const double*>(ctx->arena.base + off + oV);
+    Q.gamma = comp->gamma; Q.R = comp->R; Q.Pr = comp->Pr; Q.PrT = comp->Pr_T;
+    Q.muRef = comp->mu_ref; Q.Tref = comp->T_ref; Q.S = comp->S_suth;
+    Q.alphaT = comp->alpha_t;
+    *problem_id = pid;
+    return RSG_OK;
+}
+
+int rsg_comp_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const rsg_comp_init* init,
+                    int64_t* ticket) {
+    if (!ctx || !coeffs || !init || !ticket) return RSG_ERR_ARG;
+    if (!(init->T > 0.0) || !(init->p > 0.0) || !std::isfinite(init->k) || !std::isfinite(init->omega))
+        return set_err(ctx, RSG_ERR_ARG, "compressible initial state: T and p must be positive");
+    std::unique_ptr<Ticket> T(new Ticket());
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->engineError) return set_err(ctx, RSG_ERR_DEVICE, ctx->lastError);
+        auto it = ctx->problems.find(problem_id);
+        if (it == ctx->problems.end()) return set_err(ctx, RSG_ERR_STATE, "unknown problem id");
+        if (!it->second->comp) return set_err(ctx, RSG_ERR_ARG, "not a compressible problem");
+        const RsgCompDev& Q = it->second->compDev;
+        T->problemId = problem_id;
+        for (int i = 0; i < 11; ++i) T->coeffs[i] = coeffs[i];
+        // initUniform (CompressibleSIMPLESolver.cpp:27-72): density from the
+        // thermodynamic state, the stored (mechanical) pressure, Sutherland mu
+        const double rho0 = init->p / (Q.R * init->T);
+        const double ratio = init->T / Q.Tref;
+        const double mu0 = Q.muRef * ratio * std::sqrt(ratio) * (Q.Tref + Q.S) / (init->T + Q.S);
+        T->init[0] = init->U[0]; T->init[1] = init->U[1]; T->init[2] = init->U[2];
+        T->init[3] = init->p + (2.0 / 3.0) * rho0 * std::max(init->k, 0.0);
+        T->init[4] = init->k; T->init[5] = init->omega;
+        T->comp = true;
+        T->compT0 = init->T; T->compRho0 = rho0; T->compMu0 = mu0;
+        T->precision = RSG_PREC_FP64;
+        T->id = ctx->nextTicket++;
+        T->tSubmit = now_s(ctx->t0);
+        *ticket = T->id;
+        ctx->pending.push_back(T->id);
+        ctx->tickets[T->id] = std::move(T);
+    }
+    ctx->cvEngine.notify_all();
+    return RSG_OK;
+}
+
+int rsg_comp_fetch(rsg_context* ctx, int64_t ticket, double* T_cells, double* rho, double* p_mech, double* T_bnd,
+                   double* energy_iters) {
+    if (!ctx) return RSG_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tickets.find(ticket);
+    if (it == ctx->tickets.end()) return set_err(ctx, RSG_ERR_STATE, "unknown ticket");
+    Ticket& T = *it->second;
+    if (!T.comp) return set_err(ctx, RSG_ERR_ARG, "not a compressible solve");
+    if (T.state != RSG_TICKET_DONE) return set_err(ctx, RSG_ERR_STATE, "ticket not finished");
+    const rsg::PrepMesh& pm = ctx->meshes[ctx->problems[T.problemId]->meshId]->prep;
+    const size_t n = pm.nC;
+    if (T_cells) memcpy(T_cells, T.compCells.data(), 8 * n);
+    if (rho) memcpy(rho, T.compCells.data() + n, 8 * n);
+    if (p_mech) memcpy(p_mech, T.compCells.data() + 2 * n, 8 * n);
+    if (T_bnd && pm.nB > 0) memcpy(T_bnd, T.compTb.data(), 8 * (size_t)pm.nB);
+    if (energy_iters) *energy_iters = T.compEnergyIters;
     return RSG_OK;
 }
 

@@ -179,6 +179,27 @@ struct RsgObsDev {
     const double* fx;      // face-centre x of each listed face
 };
 
+// Compressible pressure-based SIMPLE (compressible/src/CompressibleSIMPLESolver.cpp):
+// the ideal-gas problem data (IdealGasEOS.hpp:6-38, the temperature BCs of
+// CompressibleBCs.hpp:10-25 resolved per boundary face) and the fields the
+// incompressible member struct does not carry.  p in RsgMemberDev is the
+// MECHANICAL pressure p_thermo + (2/3) rho k (the solver's working variable).
+enum { RSGT_DIRICHLET = 0, RSGT_ZERO_GRADIENT = 1 };
+struct RsgCompDev {
+    const int* tT;         // [nB] RSGT_*
+    const double* tV;      // [nB] Dirichlet temperature
+    double gamma, R, Pr, PrT, muRef, Tref, S;
+    double alphaT;
+    double Tinit;          // uniform initial temperature
+    double rho0, mu0;      // initial density and viscosity (host-evaluated EOS)
+    double *T, *Tb, *rho, *mu, *Told;
+    double *Ux0, *Uy0, *Uz0;   // velocity at the start of the outer iteration
+    double *mdu;           // muEff div U
+    double *gt[3];         // grad (muEff div U)
+    double *zeroB;         // [nB] zeros (boundary values of muEff div U)
+    double *pMech;         // mechanical pressure of a finished solve (p then holds the thermodynamic one)
+};
+
 struct RsgMemberDev {
     const RsgMeshDev* mesh;
     const RsgBcDev* bc;

@@ -7,6 +7,7 @@
 
 #include <stdexcept>
 
+#include "compressible.hpp"
 #include "forward_model.hpp"
 #include "mesh.hpp"
 #include "sensitivity.hpp"
@@ -64,6 +65,18 @@ py::dict fieldsDict(const ForwardModel& fm) {
     py::dict d;
     d["U"] = arr3(f.U); d["p"] = arr1(f.p); d["k"] = arr1(f.k); d["omega"] = arr1(f.omega);
     d["nuT"] = arr1(f.nuT); d["F1"] = arr1(f.F1); d["F2"] = arr1(f.F2); d["Pk"] = arr1(f.Pk);
+    return d;
+}
+
+// python/Bindings.cpp:247-271: "p" thermodynamic, "p_mechanical" the working variable
+py::dict compFieldsDict(const CompressibleForwardModel& fm) {
+    if (!fm.hasLastFields()) throw std::runtime_error("No fields available — call evaluate() first");
+    const CompressibleFields& F = fm.lastFields();
+    const FlowFields& f = F.flow;
+    py::dict d;
+    d["U"] = arr3(f.U); d["p"] = arr1(f.p); d["p_mechanical"] = arr1(F.pMech); d["T"] = arr1(F.T);
+    d["rho"] = arr1(F.rho); d["k"] = arr1(f.k); d["omega"] = arr1(f.omega); d["nuT"] = arr1(f.nuT);
+    d["F1"] = arr1(f.F1); d["F2"] = arr1(f.F2); d["Pk"] = arr1(f.Pk);
     return d;
 }
 
@@ -309,6 +322,70 @@ PYBIND11_MODULE(rans_sst_py, m) {
         })
         .def("cache_size", [](ForwardModel& fm) { return fm.cache().size(); })
         .def("clear_cache", [](ForwardModel& fm) { fm.cache().clear(); });
+
+    // compressible pressure-based solver (python/Bindings.cpp:585-635)
+    py::class_<IdealGasEOS>(m, "IdealGasEOS")
+        .def(py::init<>())
+        .def_readwrite("gamma", &IdealGasEOS::gamma).def_readwrite("R", &IdealGasEOS::R)
+        .def_readwrite("Pr", &IdealGasEOS::Pr).def_readwrite("Pr_T", &IdealGasEOS::Pr_T)
+        .def_readwrite("mu_ref", &IdealGasEOS::mu_ref).def_readwrite("T_ref", &IdealGasEOS::T_ref)
+        .def_readwrite("S_suth", &IdealGasEOS::S_suth)
+        .def("Cp", &IdealGasEOS::Cp)
+        .def("Cv", &IdealGasEOS::Cv)
+        .def("viscosity", &IdealGasEOS::viscosity, py::arg("T"))
+        .def("conductivity", &IdealGasEOS::conductivity, py::arg("mu"), py::arg("muT"))
+        .def("density", &IdealGasEOS::density, py::arg("p"), py::arg("T"))
+        .def("temperature", &IdealGasEOS::temperature, py::arg("p"), py::arg("rho"))
+        .def("pressure", &IdealGasEOS::pressure, py::arg("rho"), py::arg("T"))
+        .def("sound_speed", &IdealGasEOS::soundSpeed, py::arg("T"))
+        .def("mach_number", &IdealGasEOS::machNumber, py::arg("Umag"), py::arg("T"))
+        .def("total_enthalpy", &IdealGasEOS::totalEnthalpy, py::arg("T"), py::arg("Umag2"));
+
+    py::enum_<TBCType>(m, "TBCType")
+        .value("Dirichlet", TBCType::Dirichlet)
+        .value("Neumann", TBCType::Neumann)
+        .value("AdiabaticWall", TBCType::AdiabaticWall);
+
+    py::class_<PatchTBC>(m, "PatchTBC")
+        .def(py::init<>())
+        .def_readwrite("type", &PatchTBC::type)
+        .def_readwrite("value", &PatchTBC::value)
+        .def_readwrite("patch_name", &PatchTBC::patchName);
+
+    py::class_<CompressibleBoundaryConditions>(m, "CompressibleBoundaryConditions")
+        .def(py::init<>())
+        .def_static("channel_defaults", &CompressibleBoundaryConditions::channelDefaults, py::arg("mesh"),
+                    py::arg("Uin"), py::arg("T_in"), py::arg("p_out"), py::arg("kIn"), py::arg("omIn"))
+        .def_readwrite("temperature_bc", &CompressibleBoundaryConditions::temperatureBC);
+
+    py::class_<CompressibleForwardModel>(m, "CompressibleForwardModel")
+        .def(py::init<const Mesh&, const InferenceParameterSet&, const ObservationOperator&,
+                      const CompressibleBoundaryConditions&, const IdealGasEOS&, const SolverSettings&, const Vec3&,
+                      double, double, double, double>(),
+             py::arg("mesh"), py::arg("param_set"), py::arg("obs_op"), py::arg("bcs"), py::arg("eos"),
+             py::arg("settings") = SolverSettings{}, py::arg("u_init") = Vec3{1.0, 0.0, 0.0},
+             py::arg("p_init") = 101325.0, py::arg("T_init") = 300.0, py::arg("k_init") = 1e-4,
+             py::arg("omega_init") = 100.0, py::keep_alive<1, 2>())
+        .def("evaluate", &CompressibleForwardModel::evaluate, py::call_guard<py::gil_scoped_release>())
+        .def("evaluate_batch", &CompressibleForwardModel::evaluateBatch, py::call_guard<py::gil_scoped_release>(),
+             "Many thetas solved concurrently on the device; each result equals evaluate(theta) alone.")
+        .def("penalized_log_likelihood", &CompressibleForwardModel::penalizedLogLikelihood,
+             py::call_guard<py::gil_scoped_release>())
+        .def("param_set", &CompressibleForwardModel::paramSet, py::return_value_policy::reference)
+        .def("has_last_fields", &CompressibleForwardModel::hasLastFields)
+        .def("last_fields", &compFieldsDict)
+        .def("last_history", [](const CompressibleForwardModel& fm) {
+            const auto& h = fm.lastConvergence().history;
+            py::array_t<double> a({(py::ssize_t)(h.size() / 6), (py::ssize_t)6});
+            std::copy(h.begin(), h.end(), a.mutable_data());
+            return a;
+        }, "Per outer iteration: Ux, Uy, p residuals, T, k, omega change norms (settings.record_history).")
+        .def("last_inner_iterations", [](const CompressibleForwardModel& fm) {
+            const auto& c = fm.lastConvergence();
+            std::vector<double> v(c.innerIters, c.innerIters + 6);
+            v.push_back(fm.lastEnergyIterations());
+            return v;
+        }, "Cumulative inner iterations: Ux, Uy, Uz, p', k, omega, energy.");
 
     // TangentGradientResult / ParameterSensitivity (python/Bindings.cpp:521-576): the
     // forward-path members -- solve_state, observe, log_lik and the warm-FD gradient.

@@ -217,6 +217,51 @@ int rsg_problem_set_observations(rsg_context* ctx, int32_t problem_id, int32_t n
  * in InferenceParameterSet order (core/include/InferenceParameters.hpp:12-14). */
 int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const rsg_init* init,
                int32_t precision, int64_t* ticket);
+
+/* ---- Compressible pressure-based SIMPLE (ideal gas + energy equation) ----
+ * Replaces CompressibleSIMPLESolver (compressible/include/
+ * CompressibleSIMPLESolver.hpp:16-115, compressible/src/
+ * CompressibleSIMPLESolver.cpp:10-1000) behind CompressibleForwardModel::
+ * evaluate (compressible/src/CompressibleForwardModel.cpp:36-97).
+ * Temperature BCs per boundary face (TBCType, CompressibleBCs.hpp:10-25:
+ * Dirichlet; Neumann and AdiabaticWall both zero-gradient; KEEP for a face in
+ * no patch), the gas of IdealGasEOS (IdealGasEOS.hpp:6-15) and alphaT
+ * (SolverSettings::alphaT, core/include/FlowFields.hpp:62).  Velocity,
+ * pressure, k and omega BCs use rsg_bc_desc as for the incompressible solve;
+ * the outlet's prescribed pressure is thermodynamic.  Every compressible
+ * solve runs kernel-per-phase (rsg_set_grid's path, FP64), at most four at
+ * once alongside the member launches. */
+#define RSG_TBC_DIRICHLET 0
+#define RSG_TBC_ZERO_GRADIENT 1
+#define RSG_TBC_KEEP 2
+typedef struct rsg_comp_desc {
+    const int32_t* t_type;    /* [nB] RSG_TBC_* */
+    const double* t_value;    /* [nB] Dirichlet temperature */
+    double gamma, R, Pr, Pr_T, mu_ref, T_ref, S_suth;
+    double alpha_t;           /* temperature under-relaxation, (0, 1] */
+} rsg_comp_desc;
+/* Uniform initial state of CompressibleSIMPLESolver::initUniform
+ * (CompressibleSIMPLESolver.cpp:27-72): p is the THERMODYNAMIC pressure. */
+typedef struct rsg_comp_init {
+    double U[3], p, T, k, omega;
+} rsg_comp_init;
+int rsg_comp_problem_create(rsg_context* ctx, int32_t mesh_id, const rsg_bc_desc* bc, const rsg_comp_desc* comp,
+                            const rsg_settings* settings, int32_t* problem_id);
+/* Enqueue one compressible solve (FP64).  rsg_fetch then returns U, k,
+ * omega, nuT, F1, F2, Pk, CDkw, the boundary values, the observables
+ * (evaluated with nu = mu(T_0) / rho_0, CompressibleForwardModel.cpp:88-94)
+ * and p as the THERMODYNAMIC pressure p_mech - (2/3) rho max(k, 0) (boundary
+ * values mechanical); history entries are (Ux, Uy, p, T, k, omega) per outer
+ * iteration (CompressibleResidualEntry); final_iter counts outer iterations
+ * as CompressibleConvergenceHistory::finalIter. */
+int rsg_comp_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const rsg_comp_init* init,
+                    int64_t* ticket);
+/* A finished compressible solve's temperature, density and mechanical
+ * pressure (cells), temperature boundary values and energy-equation inner
+ * iterations; any pointer may be NULL.  Call before rsg_release. */
+int rsg_comp_fetch(rsg_context* ctx, int64_t ticket, double* T, double* rho, double* p_mech, double* T_bnd,
+                   double* energy_iters);
+
 int rsg_poll(rsg_context* ctx, const int64_t* tickets, int32_t n, int32_t* state_out);
 /* Blocks until every ticket is DONE/FAILED or timeout_s elapses (<0: forever). */
 int rsg_wait(rsg_context* ctx, const int64_t* tickets, int32_t n, double timeout_s);

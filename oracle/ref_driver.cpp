@@ -20,6 +20,7 @@
 #include "SIMPLESolver.hpp"
 #include "StressOperators.hpp"
 #include "ObservationOperator.hpp"
+#include "CompressibleSIMPLESolver.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -478,6 +479,49 @@ double ref_sst_point(int which, const double* coeffs, int variant, const double*
         case 4: return sst.productionOmega(a[0], a[1], a[2], a[3]);
         default: return 0.0;
     }
+}
+
+// CompressibleSIMPLESolver::solve (compressible/src/CompressibleSIMPLESolver.cpp:643-1000)
+// from initUniform on a mesh file with CompressibleBoundaryConditions::channelDefaults.
+// bc = {Uin, T_in, p_out, kIn, omIn}; eos = {gamma, R, Pr, Pr_T, mu_ref, T_ref, S_suth};
+// init = {Ux, Uy, Uz, p, T, k, omega}; alphaT the temperature relaxation.
+// Outputs (nullable): U[3n], p (mechanical), T, rho, k, omega, nuT, F1, F2, Pk [n];
+// hist 6 per iteration (Ux, Uy, p, T, k, omega); info = {finalIter, converged,
+// diverged, entries, turbEstablished}.
+int ref_comp_solve(const char* path, const double* bc, const double* eos, const double* coeffs,
+                   const double* settings, double alphaT, const double* init,
+                   double* U, double* p, double* T, double* rho, double* k, double* om, double* nuT,
+                   double* F1, double* F2, double* Pk, double* hist, int histCap, int* info) {
+    try {
+        Mesh m = loadMesh(path);
+        CompressibleBoundaryConditions bcs =
+            CompressibleBoundaryConditions::channelDefaults(m, bc[0], bc[1], bc[2], bc[3], bc[4]);
+        IdealGasEOS e;
+        e.gamma = eos[0]; e.R = eos[1]; e.Pr = eos[2]; e.Pr_T = eos[3];
+        e.mu_ref = eos[4]; e.T_ref = eos[5]; e.S_suth = eos[6];
+        SolverSettings s = unpackSettings(settings);
+        s.alphaT = alphaT;
+        SSTModel sst(unpackCoeffs(coeffs));
+        sst.variant = static_cast<SSTVariant>(s.sstVariant);
+        CompressibleSIMPLESolver solver(m, sst, bcs, e, s);
+        CompressibleFlowFields f(m);
+        solver.initUniform(f, Vec3(init[0], init[1], init[2]), init[3], init[4], init[5], init[6]);
+        CompressibleConvergenceHistory h = solver.solve(f);
+        const int n = m.nCells();
+        if (U) for (int i = 0; i < n; ++i) { U[3*i] = f.U[i].x; U[3*i+1] = f.U[i].y; U[3*i+2] = f.U[i].z; }
+        copyScalar(f.p, p, nullptr); copyScalar(f.T, T, nullptr); copyScalar(f.rho, rho, nullptr);
+        copyScalar(f.k, k, nullptr); copyScalar(f.omega, om, nullptr); copyScalar(f.nuT, nuT, nullptr);
+        copyScalar(f.F1, F1, nullptr); copyScalar(f.F2, F2, nullptr); copyScalar(f.Pk, Pk, nullptr);
+        const int ne = (int)h.entries.size();
+        if (hist) for (int q = 0; q < ne && q < histCap; ++q) {
+            const auto& r = h.entries[q];
+            hist[6*q] = r.Ux; hist[6*q+1] = r.Uy; hist[6*q+2] = r.p;
+            hist[6*q+3] = r.T; hist[6*q+4] = r.k; hist[6*q+5] = r.omega;
+        }
+        info[0] = h.finalIter; info[1] = h.converged; info[2] = h.diverged; info[3] = ne;
+        info[4] = f.turbEstablished ? 1 : 0;
+        return 0;
+    } catch (const std::exception& ex) { std::fprintf(stderr, "ref: %s\n", ex.what()); return -1; }
 }
 
 }  // extern "C"

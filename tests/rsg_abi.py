@@ -66,6 +66,21 @@ class Result(C.Structure):
                 ("log_lik", C.c_double)]
 
 
+class CompDesc(C.Structure):
+    _fields_ = [("t_type", PI), ("t_value", PD), ("gamma", C.c_double), ("R", C.c_double), ("Pr", C.c_double),
+                ("Pr_T", C.c_double), ("mu_ref", C.c_double), ("T_ref", C.c_double), ("S_suth", C.c_double),
+                ("alpha_t", C.c_double)]
+
+
+class CompInit(C.Structure):
+    _fields_ = [("U", C.c_double * 3), ("p", C.c_double), ("T", C.c_double), ("k", C.c_double),
+                ("omega", C.c_double)]
+
+
+TBC = dict(DIRICHLET=0, ZERO_GRADIENT=1, KEEP=2)
+EOS_AIR = (1.4, 287.0, 0.72, 0.9, 1.716e-5, 273.15, 110.4)
+
+
 class Observation(C.Structure):
     _fields_ = [("type", C.c_int32), ("index", C.c_int32), ("component", C.c_int32),
                 ("face_begin", C.c_int32), ("face_end", C.c_int32), ("ref_velocity", C.c_double),
@@ -217,6 +232,35 @@ class Msh:
         d = BcDesc(iptr(vt), dptr(vv), iptr(pt), dptr(pv), iptr(kt), dptr(kv), iptr(ot), dptr(ov))
         d._keep = keep
         return d
+
+    def comp_bc(self, Uin, T_in, p_out, kIn, omIn, eos=EOS_AIR, alpha_t=0.7):
+        """CompressibleBoundaryConditions::channelDefaults (CompressibleBCs.hpp:27-72)
+        as per-face arrays: (rsg_bc_desc, rsg_comp_desc)."""
+        nB = self.nF - self.nIF
+        vt = np.full(nB, BC["NEUMANN"], np.int32); vv = np.zeros(3 * nB)
+        pt = np.full(nB, BC["NEUMANN"], np.int32); pv = np.zeros(nB)
+        kt = np.full(nB, BC["NEUMANN"], np.int32); kv = np.zeros(nB)
+        ot = np.full(nB, BC["NEUMANN"], np.int32); ov = np.zeros(nB)
+        tt = np.full(nB, TBC["KEEP"], np.int32); tv = np.zeros(nB)
+        for p in range(self.nP):
+            b = self.pfaces[p] - self.nIF
+            t = self.ptypes[p]
+            tt[b] = TBC["ZERO_GRADIENT"]; tv[b] = 300.0
+            if t == "wall":
+                vt[b] = BC["WALL_NOSLIP"]; kt[b] = BC["WALL_KOMEGA"]; ot[b] = BC["WALL_KOMEGA"]
+            elif t == "inlet":
+                vt[b] = BC["INLET_VELOCITY"]
+                for i in b:
+                    vv[3 * i:3 * i + 3] = (Uin, 0, 0)
+                tt[b] = TBC["DIRICHLET"]; tv[b] = T_in
+                kt[b] = BC["DIRICHLET"]; kv[b] = kIn; ot[b] = BC["DIRICHLET"]; ov[b] = omIn
+            elif t == "outlet":
+                pt[b] = BC["OUTLET_PRESSURE"]; pv[b] = p_out
+        d = BcDesc(iptr(vt), dptr(vv), iptr(pt), dptr(pv), iptr(kt), dptr(kv), iptr(ot), dptr(ov))
+        d._keep = [vt, vv, pt, pv, kt, kv, ot, ov]
+        c = CompDesc(iptr(tt), dptr(tv), *eos, alpha_t)
+        c._keep = [tt, tv]
+        return d, c
 
 
 def new_result(n, nB, hist_cap=0):
