@@ -38,6 +38,10 @@
 #include "rsg_grid.h"
 #include "rsg_big.h"
 
+// a member's linear solve on one workgroup (rsg_chunk.hip)
+hipError_t rsg_member_solve_launch(RsgMemberDev* M, int solver, double* x, int amgMode, int maxIter, double tol,
+                                   double* out, hipStream_t s);
+
 namespace rsg {
 namespace big {
 
@@ -308,6 +312,8 @@ inline int cells_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kB
 using namespace rsg;
 using namespace rsg::big;
 
+constexpr int kSolveSlots = 8;   // one-workgroup solves of an outer iteration (memberSolve members)
+
 // Host state of one large member.
 struct RsgBig {
     RsgMemberDev hM{};            // host copy of the member struct (device pointers)
@@ -347,6 +353,11 @@ struct RsgBig {
     RsgCompDev* dQ = nullptr;
     double* compPart = nullptr;
     double compTIters = 0.0;      // energy-equation inner iterations so far
+    // small meshes: every linear solve one workgroup-wide launch of the member
+    // path's solvers (results collected once per outer iteration)
+    bool memberSolve = false;
+    double* solveOut = nullptr;   // [4 * kSolveSlots] device
+    double* hsolve = nullptr;     // pinned host copy
     // outer-iteration state (the device member's Scal, kept on the host)
     RsgScalars sc{};
     std::vector<double> hist;
@@ -878,6 +889,8 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
         Q.zeroB = B->A.take<double>(nB);
         Q.pMech = B->A.take<double>(nC);
         B->compPart = B->A.take<double>((size_t)CQ_COUNT * cells_grid(pm.nC));
+        B->memberSolve = hM.amg.rp != nullptr;   // the workspace carries the member path's AMG arrays
+        B->solveOut = B->A.take<double>(4 * kSolveSlots);
         B->dQ = B->A.take<RsgCompDev>(1);
         if (B->A.err == hipSuccess) B->A.err = hipMemsetAsync(Q.zeroB, 0, 8 * nB, B->s);
         if (B->A.err == hipSuccess)
@@ -885,6 +898,8 @@ RsgBig* rsg_big_create(const RsgMemberDev& hM, RsgMemberDev* dM, const RsgSettin
         if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(B->dQ, &Q, sizeof Q, hipMemcpyHostToDevice, B->s);
     }
     if (B->A.err == hipSuccess) B->A.err = hipHostMalloc((void**)&B->S.hdots, 16 * sizeof(double), hipHostMallocDefault);
+    if (B->A.err == hipSuccess && B->memberSolve)
+        B->A.err = hipHostMalloc((void**)&B->hsolve, 4 * kSolveSlots * sizeof(double), hipHostMallocDefault);
     if (B->A.err == hipSuccess) B->A.err = hipMemcpyAsync(dM, &hM, sizeof hM, hipMemcpyHostToDevice, B->s);
     if (B->A.err == hipSuccess) B->A.err = hipStreamSynchronize(B->s);
     if (B->A.err != hipSuccess) {
@@ -901,6 +916,7 @@ void rsg_big_destroy(RsgBig* B) {
     B->VG.clear(); B->VGf.clear();
     B->DW.release();
     if (B->S.hdots) (void)hipHostFree(B->S.hdots);
+    if (B->hsolve) (void)hipHostFree(B->hsolve);
     hipStream_t s = B->ownStream ? B->s : nullptr;
     delete B;                     // frees the member's work arrays
     if (s) (void)hipStreamDestroy(s);

@@ -195,6 +195,10 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 // kernel-per-phase members run concurrently, at most this many at once
 constexpr size_t kGridParallel = 4;
+// compressible members on meshes up to this many cells solve their linear
+// systems on one workgroup (the member path's solvers) between the grid-wide
+// assembly launches
+constexpr int kCompMemberSolveCells = 4096;
 
 struct rsg_context {
     int device = 0;
@@ -284,10 +288,13 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     const rsg::PrepMesh& pm = ME.prep;
     const int histCap = P.settings.record_history ? P.settings.max_iterations : 0;
     T.grid = T.comp || (ctx->gridMinCells > 0 && pm.nC >= ctx->gridMinCells);
+    // lean workspace (no member-path AMG arrays) unless a compressible member
+    // on a small mesh solves on one workgroup
+    const bool lean = T.grid && !(T.comp && pm.nC <= kCompMemberSolveCells);
     rsg::Carver sz;
     {
         RsgMemberDev tmp;
-        rsg::carve_member(sz, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, tmp, T.grid);
+        rsg::carve_member(sz, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, tmp, lean);
         if (P.nObs > 0) sz.take<double>((size_t)P.nObs + 1);
         if (T.grid) sz.take<RsgMemberDev>(1);
     }
@@ -298,7 +305,7 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     RsgMemberDev& M = T.M;
     memset(&M, 0, sizeof M);
     rsg::Carver cv; cv.base = ctx->arena.base + off;
-    rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M, T.grid);
+    rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M, lean);
     if (P.nObs > 0) { M.obs = P.obsDev; M.pred = cv.take<double>((size_t)P.nObs + 1); }
     T.dM = T.grid ? cv.take<RsgMemberDev>(1) : nullptr;
     M.mesh = ME.devStruct; M.bc = P.bcDev; M.set = P.setDev;

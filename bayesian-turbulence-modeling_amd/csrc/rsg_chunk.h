@@ -151,4 +151,19 @@ __global__ __launch_bounds__(BS) void k_ldu_solve(RsgMemberDev* Mg, int solver, 
     }
 }
 
+// One linear solve of a kernel-per-phase member's current LDU system into x
+// on one workgroup (member_solve_x); out = iterations, final / initial
+// residual, converged.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_member_solve(RsgMemberDev* Mg, int solver, double* x, int amgMode,
+                                                     int maxIter, double tol, double* out) {
+    __shared__ LdsArea L;
+    __shared__ LdsDesc D;
+    rsg::Thr t = make_thr(L, BS);
+    RsgMemberDev* M = stage_descriptors(D, Mg);
+    rsg::SolveRes r = rsg::member_solve_x<cmax_for(BS)>(t, M, solver, x, amgMode, maxIter, tol);
+    unstage_tables(D);
+    if (t.tid == 0) { out[0] = r.iterations; out[1] = r.finalRes; out[2] = r.initialRes; out[3] = r.converged; }
+}
+
 }  // namespace

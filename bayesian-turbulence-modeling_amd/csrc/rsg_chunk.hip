@@ -35,4 +35,10 @@ hipError_t rsg_ldu_launch(RsgMemberDev* M, int solver, int maxIter, double tol, 
     hipLaunchKernelGGL(k_ldu_solve<256>, dim3(1), dim3(256), 0, s, M, solver, maxIter, tol, stats);
     return hipGetLastError();
 }
+// A kernel-per-phase member's linear solve on one 256-thread workgroup.
+hipError_t rsg_member_solve_launch(RsgMemberDev* M, int solver, double* x, int amgMode, int maxIter, double tol,
+                                   double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_member_solve<256>, dim3(1), dim3(256), 0, s, M, solver, x, amgMode, maxIter, tol, out);
+    return hipGetLastError();
+}
 #endif

@@ -8,7 +8,10 @@
 // the order of the reference's face-loop scatter), and every linear solve is
 // rsg_big.hip's solve() on the CSR image of the assembled LDU system
 // (BiCGSTAB / PCG / AMG-PCG / Gauss-Seidel as the settings name them; the
-// energy equation always BiCGSTAB).
+// energy equation always BiCGSTAB).  On a small mesh (a workspace carved with
+// the member path's AMG arrays: memberSolve) each solve is instead one
+// workgroup-wide launch of the member path's solvers on the LDU system, and
+// the host waits once per outer iteration instead of once per dot product.
 //
 // Reference restated: CompressibleSIMPLESolver (compressible/src/
 // CompressibleSIMPLESolver.cpp): initUniform :27-72, updateDensity :125-138,
@@ -20,6 +23,9 @@
 // closure (core/src/SSTModel.cpp:176-203) and omega wall value
 // (core/include/BoundaryCondition.hpp:305-331); the wall-aware transpose
 // stress (core/include/StressOperators.hpp:31-84).
+
+// slots of the one-workgroup solves' results (memberSolve members)
+enum { SLOT_UX = 0, SLOT_UY, SLOT_UZ, SLOT_P, SLOT_T, SLOT_K, SLOT_OM };
 
 // reduction slots of the per-iteration maxima (block partials in B.compPart)
 enum {
@@ -530,13 +536,29 @@ __global__ __launch_bounds__(kBlock) void kc_export(RsgMemberDev* M, RsgCompDev*
 #undef COMP_BFACES
 
 // ---------------------------------------------------------------- host
-// Reads the CQ_COUNT maxima of the partials the iteration's kernels wrote.
+// Reads the CQ_COUNT maxima of the partials the iteration's kernels wrote
+// (and, for a memberSolve member, the results of its solves).
 hipError_t comp_maxima(RsgBig& B, int nb, double* out) {
     hipLaunchKernelGGL(kc_reduce, dim3(1), dim3(kBlock), 0, B.s, nb, (const double*)B.compPart, B.S.dots);
     hipError_t e = hipMemcpyAsync(B.S.hdots, B.S.dots, CQ_COUNT * sizeof(double), hipMemcpyDeviceToHost, B.s);
+    if (e == hipSuccess && B.memberSolve)
+        e = hipMemcpyAsync(B.hsolve, B.solveOut, 4 * kSolveSlots * sizeof(double), hipMemcpyDeviceToHost, B.s);
     if (e == hipSuccess) e = hipStreamSynchronize(B.s);
     for (int q = 0; q < CQ_COUNT; ++q) out[q] = B.S.hdots[q];
     return e;
+}
+
+// The iteration's linear solve into x: on the grid (synchronous, res filled)
+// or, for a memberSolve member, one workgroup-wide launch whose result lands
+// in `slot` (res filled after comp_maxima).
+hipError_t comp_solve(RsgBig& B, int code, double* x, bool amgReady, int amgMode, int slot, SolveRes& res) {
+    if (!B.memberSolve) return solve(B, code, x, amgReady, res, false);
+    const RsgSettingsDev& st = B.set;
+    return rsg_member_solve_launch(B.dM, code, x, amgMode, st.innerIter, st.innerTol, B.solveOut + 4 * slot, B.s);
+}
+inline void slot_result(const RsgBig& B, int slot, SolveRes& r) {
+    const double* o = B.hsolve + 4 * slot;
+    r.iterations = (int)o[0]; r.finalRes = o[1]; r.initialRes = o[2]; r.converged = (int)o[3];
 }
 
 inline bool finite_res(const SolveRes& r) { return std::isfinite(r.initialRes) && std::isfinite(r.finalRes); }
@@ -570,13 +592,22 @@ hipError_t comp_outer_iteration(RsgBig& B) {
         hipLaunchKernelGGL(kc_divu, gc, bl, 0, B.s, dM, dQ);
         hipLaunchKernelGGL(kc_gtr, gc, bl, 0, B.s, dM, dQ);
         hipLaunchKernelGGL(kc_asm_mom, gc, bl, 0, B.s, dM, dQ, comp);
-        e = solve(B, st.mSolver, ucompv[comp], false, rU[comp], false);
+        e = comp_solve(B, st.mSolver, ucompv[comp], false, 0, SLOT_UX + comp, rU[comp]);
     }
     hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, dM, 1, 0.0);
     // 4. pressure correction (density-weighted)
     hipLaunchKernelGGL(kc_asm_pc, gc, bl, 0, B.s, dM, dQ);
     bool amgReady = false;
-    if (e == hipSuccess && st.pSolver == 0) {
+    int amgMode = 0;
+    if (e == hipSuccess && st.pSolver == 0 && B.memberSolve) {
+        // the member path's device AMG (aggregated on one thread of the solve)
+        const bool rebuild = sc.amgBuiltIter < 0 || iter - sc.amgBuiltIter >= st.amgRefresh;
+        const bool coarse = rebuild || iter - sc.coarseIter >= st.coarseRefresh;
+        amgMode = (sc.amgBuiltIter < 0 ? 4 : 0) | (rebuild ? 1 : 0) | (coarse ? 2 : 0);
+        if (rebuild) sc.amgBuiltIter = iter;
+        if (coarse) sc.coarseIter = iter;
+        amgReady = true;
+    } else if (e == hipSuccess && st.pSolver == 0) {
         const bool rebuild = sc.amgBuiltIter < 0 || iter - sc.amgBuiltIter >= st.amgRefresh || !B.haveH;
         const bool coarse = rebuild || iter - sc.coarseIter >= st.coarseRefresh;
         if (rebuild) {
@@ -591,7 +622,7 @@ hipError_t comp_outer_iteration(RsgBig& B) {
         amgReady = true;
     }
     SolveRes rP{0, 0.0, 0.0, 0};
-    if (e == hipSuccess) e = solve(B, st.pSolver, M.pp, amgReady, rP, false);
+    if (e == hipSuccess) e = comp_solve(B, st.pSolver, M.pp, amgReady, amgMode, SLOT_P, rP);
     hipLaunchKernelGGL(k_ppb, gc, bl, 0, B.s, dM);
     hipLaunchKernelGGL(k_grad_pp, gc, bl, 0, B.s, dM);
     hipLaunchKernelGGL(k_correct, gc, bl, 0, B.s, dM);
@@ -599,18 +630,18 @@ hipError_t comp_outer_iteration(RsgBig& B) {
     // 5. energy (BiCGSTAB), T >= 1, temperature BC, density
     hipLaunchKernelGGL(kc_asm_e, gc, bl, 0, B.s, dM, dQ, B.compPart);
     SolveRes rT{0, 0.0, 0.0, 0};
-    if (e == hipSuccess) e = solve(B, 2, Q.T, false, rT, false);
+    if (e == hipSuccess) e = comp_solve(B, 2, Q.T, false, 0, SLOT_T, rT);
     hipLaunchKernelGGL(kc_tpost, gc, bl, 0, B.s, dM, dQ, B.compPart);
     hipLaunchKernelGGL(kc_state, gc, bl, 0, B.s, dM, dQ, 0, 1, 1);
     // 6. turbulence
     SolveRes rK{0, 0.0, 0.0, 0}, rOm{0, 0.0, 0.0, 0};
     if (turbUpdate && e == hipSuccess) {
         hipLaunchKernelGGL(kc_asm_turb, gc, bl, 0, B.s, dM, dQ, 0);
-        e = solve(B, st.tSolver, M.k, false, rK, false);
+        e = comp_solve(B, st.tSolver, M.k, false, 0, SLOT_K, rK);
         hipLaunchKernelGGL(k_clamp, gc, bl, 0, B.s, dM, 0);
         hipLaunchKernelGGL(k_bcs, gc, bl, 0, B.s, dM, 4, 0.0);
         hipLaunchKernelGGL(kc_asm_turb, gc, bl, 0, B.s, dM, dQ, 1);
-        if (e == hipSuccess) e = solve(B, st.tSolver, M.om, false, rOm, false);
+        if (e == hipSuccess) e = comp_solve(B, st.tSolver, M.om, false, 0, SLOT_OM, rOm);
         hipLaunchKernelGGL(k_clamp, gc, bl, 0, B.s, dM, 1);
         hipLaunchKernelGGL(kc_ombc, gc, bl, 0, B.s, dM, dQ, M.coeffs[2]);
         hipLaunchKernelGGL(kc_pin, gc, bl, 0, B.s, dM, dQ, B.compPart);
@@ -621,6 +652,13 @@ hipError_t comp_outer_iteration(RsgBig& B) {
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = comp_maxima(B, nb, mx);
     if (e != hipSuccess) return e;
+    if (B.memberSolve) {
+        for (int c = 0; c < 3; ++c)
+            if (!(c == 2 && M.planarZ)) slot_result(B, SLOT_UX + c, rU[c]);
+        slot_result(B, SLOT_P, rP);
+        slot_result(B, SLOT_T, rT);
+        if (turbUpdate) { slot_result(B, SLOT_K, rK); slot_result(B, SLOT_OM, rOm); }
+    }
     const double uChange = mx[CQ_UDIFF] / mx[CQ_UVAL];
     const double tChange = mx[CQ_TDIFF] / mx[CQ_TVAL];
     if (turbUpdate) {

@@ -1950,6 +1950,55 @@ RSG_D SolveRes ldu_solve_member(Thr t, RsgMemberDev* M, int solver, int maxIter,
     return r;
 }
 
+// One linear solve of the member's current LDU system into x (in/out) on one
+// workgroup: the member path's solvers, for a kernel-per-phase member whose
+// mesh is small enough that a grid-wide Krylov iteration (a host round trip
+// per dot product) costs more than one workgroup's.  AMG (solver 0):
+// amgMode bit 4 copies the level-0 pattern (first use), bit 1 aggregates the
+// hierarchy anew, bit 2 refreshes its coarse values and the coarsest
+// inverse; otherwise the LDS copy of the coarsest inverse is restored.
+template <int CMAX>
+RSG_D SolveRes member_solve_x(Thr t, RsgMemberDev* M, int solver, double* x, int amgMode, int maxIter,
+                              double tol) {
+    Ctx C;
+    ctx_init(C, t, M);
+    bool amg = false;
+    if (solver == 0) {
+        RsgAmgDev& A = M->amg;
+        const RsgMeshDev* m = C.m;
+        if (amgMode & 4) {
+            for (int i = t.tid; i <= m->nC; i += t.bs) A.rp[i] = m->rp0[i];
+            for (int e = t.tid; e < m->nnz0; e += t.bs) A.ci[e] = m->ci0[e];
+            if (t.tid == 0) { A.lvN[0] = m->nC; A.lvRow[0] = 0; A.lvRp[0] = 0; A.lvNz[0] = 0; A.lvN[RSG_MAX_LEVELS] = 1; }
+            RSG_SYNC();
+        }
+        const bool rebuild = (amgMode & 1) != 0, coarse = rebuild || (amgMode & 2) != 0;
+        if (rebuild) {
+            amg_refresh_values(C, false);
+            if (t.tid == 0) amg_setup_serial(m, A);
+            RSG_SYNC();
+        } else {
+            const bool resident = res_fits(C, CMAX) && res_amg_fits(C);
+            if (coarse || !resident) amg_refresh_values(C, coarse);
+        }
+        if (coarse) {
+            amg_coarse_inverse(C);
+        } else if (M->sc->coarseDense) {
+            const int nL = A.lvN[A.lvN[RSG_MAX_LEVELS] - 1];
+            for (int idx = t.tid; idx < nL * RSG_COARSE_DENSE; idx += t.bs) t.lds[idx] = A.cinv[idx];
+            RSG_SYNC();
+        }
+        amg = true;
+    }
+    SolveRes r;
+    if (solver == RSG_SOLVER_GAUSS_SEIDEL) r = gauss_seidel(C, x, maxIter, tol);
+    else if (res_fits(C, CMAX) && (!amg || res_amg_fits(C))) r = lsolve_res<CMAX>(C, solver, x, amg, maxIter, tol, false);
+    else if (solver == 2) r = bicgstab(C, x, maxIter, tol);
+    else r = pcg(C, x, maxIter, tol, amg);
+    RSG_SYNC();
+    return r;
+}
+
 #undef FOR_CELLS
 #undef FOR_IFACES
 #undef FOR_BFACES

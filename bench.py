@@ -489,6 +489,103 @@ def bench_c0(args):
         "roofline": None, "cpu_baseline": cpu}), flush=True)
 
 
+def bench_comp(args):
+    """Compressible pressure-based SIMPLE (kernel per phase, FP64) on the
+    reference's Ma 0.1 regression channel (40 x 30, its own settings):
+    single-solve latency and a batch of theta samples solved concurrently,
+    beside the reference solver (oracle/_ref/libref.so, one core) on the
+    same case; then the outer-iteration time on a finer channel of the same
+    family (--comp-cells), with the reference timed over a bounded sample of
+    its iterations."""
+    import ctypes as C
+    import rans_sst_py as rs
+    import compressible_diagnostics as cd
+    D = lambda a: (C.c_double * len(a))(*a)
+    case = cd.make_validation_case("ma_0_1", 0.1, nx=40, ny=30, max_iterations=3000)
+    fm = case["forward"]
+    theta0 = list(case["param_set"].pack(rs.SSTCoefficients()))
+    lat, iters, conv = [], 0, 0
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        r = fm.evaluate(theta0)
+        dt = time.perf_counter() - t0
+        print("bench: comp solve %.2f s (%d iterations, %s)" % (dt, r.simple_iters, r.status), file=sys.stderr,
+              flush=True)
+        if k >= args.warmup:
+            lat.append(dt); iters = r.simple_iters
+            conv += str(r.status).split(".")[-1] == "Converged"
+    summ = cd.summarise_fields(fm.last_fields(), case["eos"])
+    rng = np.random.default_rng(args.seed)
+    thetas = [[0.31 * (1 + 0.1 * rng.uniform(-1, 1)), 0.09 * (1 + 0.1 * rng.uniform(-1, 1))]
+              for _ in range(args.comp_members)]
+    t0 = time.perf_counter()
+    res = fm.evaluate_batch(thetas)
+    bwall = time.perf_counter() - t0
+    bconv = sum(str(r.status).split(".")[-1] == "Converged" for r in res)
+    print("bench: comp batch of %d: %.2f s, %d converged" % (len(thetas), bwall, bconv), file=sys.stderr, flush=True)
+    batch = {"members": len(thetas), "wall_s": bwall, "converged": int(bconv), "solves_per_s": bconv / bwall,
+             "outer_iters_median": float(np.median([r.simple_iters for r in res]))}
+
+    def ref_comp(path, s_packed, maxit, Uin, kIn, omIn):
+        L = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
+        info = (C.c_int * 5)()
+        t0 = time.perf_counter()
+        rc = L.ref_comp_solve(path.encode(), D([Uin, 300.0, 101325.0, kIn, omIn]),
+                              D([1.4, 287.0, 0.72, 0.9, 1.716e-5, 273.15, 110.4]),
+                              D([0.85, 0.5, 0.075, 5.0 / 9.0, 1.0, 0.856, 0.0828, 0.44, 0.09, 0.31, 0.41]),
+                              D(s_packed), C.c_double(0.7), D([Uin, 0, 0, 101325.0, 300.0, kIn, omIn]),
+                              *([None] * 10), None, 0, info)
+        return rc, time.perf_counter() - t0, list(info)
+
+    def packed(s, maxit):
+        return [maxit, s.convergence_tol, s.divergence_limit, s.alpha_u, s.alpha_p, s.alpha_k, s.alpha_omega,
+                s.inner_iterations, s.inner_tolerance, s.turb_start_iter, s.turb_update_interval, s.nut_floor_iters,
+                0, 0.0, 0.0, 0.0, s.k_min, s.k_max, s.omega_min, 0, s.von_karman, 0, 0, 2, 2]
+
+    cpu = None
+    have_ref = os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
+    if not args.no_cpu and have_ref:
+        path = "/tmp/bench_comp40.msh"
+        case["mesh"].save_binary(path)
+        kIn = 1.5 * (case["Uin"] * 0.05) ** 2
+        omIn = kIn / (case["mu_in"] / case["rho_in"] * 100.0)
+        rc, dt, info = ref_comp(path, packed(case["settings"], 3000), 3000, case["Uin"], kIn, omIn)
+        cpu = {"value": (1.0 / dt) if info[1] else 0.0, "unit": "solves/s", "cores": 1, "kind": "reference",
+               "sample": "one full solve of the 40 x 30 case by the reference solver: %d iterations, converged %d, "
+                         "%.1f s" % (info[0], info[1], dt), "lscpu": lscpu_info()}
+    # the finer channel: outer-iteration time, device against the reference
+    nx = int(round(math.sqrt(args.comp_cells * 4.0 / 3.0)))
+    ny = max(2, int(round(args.comp_cells / nx)))
+    fine = cd.make_validation_case("ma_0_1_fine", 0.1, nx=nx, ny=ny, max_iterations=args.comp_fine_iters,
+                                   convergence_tol=1e-12)
+    t0 = time.perf_counter()
+    rf = fine["forward"].evaluate(theta0)
+    fwall = time.perf_counter() - t0
+    fine_out = {"cells": nx * ny, "status": str(rf.status).split(".")[-1], "outer_iterations": int(rf.simple_iters),
+                "wall_s": fwall,
+                "ms_per_outer_iteration": 1e3 * fwall / max(rf.simple_iters, 1), "reference_ms_per_outer_iteration": None}
+    if not args.no_cpu and have_ref:
+        path = "/tmp/bench_comp_fine.msh"
+        fine["mesh"].save_binary(path)
+        kIn = 1.5 * (fine["Uin"] * 0.05) ** 2
+        omIn = kIn / (fine["mu_in"] / fine["rho_in"] * 100.0)
+        sample = max(5, min(args.comp_fine_iters, 60))
+        rc, dt, info = ref_comp(path, packed(fine["settings"], sample), sample, fine["Uin"], kIn, omIn)
+        fine_out["reference_ms_per_outer_iteration"] = 1e3 * dt / max(info[0], 1)
+        fine_out["reference_sample"] = "%d outer iterations, one core, %.1f s" % (info[0], dt)
+    print("bench: comp fine %s" % fine_out, file=sys.stderr, flush=True)
+    t = float(np.median(lat))
+    print(json.dumps({
+        "metric": "compressible converged solves/s", "value": conv / sum(lat), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "compressible Ma 0.1 channel 40x30 (the reference's regression case), pressure-based "
+                               "SIMPLE + SST + energy to its convergence_tol 1e-3, one solve at a time",
+                   "cells": 1200},
+        "p50_latency_s": t, "outer_iterations": int(iters), "converged": conv, "summary": summ, "batch": batch,
+        "fine_channel": fine_out, "roofline": None, "cpu_baseline": cpu}), flush=True)
+
+
 def bench_walldist(args):
     """Mesh::computeWallDistance (core/src/Mesh.cpp:345-373) at the 2M-cell
     size: a 2500 x 800 wall-bounded channel (1.0 x 0.3, wall-clustered), every
@@ -1026,7 +1123,11 @@ def main():
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--c4-members", type=int, default=65, help="c4 workload: ensemble members")
     ap.add_argument("--c4-iters", type=int, default=40, help="c4 workload: outer iterations per member and precision")
-    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump", "c4"), default="c2",
+    ap.add_argument("--comp-members", type=int, default=16, help="comp workload: theta samples in the batch")
+    ap.add_argument("--comp-cells", type=int, default=76800, help="comp workload: cells of the finer channel")
+    ap.add_argument("--comp-fine-iters", type=int, default=200,
+                    help="comp workload: outer iterations timed on the finer channel")
+    ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump", "c4", "comp"), default="c2",
                     help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
                          "c0: single-solve latency of the Re_tau 550 channel; walldist: the wall-distance "
                          "kernel on a 2M-cell mesh; c4p: the pressure system at 2M cells; bump: team solves "
@@ -1055,7 +1156,7 @@ def main():
               "exchange between them, and runs on several GPUs are not part of this benchmark", file=sys.stderr)
         sys.exit(2)
     {"c1": bench_c1, "c0": bench_c0, "walldist": bench_walldist, "bump": bench_bump,
-     "c4p": lambda a: bench_c1(a, big=True), "c2": bench_c2, "c4": bench_c4}[args.workload](args)
+     "c4p": lambda a: bench_c1(a, big=True), "c2": bench_c2, "c4": bench_c4, "comp": bench_comp}[args.workload](args)
 
 
 if __name__ == "__main__":
