@@ -501,6 +501,7 @@ def bench_comp(args):
     import rans_sst_py as rs
     import compressible_diagnostics as cd
     D = lambda a: (C.c_double * len(a))(*a)
+    rs.engine_set_graphs(bool(args.graphs))
     case = cd.make_validation_case("ma_0_1", 0.1, nx=40, ny=30, max_iterations=3000)
     fm = case["forward"]
     theta0 = list(case["param_set"].pack(rs.SSTCoefficients()))
@@ -1125,7 +1126,7 @@ def main():
     ap.add_argument("--c4-iters", type=int, default=40, help="c4 workload: outer iterations per member and precision")
     ap.add_argument("--comp-members", type=int, default=16, help="comp workload: theta samples in the batch")
     ap.add_argument("--comp-cells", type=int, default=76800, help="comp workload: cells of the finer channel")
-    ap.add_argument("--comp-fine-iters", type=int, default=200,
+    ap.add_argument("--comp-fine-iters", type=int, default=100,
                     help="comp workload: outer iterations timed on the finer channel")
     ap.add_argument("--workload", choices=("c2", "c1", "c0", "walldist", "c4p", "bump", "c4", "comp"), default="c2",
                     help="c2: the 512-member ensemble (default); c1: the 261,600-cell pressure system; "
