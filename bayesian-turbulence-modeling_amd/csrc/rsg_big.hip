@@ -949,7 +949,8 @@ int rsg_big_run(RsgBig* B, int iters, RsgScalars& out, std::string& err) {
             e = hipMemcpyAsync(B->hM.hist, B->hist.data(), 8 * B->hist.size(), hipMemcpyHostToDevice, B->s);
         if (e == hipSuccess) e = hipMemcpyAsync(B->hM.sc, &B->sc, sizeof B->sc, hipMemcpyHostToDevice, B->s);
         if (e == hipSuccess && B->comp) {
-            hipLaunchKernelGGL(kc_export, dim3(cells_grid(B->nC)), dim3(kBlock), 0, B->s, B->dM, B->dQ, B->nC);
+            hipLaunchKernelGGL(kc_export, dim3(cells_grid(B->nC)), dim3(kBlock), 0, B->s, B->dM, B->dQ);
+            hipLaunchKernelGGL(kc_export_nu, dim3(1), dim3(64), 0, B->s, B->dM, B->dQ);
             e = hipGetLastError();
         }
         if (e == hipSuccess && B->hM.obs) {

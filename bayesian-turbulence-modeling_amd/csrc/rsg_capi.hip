@@ -79,6 +79,8 @@ int rsg_chunk_occupancy_512_0();
 int rsg_chunk_occupancy_512_1();
 hipError_t rsg_ldu_launch(RsgMemberDev* M, int solver, int maxIter, double tol, double* stats, hipStream_t s);
 
+hipError_t rsg_comp_chunk_launch(int grid, RsgMemberDev* members, RsgScalars* statusOut, hipStream_t s);
+int rsg_comp_chunk_occupancy();
 static hipError_t rsg_chunk_launch(int bs, int prec, int grid, RsgMemberDev* members, RsgScalars* statusOut,
                                    hipStream_t s) {
     if (bs == 512) return prec ? rsg_chunk_launch_512_1(grid, members, statusOut, s)
@@ -182,6 +184,8 @@ struct Ticket {
     RsgBig* big = nullptr;
     // compressible solve: initial T / rho / mu, results T, rho, p_mech (cells), T (boundary)
     bool comp = false;
+    RsgCompDev compDev{};            // member-kernel compressible solve: its arrays (arena) and struct
+    RsgCompDev* dComp = nullptr;
     double compT0 = 0, compRho0 = 0, compMu0 = 0;
     std::vector<double> compCells, compTb;
     double compEnergyIters = 0;
@@ -195,9 +199,10 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 // kernel-per-phase members run concurrently, at most this many at once
 constexpr size_t kGridParallel = 4;
-// compressible members on meshes up to this many cells solve their linear
-// systems on one workgroup (the member path's solvers) between the grid-wide
-// assembly launches
+// compressible members on meshes up to this many cells run on the member
+// kernel (one workgroup each, k_comp_chunk), larger ones kernel-per-phase;
+// on the kernel-per-phase path (rsg_set_grid) members up to this size solve
+// their linear systems on one workgroup between the grid-wide assembly launches
 constexpr int kCompMemberSolveCells = 4096;
 
 struct rsg_context {
@@ -232,6 +237,7 @@ struct rsg_context {
     RsgMemberDev* dTeamM = nullptr;
     RsgScalars* dTeamOut = nullptr;
     int resident256 = 0, resident512 = 0;   // member workgroups the device holds at once, per shape
+    int residentComp = 0;                    // compressible member workgroups (k_comp_chunk)
     size_t rot = 0;          // first active member of the next launch (rotating window)
     int schedule = RSG_SCHEDULE_ROUND_ROBIN;
     int maxSlots = 4096;
@@ -281,13 +287,26 @@ void member_init_scalars(RsgScalars& s, int turbEstablished) {
     s.turbEstablished = turbEstablished;
 }
 
+// The compressible fields of a member-kernel solve (RsgCompDev's arrays).
+void carve_comp(rsg::Carver& cv, int nC, int nB, RsgCompDev& Q) {
+    const size_t n = nC, nb = std::max(nB, 1);
+    Q.T = cv.take<double>(n); Q.Tb = cv.take<double>(nb); Q.rho = cv.take<double>(n);
+    Q.mu = cv.take<double>(n); Q.Told = cv.take<double>(n);
+    Q.Ux0 = cv.take<double>(n); Q.Uy0 = cv.take<double>(n); Q.Uz0 = cv.take<double>(n);
+    Q.mdu = cv.take<double>(n);
+    for (int q = 0; q < 3; ++q) Q.gt[q] = cv.take<double>(n);
+    Q.zeroB = cv.take<double>(nb);
+    Q.pMech = cv.take<double>(n);
+}
+
 // Admit one ticket (ctx->mu held).  Returns false when the arena is full.
 bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     ProblemEntry& P = *ctx->problems[T.problemId];
     MeshEntry& ME = *ctx->meshes[P.meshId];
     const rsg::PrepMesh& pm = ME.prep;
     const int histCap = P.settings.record_history ? P.settings.max_iterations : 0;
-    T.grid = T.comp || (ctx->gridMinCells > 0 && pm.nC >= ctx->gridMinCells);
+    T.grid = (T.comp && pm.nC > kCompMemberSolveCells) || (ctx->gridMinCells > 0 && pm.nC >= ctx->gridMinCells);
+    const bool compMember = T.comp && !T.grid;
     // lean workspace (no member-path AMG arrays) unless a compressible member
     // on a small mesh solves on one workgroup
     const bool lean = T.grid && !(T.comp && pm.nC <= kCompMemberSolveCells);
@@ -297,6 +316,7 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
         rsg::carve_member(sz, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, tmp, lean);
         if (P.nObs > 0) sz.take<double>((size_t)P.nObs + 1);
         if (T.grid) sz.take<RsgMemberDev>(1);
+        if (compMember) { RsgCompDev q; carve_comp(sz, pm.nC, pm.nB, q); sz.take<RsgCompDev>(1); }
     }
     size_t bytes = sz.off;
     size_t off;
@@ -308,6 +328,14 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     rsg::carve_member(cv, pm.nC, pm.nIF, pm.nB, pm.nnz0, histCap, M, lean);
     if (P.nObs > 0) { M.obs = P.obsDev; M.pred = cv.take<double>((size_t)P.nObs + 1); }
     T.dM = T.grid ? cv.take<RsgMemberDev>(1) : nullptr;
+    if (compMember) {
+        RsgCompDev& Q = T.compDev;
+        Q = P.compDev;
+        Q.Tinit = T.compT0; Q.rho0 = T.compRho0; Q.mu0 = T.compMu0; Q.energyIters = 0.0;
+        carve_comp(cv, pm.nC, pm.nB, Q);
+        T.dComp = cv.take<RsgCompDev>(1);
+        M.comp = T.dComp;
+    }
     M.mesh = ME.devStruct; M.bc = P.bcDev; M.set = P.setDev;
     M.nu = T.comp ? T.compMu0 / T.compRho0 : P.nu;   // compressible: the initial mu / rho
     for (int i = 0; i < 11; ++i) M.coeffs[i] = T.coeffs[i];
@@ -320,6 +348,12 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
     RsgScalars s;
     member_init_scalars(s, T.warm ? T.warmTurb : 0);
     hipError_t e = hipMemcpyAsync(M.sc, &s, sizeof s, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && compMember) {
+        e = hipMemsetAsync(T.compDev.zeroB, 0, 8 * (size_t)std::max(pm.nB, 1), ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(T.dComp, &T.compDev, sizeof(RsgCompDev), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);   // T.compDev is the source
+    }
     if (e == hipSuccess && T.warm) {
         const int n = pm.nC, nB = pm.nB;
         double* cdst[] = {M.Ux, M.Uy, M.Uz, M.p, M.k, M.om, M.nuT, M.F1, M.F2, M.Pk, M.CDkw};
@@ -376,6 +410,23 @@ hipError_t collect(Ticket& T, const rsg::PrepMesh& pm, const rsg_settings& st, i
         T.pred.resize((size_t)nObs + 1);
         e = hipMemcpy(T.pred.data(), M.pred, 8 * T.pred.size(), hipMemcpyDeviceToHost);
     }
+    return e;
+}
+
+// A finished member-kernel compressible solve's T, rho, p_mech, T boundary
+// values and energy-equation iterations to host (ctx->mu not held).
+hipError_t collect_comp(Ticket& T, const rsg::PrepMesh& pm) {
+    const size_t n = pm.nC, nB = pm.nB;
+    const RsgCompDev& Q = T.compDev;
+    T.compCells.assign(3 * n, 0.0);
+    T.compTb.assign(std::max<size_t>(nB, 1), 0.0);
+    hipError_t e = hipMemcpy(T.compCells.data(), Q.T, 8 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(T.compCells.data() + n, Q.rho, 8 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(T.compCells.data() + 2 * n, Q.pMech, 8 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nB > 0) e = hipMemcpy(T.compTb.data(), Q.Tb, 8 * nB, hipMemcpyDeviceToHost);
+    RsgCompDev d;
+    if (e == hipSuccess) e = hipMemcpy(&d, T.dComp, sizeof d, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) T.compEnergyIters = d.energyIters;
     return e;
 }
 
@@ -481,11 +532,15 @@ void engine_loop(rsg_context* ctx) {
                                  : std::max(kTeamMinBlocks, (int)std::lround(kTeamBlocksAt40k * std::sqrt(nC / 40000.0)));
             teamNb[q] = std::min(want, ctx->teamCap);
         }
-        // FP64 members first, then mixed ones: each group runs the kernel
-        // built for its precision (one launch each, back to back)
-        std::stable_partition(batch.begin(), batch.end(), [&](int64_t id) { return ctx->tickets[id]->precision == 0; });
-        int n64 = 0;
-        for (int64_t id : batch) n64 += ctx->tickets[id]->precision == 0;
+        // FP64 members first, then mixed ones, then compressible ones: each
+        // group runs its own kernel (one launch each, back to back)
+        auto group = [&](int64_t id) {
+            const Ticket& T = *ctx->tickets[id];
+            return T.comp ? 2 : (T.precision == 0 ? 0 : 1);
+        };
+        std::stable_sort(batch.begin(), batch.end(), [&](int64_t a, int64_t b) { return group(a) < group(b); });
+        int n64 = 0, nmix = 0;
+        for (int64_t id : batch) { n64 += group(id) == 0; nmix += group(id) == 1; }
         ctx->hMembers.resize(nr);
         ctx->hStatus.resize(nr);
         for (int i = 0; i < nr; ++i) {
@@ -573,8 +628,11 @@ void engine_loop(rsg_context* ctx) {
             hipEventRecord(ctx->ev0, ctx->stream);
             hipError_t el = hipSuccess;
             if (n64 > 0) el = rsg_chunk_launch(bs, 0, n64, ctx->dMembers, ctx->dStatus, ctx->stream);
-            if (el == hipSuccess && nr > n64)
-                el = rsg_chunk_launch(bs, 1, nr - n64, ctx->dMembers + n64, ctx->dStatus + n64, ctx->stream);
+            if (el == hipSuccess && nmix > 0)
+                el = rsg_chunk_launch(bs, 1, nmix, ctx->dMembers + n64, ctx->dStatus + n64, ctx->stream);
+            if (el == hipSuccess && nr > n64 + nmix)
+                el = rsg_comp_chunk_launch(nr - n64 - nmix, ctx->dMembers + n64 + nmix, ctx->dStatus + n64 + nmix,
+                                           ctx->stream);
             hipEventRecord(ctx->ev1, ctx->stream);
             if (e == hipSuccess) e = el;
             if (e == hipSuccess)
@@ -627,6 +685,7 @@ void engine_loop(rsg_context* ctx) {
             if (T.sc.status != RSGM_RUNNING) {
                 ProblemEntry& P = *ctx->problems[T.problemId];
                 e = collect(T, ctx->meshes[P.meshId]->prep, P.settings, P.nObs);
+                if (e == hipSuccess && T.comp) e = collect_comp(T, ctx->meshes[P.meshId]->prep);
                 done.push_back(batch[i]);
             }
         }
@@ -685,6 +744,7 @@ int rsg_context_create(int32_t device, uint64_t hbm_budget_bytes, rsg_context** 
     }
     c->resident256 = rsg_chunk_occupancy(256) * c->cuCount;
     c->resident512 = rsg_chunk_occupancy(512) * c->cuCount;
+    c->residentComp = rsg_comp_chunk_occupancy() * c->cuCount;
     if (hbm_budget_bytes == 0) hbm_budget_bytes = 8ull << 30;
     if (hipMalloc(&c->arena.base, hbm_budget_bytes) != hipSuccess) return RSG_ERR_NOMEM;
     c->arena.size = hbm_budget_bytes;

@@ -151,6 +151,20 @@ __global__ __launch_bounds__(BS) void k_ldu_solve(RsgMemberDev* Mg, int solver, 
     }
 }
 
+// Compressible members (M->comp set), one workgroup each, like
+// k_simple_chunk: persistent over the launch's iteration budget or slice.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_comp_chunk(RsgMemberDev* members, RsgScalars* statusOut) {
+    __shared__ typename LdsFor<BS>::type L;
+    __shared__ LdsDesc D;
+    rsg::Thr t = make_thr(L, BS);
+    RsgMemberDev* M = stage_descriptors(D, members + blockIdx.x);
+    rsg::run_comp_member<cmax_for(BS)>(t, M);
+    unstage_tables(D);
+    __syncthreads();
+    if (threadIdx.x == 0) statusOut[blockIdx.x] = *M->sc;
+}
+
 // One linear solve of a kernel-per-phase member's current LDU system into x
 // on one workgroup (member_solve_x); out = iterations, final / initial
 // residual, converged.

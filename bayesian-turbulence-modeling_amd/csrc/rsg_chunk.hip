@@ -42,3 +42,16 @@ hipError_t rsg_member_solve_launch(RsgMemberDev* M, int solver, double* x, int a
     return hipGetLastError();
 }
 #endif
+
+#if RSG_CHUNK_BS == 512 && RSG_CHUNK_PREC == 0
+// Compressible members: 512-thread workgroups, FP64.
+hipError_t rsg_comp_chunk_launch(int grid, RsgMemberDev* members, RsgScalars* statusOut, hipStream_t s) {
+    hipLaunchKernelGGL(k_comp_chunk<512>, dim3(grid), dim3(512), 0, s, members, statusOut);
+    return hipGetLastError();
+}
+int rsg_comp_chunk_occupancy() {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_comp_chunk<512>, 512, 0) != hipSuccess) return 0;
+    return per;
+}
+#endif

@@ -198,6 +198,7 @@ struct RsgCompDev {
     double *gt[3];         // grad (muEff div U)
     double *zeroB;         // [nB] zeros (boundary values of muEff div U)
     double *pMech;         // mechanical pressure of a finished solve (p then holds the thermodynamic one)
+    double energyIters;    // energy-equation inner iterations so far (member kernel)
 };
 
 struct RsgMemberDev {
@@ -228,4 +229,5 @@ struct RsgMemberDev {
     RsgScalars* sc;
     const RsgObsDev* obs;  // observables of the problem (null: none)
     double* pred;          // [obs->n + 1]: predictions, then the log-likelihood
+    RsgCompDev* comp;      // compressible member on the member kernel (null: incompressible)
 };

@@ -1508,12 +1508,14 @@ RSG_D void init_uniform(Ctx& C) {
 
 // Step 1 of the outer iteration for cell c: Green-Gauss gradients of U, k,
 // omega, |S| and the SST closure (computeFields, SSTModel.cpp:122-172), with
-// the startup eddy-viscosity floor nuTMin.
-RSG_D void sst_closure_cell(Ctx& C, int c, double nuTMin) {
+// the startup eddy-viscosity floor nuTMin.  kbGrad: boundary values of k for
+// its gradient (null: the member's kb; the compressible solver hands the
+// closure a copy of k whose boundary values are zero).
+RSG_D void sst_closure_cell(Ctx& C, int c, double nuTMin, const double* kbGrad = nullptr) {
     const RsgMeshDev* m = C.m; RsgMemberDev* M = C.M; const RsgSettingsDev* st = C.s;
     const double nu = C.nu;
     const double* const phi[5] = {M->Ux, M->Uy, M->Uz, M->k, M->om};
-    const double* const phib[5] = {M->Ubx, M->Uby, M->Ubz, M->kb, M->omb};
+    const double* const phib[5] = {M->Ubx, M->Uby, M->Ubz, kbGrad ? kbGrad : M->kb, M->omb};
     double g[5][3];
     gg_cell<5>(m, c, phi, phib, g);
 #pragma unroll
@@ -1998,6 +2000,8 @@ RSG_D SolveRes member_solve_x(Thr t, RsgMemberDev* M, int solver, double* x, int
     RSG_SYNC();
     return r;
 }
+
+#include "rsg_comp_dev.h"
 
 #undef FOR_CELLS
 #undef FOR_IFACES

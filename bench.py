@@ -502,6 +502,8 @@ def bench_comp(args):
     import compressible_diagnostics as cd
     D = lambda a: (C.c_double * len(a))(*a)
     rs.engine_set_graphs(bool(args.graphs))
+    rs.engine_set_launch(args.iters_per_launch, 512)
+    rs.engine_set_time_slice(int(args.slice_ms * 1000))
     case = cd.make_validation_case("ma_0_1", 0.1, nx=40, ny=30, max_iterations=3000)
     fm = case["forward"]
     theta0 = list(case["param_set"].pack(rs.SSTCoefficients()))
@@ -1124,7 +1126,7 @@ def main():
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--c4-members", type=int, default=65, help="c4 workload: ensemble members")
     ap.add_argument("--c4-iters", type=int, default=40, help="c4 workload: outer iterations per member and precision")
-    ap.add_argument("--comp-members", type=int, default=16, help="comp workload: theta samples in the batch")
+    ap.add_argument("--comp-members", type=int, default=256, help="comp workload: theta samples in the batch")
     ap.add_argument("--comp-cells", type=int, default=76800, help="comp workload: cells of the finer channel")
     ap.add_argument("--comp-fine-iters", type=int, default=100,
                     help="comp workload: outer iterations timed on the finer channel")

@@ -63,6 +63,7 @@ def emu():
     out_dir = os.path.join(TESTS, "emu", "_build")
     path = os.path.join(out_dir, "librsg_emu.so")
     srcs = [os.path.join(TESTS, "emu", "rsg_emu.cpp"), os.path.join(PKG, "csrc", "rsg_member.h"),
+            os.path.join(PKG, "csrc", "rsg_comp_dev.h"),
             os.path.join(PKG, "csrc", "rsg_prep.h"), os.path.join(PKG, "csrc", "rsg_layout.h"),
             os.path.join(PKG, "csrc", "rsg_resident.h")]
     if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(s) for s in srcs):

@@ -126,6 +126,88 @@ extern "C" int emu_solve_warm(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd,
     return emu_run(mesh, bcd, nu, s, coeffs, init, state, turbEstablished, 0, out);
 }
 
+// A compressible solve on the member kernel's code (run_comp_member), from
+// the uniform initial state of CompressibleSIMPLESolver::initUniform the
+// engine's rsg_comp_submit computes: cd the temperature BCs and gas,
+// init = U[3], p (thermodynamic), T, k, omega.  T, rho, pMech: [n] outputs.
+extern "C" int emu_comp_solve(const rsg_mesh_desc* mesh, const rsg_bc_desc* bcd, const rsg_comp_desc* cd,
+                              const rsg_settings* s, const double* coeffs, const double* init, rsg_result* out,
+                              double* Tout, double* rhoOut, double* pMechOut) {
+    rsg::PrepMesh P;
+    std::string err;
+    if (!rsg::prep_mesh(mesh, P, err)) { fprintf(stderr, "emu: %s\n", err.c_str()); return -1; }
+    if (!rsg::check_settings(s, err)) { fprintf(stderr, "emu: %s\n", err.c_str()); return -1; }
+    rsg::MeshOffsets mo = rsg::mesh_offsets(P);
+    std::vector<char> mblk(mo.total + 256);
+    rsg::mesh_stage(P, mo, mblk.data());
+    RsgMeshDev md = rsg::mesh_view(P, mo, mblk.data());
+    rsg::BcOffsets bo = rsg::bc_offsets(P.nB);
+    std::vector<char> bblk(bo.total + 256);
+    rsg::bc_stage(bcd, P.nB, bo, bblk.data());
+    RsgBcDev bc = rsg::bc_view(bo, bblk.data());
+    RsgSettingsDev sd = rsg::settings_dev(s);
+    RsgMemberDev M;
+    memset(&M, 0, sizeof M);
+    const int histCap = s->record_history ? s->max_iterations : 0;
+    size_t bytes = rsg::member_bytes(P.nC, P.nIF, P.nB, P.nnz0, histCap);
+    std::vector<char> wblk(bytes + 256, 0);
+    rsg::Carver cv; cv.base = wblk.data();
+    rsg::carve_member(cv, P.nC, P.nIF, P.nB, P.nnz0, histCap, M);
+    const int n = P.nC, nB = P.nB, nb1 = nB > 0 ? nB : 1;
+    std::vector<double> f(13 * (size_t)n + 2 * (size_t)nb1, 0.0), tV(nb1, 0.0);
+    std::vector<int> tT(nb1, RSGT_ZERO_GRADIENT);
+    for (int b = 0; b < nB; ++b) { tT[b] = cd->t_type[b]; tV[b] = cd->t_value[b]; }
+    RsgCompDev Q;
+    memset(&Q, 0, sizeof Q);
+    double* q = f.data();
+    Q.T = q; q += n; Q.rho = q; q += n; Q.mu = q; q += n; Q.Told = q; q += n;
+    Q.Ux0 = q; q += n; Q.Uy0 = q; q += n; Q.Uz0 = q; q += n; Q.mdu = q; q += n;
+    for (int k = 0; k < 3; ++k) { Q.gt[k] = q; q += n; }
+    Q.pMech = q; q += n;
+    Q.Tb = q; q += nb1; Q.zeroB = q;
+    Q.tT = tT.data(); Q.tV = tV.data();
+    Q.gamma = cd->gamma; Q.R = cd->R; Q.Pr = cd->Pr; Q.PrT = cd->Pr_T;
+    Q.muRef = cd->mu_ref; Q.Tref = cd->T_ref; Q.S = cd->S_suth; Q.alphaT = cd->alpha_t;
+    // initUniform's thermodynamic state, as rsg_comp_submit evaluates it
+    const double p0 = init[3], T0 = init[4], k0 = init[5];
+    const double rho0 = p0 / (Q.R * T0);
+    const double ratio = T0 / Q.Tref;
+    const double mu0 = Q.muRef * ratio * sqrt(ratio) * (Q.Tref + Q.S) / (T0 + Q.S);
+    Q.Tinit = T0; Q.rho0 = rho0; Q.mu0 = mu0;
+    M.mesh = &md; M.bc = &bc; M.set = &sd; M.nu = mu0 / rho0; M.comp = &Q;
+    for (int i = 0; i < 11; ++i) M.coeffs[i] = coeffs[i];
+    M.init[0] = init[0]; M.init[1] = init[1]; M.init[2] = init[2];
+    M.init[3] = p0 + (2.0 / 3.0) * rho0 * (k0 > 0.0 ? k0 : 0.0);
+    M.init[4] = k0; M.init[5] = init[6];
+    M.planarZ = rsg::planar_z(P, P.nB == 0 || rsg::bc_zero_z(bcd, P.nB), s, init[2], nullptr, nullptr) ? 1 : 0;
+    M.iterBudget = s->max_iterations;
+    M.sc->status = RSGM_RUNNING;
+    M.sc->amgBuiltIter = -1;
+    M.sc->coarseIter = -1000000000;
+    M.sc->lastK = M.sc->lastOm = 1.0;
+    rsg::Thr t = emu_thr();
+    while (M.sc->status == RSGM_RUNNING) rsg::run_comp_member<RSG_RES_MAXN>(t, &M);
+    RsgScalars* sc = M.sc;
+    out->status = RSG_TICKET_DONE;
+    out->final_iter = sc->finalIter;
+    out->converged = sc->status == RSGM_CONVERGED;
+    out->diverged = sc->status == RSGM_DIVERGED;
+    out->turb_established = sc->turbEstablished;
+    for (int i = 0; i < 6; ++i) out->inner_iters[i] = sc->innerIters[i];
+    if (out->U) for (int c = 0; c < n; ++c) { out->U[3*c] = M.Ux[c]; out->U[3*c+1] = M.Uy[c]; out->U[3*c+2] = M.Uz[c]; }
+    auto cp = [](double* dst, const double* src, int k) { if (dst) memcpy(dst, src, 8 * (size_t)k); };
+    cp(out->p, M.p, n); cp(out->k, M.k, n); cp(out->omega, M.om, n); cp(out->nuT, M.nuT, n);
+    cp(out->F1, M.F1, n); cp(out->F2, M.F2, n); cp(out->Pk, M.Pk, n); cp(out->CDkw, M.CDkw, n);
+    cp(Tout, Q.T, n); cp(rhoOut, Q.rho, n); cp(pMechOut, Q.pMech, n);
+    out->history_len = 0;
+    if (out->history && M.hist) {
+        int len = sc->histLen < out->history_cap ? sc->histLen : out->history_cap;
+        memcpy(out->history, M.hist, 48 * (size_t)len);
+        out->history_len = len;
+    }
+    return 0;
+}
+
 // Stand-alone LDU solve through the same topology preparation and solver
 // body the engine's rsg_ldu_solve uses.  stats = {iters, finalRes, initialRes, converged, levels}.
 extern "C" int emu_ldu_solve(int n, int nif, const int* own, const int* nbr, const double* diag,

@@ -110,13 +110,22 @@ CASES = {"pcg52t": (dict(iters=52, pressure_solver=1, **TIGHT), 1e-7, 1e-8),
          "amg52t": (dict(iters=52, pressure_solver=0, **TIGHT), 1e-7, 1e-8)}
 
 
+def _path(lib, ctx, path):
+    """member: one workgroup per solve (k_comp_chunk, the default on these
+    meshes); grid: kernel per phase with one-workgroup linear solves."""
+    if path == "grid":
+        chk(lib, ctx, lib.rsg_set_grid(ctx, 1, 20))
+
+
+@pytest.mark.parametrize("path", ["member", "grid"])
 @pytest.mark.parametrize("case", list(CASES))
-def test_comp_trajectory_vs_reference(rsg_lib, gpu_ctx, mesh16, golden16, case):
+def test_comp_trajectory_vs_reference(rsg_lib, gpu_ctx, mesh16, golden16, case, path):
     g = {k[len(case) + 1:]: v for k, v in golden16.items() if k.startswith(case + "_")}
     c, tol_hist, tol_field = CASES[case]
     c = dict(c)
     s = CC.settings(c.pop("iters"), record_history=1, **c)
-    print("compressible trajectory", case, flush=True)
+    print("compressible trajectory", case, path, flush=True)
+    _path(rsg_lib, gpu_ctx, path)
     P = CompProblem(rsg_lib, gpu_ctx, mesh16, s)
     try:
         r, a = P.solve()
@@ -143,10 +152,10 @@ def test_comp_trajectory_vs_reference(rsg_lib, gpu_ctx, mesh16, golden16, case):
 
 
 # Converged-state agreement with the reference solver (convergence_tol 1e-3
-# on the field-change norms): measured U 1.4e-6, p_mech 2e-9, k 1.8e-5,
-# omega 2e-7, nuT 9e-5, T - T_in 4e-3 and rho - rho_0 1e-3 (the thermal
-# variation is 0.07 K of 300 K: 2.8e-5 K absolute)
-CONVERGED_TOL = {"U": 1e-5, "p_mech": 1e-7, "T-T_in": 2e-2, "rho": 1e-2, "k": 2e-4, "omega": 2e-6, "nuT": 1e-3}
+# on the field-change norms), measured on both device paths: U 1.3e-7, p_mech
+# 2e-9, k 1.1e-7, omega 1.9e-7, nuT 2e-7, T - T_in 4e-3 and rho - rho_0 8e-4
+# (the thermal variation is 0.07 K of 300 K: 2.8e-5 K absolute)
+CONVERGED_TOL = {"U": 1e-6, "p_mech": 2e-8, "T-T_in": 2e-2, "rho": 1e-2, "k": 1e-6, "omega": 2e-6, "nuT": 2e-6}
 
 
 def _check_converged(errs):
@@ -154,15 +163,17 @@ def _check_converged(errs):
         assert e < CONVERGED_TOL[k], (k, e)
 
 
-def test_comp_converged_state_vs_reference(rsg_lib, gpu_ctx, rs, tmp_path):
+@pytest.mark.parametrize("path", ["member", "grid"])
+def test_comp_converged_state_vs_reference(rsg_lib, gpu_ctx, rs, tmp_path, path):
     """The reference's regression case through the C-ABI with the p' AMG
     re-aggregated every outer iteration (as the reference's AMG solver does):
     the same iteration count and the reference solver's converged fields."""
     g = dict(np.load(os.path.join(HERE, "golden", "comp_channel_40x30.npz")))
-    path = str(tmp_path / "c40.msh")
-    CC.mesh(rs, 40, 30, 0.1).save_binary(path)
-    print("compressible 40 x 30 through the C-ABI", flush=True)
-    P = CompProblem(rsg_lib, gpu_ctx, path, CC.settings(3000))
+    mpath = str(tmp_path / "c40.msh")
+    CC.mesh(rs, 40, 30, 0.1).save_binary(mpath)
+    print("compressible 40 x 30 through the C-ABI", path, flush=True)
+    _path(rsg_lib, gpu_ctx, path)
+    P = CompProblem(rsg_lib, gpu_ctx, mpath, CC.settings(3000))
     try:
         r, a = P.solve()
     finally:
@@ -173,6 +184,32 @@ def test_comp_converged_state_vs_reference(rsg_lib, gpu_ctx, rs, tmp_path):
             "k": rel(a["k"], g["k"]), "omega": rel(a["omega"], g["om"]), "nuT": rel(a["nuT"], g["nuT"])}
     print("iterations", r.final_iter, "converged field rel errors", errs, flush=True)
     _check_converged(errs)
+
+
+def test_comp_member_batch_equals_alone(rsg_lib, gpu_ctx, mesh16):
+    """Many compressible members in one member-kernel launch (slices, rounds)
+    each equal to its solve alone, bit for bit."""
+    s = CC.settings(150, record_history=1)
+    P = CompProblem(rsg_lib, gpu_ctx, mesh16, s)
+    try:
+        rng = np.random.default_rng(5)
+        coeffs = []
+        for i in range(40):
+            c = list(A.DEFAULT_COEFFS)
+            c[9] *= 1 + 0.1 * rng.uniform(-1, 1); c[8] *= 1 + 0.1 * rng.uniform(-1, 1)
+            coeffs.append(c)
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_time_slice(gpu_ctx, 20000))
+        tickets = [P.submit(c) for c in coeffs]
+        many = [P.fetch(t) for t in tickets]
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_set_time_slice(gpu_ctx, 0))
+        for i in (0, 17, 39):
+            r, a = P.solve(coeffs[i])
+            rb, b = many[i]
+            assert r.final_iter == rb.final_iter
+            for k in ("U", "p", "p_mech", "T", "rho", "k", "omega", "nuT", "hist"):
+                assert np.array_equal(a[k], b[k]), (i, k)
+    finally:
+        P.close()
 
 
 def test_comp_arguments(rsg_lib, gpu_ctx, mesh16):
