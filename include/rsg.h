@@ -228,9 +228,11 @@ int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const
  * no patch), the gas of IdealGasEOS (IdealGasEOS.hpp:6-15) and alphaT
  * (SolverSettings::alphaT, core/include/FlowFields.hpp:62).  Velocity,
  * pressure, k and omega BCs use rsg_bc_desc as for the incompressible solve;
- * the outlet's prescribed pressure is thermodynamic.  Every compressible
- * solve runs kernel-per-phase (rsg_set_grid's path, FP64), at most four at
- * once alongside the member launches. */
+ * the outlet's prescribed pressure is thermodynamic.  Compressible solves are
+ * FP64.  On meshes of up to 4,096 cells each runs as one workgroup of the
+ * compressible member kernel (like the incompressible ensemble: persistent
+ * time-sliced launches, continuous batching); larger meshes, or meshes of at
+ * least rsg_set_grid's min_cells, run kernel-per-phase, at most four at once. */
 #define RSG_TBC_DIRICHLET 0
 #define RSG_TBC_ZERO_GRADIENT 1
 #define RSG_TBC_KEEP 2
