@@ -29,6 +29,10 @@
  *                          ObservationOperator::evaluate / logLikelihood
  *                          (core/include/ObservationOperator.hpp:159-378) as called
  *                          from ForwardModel::evaluate (ForwardModel.cpp:95-97)
+ *   rsg_comp_problem_create / rsg_comp_submit / rsg_comp_fetch
+ *                          CompressibleSIMPLESolver ctor, initUniform + solve as called
+ *                          from CompressibleForwardModel::evaluate
+ *                          (compressible/src/CompressibleForwardModel.cpp:36-97)
  */
 #ifndef RSG_H
 #define RSG_H
