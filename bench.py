@@ -557,6 +557,9 @@ def bench_comp(args):
                "sample": "one full solve of the 40 x 30 case by the reference solver: %d iterations, converged %d, "
                          "%.1f s" % (info[0], info[1], dt), "lscpu": lscpu_info()}
     # the finer channel: outer-iteration time, device against the reference
+    if args.comp_fine_iters <= 0:
+        fine_out = None
+        return _comp_line(args, lat, iters, conv, summ, batch, fine_out, cpu)
     nx = int(round(math.sqrt(args.comp_cells * 4.0 / 3.0)))
     ny = max(2, int(round(args.comp_cells / nx)))
     fine = cd.make_validation_case("ma_0_1_fine", 0.1, nx=nx, ny=ny, max_iterations=args.comp_fine_iters,
@@ -577,6 +580,10 @@ def bench_comp(args):
         fine_out["reference_ms_per_outer_iteration"] = 1e3 * dt / max(info[0], 1)
         fine_out["reference_sample"] = "%d outer iterations, one core, %.1f s" % (info[0], dt)
     print("bench: comp fine %s" % fine_out, file=sys.stderr, flush=True)
+    return _comp_line(args, lat, iters, conv, summ, batch, fine_out, cpu)
+
+
+def _comp_line(args, lat, iters, conv, summ, batch, fine_out, cpu):
     t = float(np.median(lat))
     print(json.dumps({
         "metric": "compressible converged solves/s", "value": conv / sum(lat), "unit": "solves/s", "n_gpus": 1,
