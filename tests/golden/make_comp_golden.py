@@ -8,7 +8,11 @@ reference's compressible tests, from its uniform initial state:
   amg52t  the same with the AMG pressure solver
 
 and tests/golden/comp_channel_40x30.npz: the converged fields of the
-reference's Ma 0.1 regression case (40 x 30 channel, its own settings).
+reference's Ma 0.1 regression case (40 x 30 channel, its own settings);
+tests/golden/comp_ladder.npz: the reference's validation ladder
+(python/examples/compressible_validation_ladder.py: Ma 0.3 on 40 x 30 for
+4000 iterations, Ma 0.5 on 48 x 36 for 6000, Ma 0.7 on 64 x 48 for 10000):
+the run information of each case and the Ma 0.3 fields.
 
 Per case: fields (U, mechanical p, T, rho, k, omega, nuT, F1, F2, Pk), the
 residual history (Ux, Uy, p, T, k, omega per outer iteration) and
@@ -31,6 +35,10 @@ import rans_sst_py as rs  # noqa: E402
 KEYS = ("U", "p", "T", "rho", "k", "om", "nuT", "F1", "F2", "Pk")
 TIGHT = dict(convergence_tol=1e-12, inner_tolerance=1e-14, inner_iterations=2000)
 CASES = {"pcg52t": dict(iters=52, pressure_solver=1, **TIGHT), "amg52t": dict(iters=52, pressure_solver=0, **TIGHT)}
+
+
+# the reference's validation ladder: name, Mach, nx, ny, outer-iteration cap
+LADDER = [("ma_0_3", 0.3, 40, 30, 4000), ("ma_0_5", 0.5, 48, 36, 6000), ("ma_0_7", 0.7, 64, 48, 10000)]
 
 
 def case_settings(name):
@@ -72,6 +80,17 @@ def main():
     print("c40", info)
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "comp_channel_40x30.npz"), info=info,
                         **{k: out[k] for k in ("U", "p", "T", "rho", "k", "om", "nuT")})
+    lad = {}
+    for name, Ma, nx, ny, it in LADDER:
+        path = "/tmp/comp_%s.msh" % name
+        CC.mesh(rs, nx, ny, Ma).save_binary(path)
+        out, hist, info = ref_comp_solve(ref, path, CC.settings(it, Ma=Ma), Ma=Ma)
+        print(name, info)
+        lad[name + "_info"] = info
+        if name == "ma_0_3":
+            for k in ("U", "p", "T", "rho", "k", "om", "nuT"):
+                lad[name + "_" + k] = out[k]
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "comp_ladder.npz"), **lad)
 
 
 if __name__ == "__main__":

@@ -319,3 +319,46 @@ def test_comp_batch_equals_alone(rs):
         assert np.array_equal(np.asarray(a.predictions), np.asarray(b.predictions))
     ff = fm.last_fields()
     assert np.all(np.asarray(ff["T"]) > 0)
+
+
+LADDER = [("ma_0_3", 0.3, 40, 30, 4000), ("ma_0_5", 0.5, 48, 36, 6000), ("ma_0_7", 0.7, 64, 48, 10000)]
+
+
+@pytest.mark.parametrize("name,Ma,nx,ny,iters", LADDER)
+def test_comp_ladder_matches_reference(rsg_lib, gpu_ctx, rs, tmp_path, name, Ma, nx, ny, iters):
+    """The reference's validation ladder (python/examples/
+    compressible_validation_ladder.py, production meshes) against the
+    reference solver's outcomes (tests/golden/comp_ladder.npz): Ma 0.3 runs
+    to its 4000-iteration cap unconverged, Ma 0.5 and 0.7 diverge (the
+    reference: at outer iterations 253 and 11).  The 48 x 36 and 64 x 48
+    meshes exceed the resident-solver size, so the member kernel's
+    global-memory solvers run them."""
+    g = dict(np.load(os.path.join(HERE, "golden", "comp_ladder.npz")))
+    info = g[name + "_info"]
+    path = str(tmp_path / (name + ".msh"))
+    CC.mesh(rs, nx, ny, Ma).save_binary(path)
+    print("compressible ladder", name, flush=True)
+    P = CompProblem(rsg_lib, gpu_ctx, path, CC.settings(iters, Ma=Ma), Ma=Ma)
+    try:
+        r, a = P.solve()
+    finally:
+        P.close()
+    print(name, "device", (r.final_iter, r.converged, r.diverged), "reference", list(info), flush=True)
+    assert (r.converged, r.diverged) == (info[1], info[2])
+    if info[2]:
+        assert abs(r.final_iter - int(info[0])) <= max(2, int(0.05 * info[0])), (r.final_iter, info[0])
+    else:
+        assert r.final_iter == info[0]
+        errs = {"U": rel(a["U"], g[name + "_U"]), "p_mech": rel(a["p_mech"], g[name + "_p"]),
+                "k": rel(a["k"], g[name + "_k"]), "omega": rel(a["omega"], g[name + "_om"]),
+                "nuT": rel(a["nuT"], g[name + "_nuT"])}
+        print(name, "field rel errors", errs, flush=True)
+        for k, e in errs.items():
+            assert e < LADDER_TOL, (name, k, e)
+
+
+# Ma 0.3 after 4000 unconverged outer iterations (both solvers still moving;
+# the inner solves stop where round-off puts them): measured U 8e-7, p_mech
+# 3e-8, k 7e-7, omega 8e-8, nuT 1.2e-6; Ma 0.5 and 0.7 diverge at the
+# reference's iterations exactly (253, 11)
+LADDER_TOL = 1e-5
