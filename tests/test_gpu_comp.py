@@ -362,3 +362,53 @@ def test_comp_ladder_matches_reference(rsg_lib, gpu_ctx, rs, tmp_path, name, Ma,
 # 3e-8, k 7e-7, omega 8e-8, nuT 1.2e-6; Ma 0.5 and 0.7 diverge at the
 # reference's iterations exactly (253, 11)
 LADDER_TOL = 1e-5
+
+
+def test_comp_observables_match_oracle(rsg_lib, gpu_ctx, oracle, mesh16):
+    """A compressible solve's observables (CompressibleForwardModel.cpp:72-97:
+    the incompressible observation operator on the thermodynamic pressure
+    with nu = mu(T_0) / rho_0) against the oracle's observation operator (bit
+    for bit with the compiled reference's) on the fields the solve returns:
+    every observable type but drag, which the reference rejects for
+    compressible models."""
+    s = CC.settings(300)
+    P = CompProblem(rsg_lib, gpu_ctx, mesh16, s)
+    try:
+        msh = P.msh
+        full = A.observation_set(msh, np.random.default_rng(11))
+        keep = np.nonzero(full["type"] != 0)[0]
+        o = {k: (v[keep] if k != "loc" else v.reshape(-1, 3)[keep].ravel()) for k, v in full.items()}
+        recs, faces, fx = A.resolve_observations(msh, o)
+        n = len(o["type"])
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_problem_set_observations(gpu_ctx, P.pid, n, recs, len(faces),
+                                                                   A.iptr(faces), A.dptr(fx)))
+        t = P.submit()
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_wait(gpu_ctx, (C.c_int64 * 1)(t), 1, C.c_double(600.0)))
+        r, a = A.new_result(msh.nC, msh.nF - msh.nIF)
+        pred = np.full(n, np.nan)
+        r.predictions = A.dptr(pred); r.predictions_cap = n
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_fetch(gpu_ctx, C.c_int64(t), C.byref(r)))
+        T = np.zeros(msh.nC); rho = np.zeros(msh.nC)
+        chk(rsg_lib, gpu_ctx, rsg_lib.rsg_comp_fetch(gpu_ctx, C.c_int64(t), A.dptr(T), A.dptr(rho), None, None,
+                                                     None))
+        rsg_lib.rsg_release(gpu_ctx, C.c_int64(t))
+        assert r.status == 2 and r.n_predictions == n
+    finally:
+        P.close()
+    _, R, _, _, mu_ref, T_ref, S = A.EOS_AIR
+    ratio = T[0] / T_ref
+    nu_eff = mu_ref * ratio * np.sqrt(ratio) * (T_ref + S) / (T[0] + S) / max(rho[0], 1e-30)
+    m = oracle.om_mesh_load(mesh16.encode())
+    try:
+        po = np.zeros(n)
+        oracle.om_observe(C.c_void_p(m), n, A.iptr(o["type"]), A.iptr(o["patch"]), A.dptr(o["loc"]),
+                          A.iptr(o["comp"]), A.dptr(o["refArea"]), A.dptr(o["refVel"]), A.dptr(o["refP"]),
+                          A.dptr(o["refRho"]), A.dptr(a["U"]), A.dptr(a["U_bnd"]), A.dptr(a["p"]),
+                          C.c_double(float(nu_eff)), A.dptr(po))
+    finally:
+        oracle.om_mesh_free(C.c_void_p(m))
+    llo = oracle.om_loglik(n, A.dptr(po), A.dptr(o["observed"]), A.dptr(o["sigma"]), A.dptr(o["sigmaModel"]))
+    bad = np.nonzero(pred != po)[0]
+    assert bad.size == 0, [(int(o["type"][i]), pred[i], po[i]) for i in bad]
+    assert r.log_lik == llo
+    assert set(np.unique(o["type"]).tolist()) == {1, 2, 3, 4, 5}
