@@ -1,8 +1,8 @@
 """Black-box forward-model protocol on the MI355X engine.
 
 Mirrors the boundary part of the reference's python/forward_model_interface.py
-(:42-127): EvaluationResult, ForwardModelBase and the wrapper of the C++
-incompressible ForwardModel, with the same evaluate(theta) -> EvaluationResult
+(:42-155): EvaluationResult, ForwardModelBase and the wrappers of the C++
+incompressible and compressible forward models, with the same evaluate(theta) -> EvaluationResult
 (.predictions list[float], .status str) contract that BayesianInferenceKOH
 consumes.  EnsembleForwardModel adds the batched form: many thetas per call,
 solved concurrently on the device.
@@ -49,11 +49,15 @@ def _wrap(result) -> EvaluationResult:
 
 
 def _param_names(param_set) -> List[str]:
+    """Reference python/forward_model_interface.py:541-547: the set's own
+    names, a dict's "names", else theta_0 .. theta_{n-1} (n = n_active(),
+    default 2)."""
     if hasattr(param_set, "active_names"):
         return list(param_set.active_names())
-    if isinstance(param_set, dict):
-        return list(param_set.get("names", param_set.keys()))
-    return [str(p) for p in param_set]
+    if isinstance(param_set, dict) and "names" in param_set:
+        return list(param_set["names"])
+    n = param_set.n_active() if hasattr(param_set, "n_active") else 2
+    return [f"theta_{i}" for i in range(n)]
 
 
 class InternalIncompressibleForwardModel(ForwardModelBase):
@@ -71,6 +75,13 @@ class InternalIncompressibleForwardModel(ForwardModelBase):
 
     def parameter_names(self) -> List[str]:
         return _param_names(self._param_set)
+
+
+class InternalCompressibleForwardModel(InternalIncompressibleForwardModel):
+    """Wraps rans_sst_py.CompressibleForwardModel (pressure-based SIMPLE +
+    SST + energy, Ma below about 0.5); same contract as the incompressible
+    wrapper (reference python/forward_model_interface.py:130-155).  A
+    diverged solve returns non-finite predictions, so converged is False."""
 
 
 class EnsembleForwardModel(ForwardModelBase):

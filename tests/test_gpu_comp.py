@@ -420,3 +420,18 @@ def test_comp_observables_match_oracle(rsg_lib, gpu_ctx, oracle, mesh16):
     assert bad.size == 0, [(int(o["type"][i]), pred[i], po[i]) for i in bad]
     assert r.log_lik == llo
     assert set(np.unique(o["type"]).tolist()) == {1, 2, 3, 4, 5}
+
+
+def test_comp_python_wrapper(rs):
+    """forward_model_interface.InternalCompressibleForwardModel (reference
+    python/forward_model_interface.py:130-155) over the device-backed model:
+    the same predictions and status as the C++ model's own evaluate."""
+    import forward_model_interface as FMI
+    mesh, eos, ps, fm = _validation_case(rs, 0.1, 16, 12, 200)
+    w = FMI.InternalCompressibleForwardModel(fm, ps)
+    assert w.parameter_names() == list(ps.active_names())
+    th = [0.31, 0.09]
+    r, d = w.evaluate(th), fm.evaluate(th)
+    assert r.status == str(d.status) and r.metadata["simple_iters"] == d.simple_iters
+    assert np.array_equal(np.asarray(r.predictions), np.asarray(d.predictions))
+    assert r.converged == all(np.isfinite(d.predictions))
