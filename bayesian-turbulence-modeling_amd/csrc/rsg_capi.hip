@@ -218,6 +218,7 @@ struct rsg_context {
     std::map<int64_t, std::unique_ptr<Ticket>> tickets;
     std::deque<int64_t> pending;
     std::vector<int64_t> active;
+    int holds = 0;           // rsg_hold_admission depth: > 0 admits no pending ticket
     int nextMesh = 1, nextProblem = 1;
     int64_t nextTicket = 1;
     int itersPerLaunch = 200;
@@ -434,11 +435,13 @@ void engine_loop(rsg_context* ctx) {
     hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(ctx->mu);
     while (true) {
-        ctx->cvEngine.wait(lk, [&] { return ctx->stop || !ctx->pending.empty() || !ctx->active.empty(); });
+        ctx->cvEngine.wait(lk, [&] {
+            return ctx->stop || (!ctx->pending.empty() && ctx->holds == 0) || !ctx->active.empty();
+        });
         if (ctx->stop) break;
-        // admission (HBM footprint first-fit, FIFO)
+        // admission (HBM footprint first-fit, FIFO), none while a submitter holds it
         std::string err;
-        while (!ctx->pending.empty() && (int)ctx->active.size() < ctx->maxSlots) {
+        while (ctx->holds == 0 && !ctx->pending.empty() && (int)ctx->active.size() < ctx->maxSlots) {
             int64_t id = ctx->pending.front();
             Ticket& T = *ctx->tickets[id];
             if (!admit(ctx, T, err)) {
@@ -1261,6 +1264,18 @@ int rsg_set_time_slice(rsg_context* ctx, int32_t microseconds) {
     if (microseconds > 0 && ctx->clockKHz <= 0)
         return set_err(ctx, RSG_ERR_UNSUPPORTED, "device wall clock rate unknown: time slices unavailable");
     ctx->sliceUs = microseconds;
+    return RSG_OK;
+}
+
+int rsg_hold_admission(rsg_context* ctx, int32_t hold) {
+    if (!ctx || (hold != 0 && hold != 1)) return RSG_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (hold) ctx->holds += 1;
+        else if (ctx->holds > 0) ctx->holds -= 1;
+        else return set_err(ctx, RSG_ERR_STATE, "rsg_hold_admission: no hold to release");
+    }
+    ctx->cvEngine.notify_all();
     return RSG_OK;
 }
 

@@ -470,6 +470,13 @@ PYBIND11_MODULE(rans_sst_py, m) {
           "Replay AMG V-cycles of kernel-per-phase members from hipGraphs (default on); results are identical "
           "either way.");
     m.def("engine_set_time_slice", [](int us) { Engine::get().setTimeSlice(us); }, py::arg("microseconds"));
+    m.def("engine_hold_admission", [](bool hold) {
+              rsg_context* c = Engine::get().ctx();
+              const int rc = rsg_hold_admission(c, hold ? 1 : 0);
+              if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+          }, py::arg("hold"),
+          "hold=True stops the engine admitting submitted solves until the matching hold=False, so that a batch "
+          "submitted in between starts in one launch; never wait for a solve while holding.");
     m.def("engine_set_schedule", [](const std::string& p) {
               if (p == "round_robin") Engine::get().setSchedule(RSG_SCHEDULE_ROUND_ROBIN);
               else if (p == "largest_first") Engine::get().setSchedule(RSG_SCHEDULE_LARGEST_FIRST);

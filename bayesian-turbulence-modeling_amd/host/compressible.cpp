@@ -211,13 +211,16 @@ EvaluationResult CompressibleForwardModel::evaluate(const std::vector<double>& t
 std::vector<EvaluationResult> CompressibleForwardModel::evaluateBatch(const std::vector<std::vector<double>>& thetas) {
     std::vector<EvaluationResult> out(thetas.size());
     std::vector<int64_t> tickets(thetas.size(), -1);
-    for (size_t i = 0; i < thetas.size(); ++i) {
-        if (!paramSet_.inBounds(thetas[i])) {
-            out[i].status = EvaluationStatus::InvalidParameters;
-            out[i].loglik = -1e30;
-            continue;
+    {
+        AdmissionHold hold;
+        for (size_t i = 0; i < thetas.size(); ++i) {
+            if (!paramSet_.inBounds(thetas[i])) {
+                out[i].status = EvaluationStatus::InvalidParameters;
+                out[i].loglik = -1e30;
+                continue;
+            }
+            tickets[i] = submit(thetas[i]);
         }
-        tickets[i] = submit(thetas[i]);
     }
     for (size_t i = 0; i < thetas.size(); ++i)
         if (tickets[i] >= 0) out[i] = complete(tickets[i]);

@@ -17,6 +17,15 @@ std::string engineError(rsg_context* ctx, int rc) {
     return "rsg error " + std::to_string(rc) + ": " + msg;
 }
 
+AdmissionHold::AdmissionHold() : ctx_(Engine::get().ctx()), gen_(Engine::get().generation()) {
+    const int rc = rsg_hold_admission(ctx_, 1);
+    if (rc != RSG_OK) throw std::runtime_error(engineError(ctx_, rc));
+}
+
+AdmissionHold::~AdmissionHold() {
+    if (rsg_context* c = Engine::get().existing(gen_)) rsg_hold_admission(c, 0);
+}
+
 rsg_context* Engine::ctx() {
     std::lock_guard<std::mutex> l(mu_);
     if (ctx_) return ctx_;
@@ -280,13 +289,16 @@ EvaluationResult ForwardModel::evaluate(const std::vector<double>& theta) {
 std::vector<EvaluationResult> ForwardModel::evaluateBatch(const std::vector<std::vector<double>>& thetas) {
     std::vector<EvaluationResult> out(thetas.size());
     std::vector<int64_t> tickets(thetas.size(), -1);
-    for (size_t i = 0; i < thetas.size(); ++i) {
-        if (!paramSet_.inBounds(thetas[i])) {
-            out[i].status = EvaluationStatus::InvalidParameters;
-            out[i].loglik = -1e30;
-            continue;
+    {
+        AdmissionHold hold;
+        for (size_t i = 0; i < thetas.size(); ++i) {
+            if (!paramSet_.inBounds(thetas[i])) {
+                out[i].status = EvaluationStatus::InvalidParameters;
+                out[i].loglik = -1e30;
+                continue;
+            }
+            tickets[i] = submit(thetas[i], true);
         }
-        tickets[i] = submit(thetas[i], true);
     }
     for (size_t i = 0; i < thetas.size(); ++i) {
         if (tickets[i] < 0) continue;
@@ -305,13 +317,16 @@ std::vector<EvaluationResult> evaluateMany(const std::vector<ForwardModel*>& fms
     if (fms.size() != thetas.size()) throw std::invalid_argument("evaluate_many: models and thetas differ in length");
     std::vector<EvaluationResult> out(fms.size());
     std::vector<int64_t> tickets(fms.size(), -1);
-    for (size_t i = 0; i < fms.size(); ++i) {
-        if (!fms[i]->paramSet().inBounds(thetas[i])) {
-            out[i].status = EvaluationStatus::InvalidParameters;
-            out[i].loglik = -1e30;
-            continue;
+    {
+        AdmissionHold hold;
+        for (size_t i = 0; i < fms.size(); ++i) {
+            if (!fms[i]->paramSet().inBounds(thetas[i])) {
+                out[i].status = EvaluationStatus::InvalidParameters;
+                out[i].loglik = -1e30;
+                continue;
+            }
+            tickets[i] = fms[i]->submit(thetas[i], true);
         }
-        tickets[i] = fms[i]->submit(thetas[i], true);
     }
     for (size_t i = 0; i < fms.size(); ++i) {
         if (tickets[i] < 0) continue;

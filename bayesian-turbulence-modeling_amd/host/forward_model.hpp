@@ -64,6 +64,20 @@ private:
 
 std::string engineError(rsg_context* ctx, int rc);
 
+// Holds the engine's admission while it lives (rsg_hold_admission), so that
+// the solves a batch submits under it start together in one launch.  Nothing
+// that waits for a solve may run while it is held.
+class AdmissionHold {
+public:
+    AdmissionHold();
+    ~AdmissionHold();
+    AdmissionHold(const AdmissionHold&) = delete;
+    AdmissionHold& operator=(const AdmissionHold&) = delete;
+private:
+    rsg_context* ctx_;
+    uint64_t gen_;
+};
+
 // Mesh::computeWallDistance on the device (rsg_wall_distance): the same
 // distances, bit for bit; returns the kernel's seconds.
 double computeWallDistanceOnDevice(Mesh& mesh);

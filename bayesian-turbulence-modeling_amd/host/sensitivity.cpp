@@ -165,12 +165,15 @@ TangentGradientResult ParameterSensitivity::etaJacobianWarmFD(const std::vector<
     };
     std::vector<double> h(11);
     std::vector<int64_t> tickets(22);
-    for (int j = 0; j < 11; ++j) {
-        h[j] = stepFor(j, theta11[j]);
-        std::vector<double> tp = theta11, tm = theta11;
-        tp[j] += h[j]; tm[j] -= h[j];
-        tickets[2 * j] = submit(pid, tp, &state_);
-        tickets[2 * j + 1] = submit(pid, tm, &state_);
+    {
+        AdmissionHold hold;
+        for (int j = 0; j < 11; ++j) {
+            h[j] = stepFor(j, theta11[j]);
+            std::vector<double> tp = theta11, tm = theta11;
+            tp[j] += h[j]; tm[j] -= h[j];
+            tickets[2 * j] = submit(pid, tp, &state_);
+            tickets[2 * j + 1] = submit(pid, tm, &state_);
+        }
     }
     for (int j = 0; j < 11; ++j) {
         std::vector<double> eta[2];

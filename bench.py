@@ -127,7 +127,11 @@ def run_batch(rs, specs, tol, fms_cache=None, max_iterations=200000, precision="
     # every member is submitted before any is awaited (what evaluate_many
     # does); finished members are collected as they retire, with a progress
     # line per 32 retirements
-    tickets = [fm.submit(sp["theta"]) for fm, sp in zip(fms, specs)]
+    rs.engine_hold_admission(True)     # the whole ensemble starts in one launch
+    try:
+        tickets = [fm.submit(sp["theta"]) for fm, sp in zip(fms, specs)]
+    finally:
+        rs.engine_hold_admission(False)
     res = [None] * len(fms)
     left = set(range(len(fms)))
     done = 0

@@ -17,8 +17,8 @@
  *   rsg_fetch              FlowFields + ConvergenceHistory returned by solve
  *                          (core/include/FlowFields.hpp:9-49)
  *   rsg_ldu_solve          ILinearSolver::solve (core/include/LinearSolver.hpp:115-120)
- *   rsg_set_launch / rsg_set_time_slice / rsg_set_schedule / rsg_set_team
- *                          engine tuning (no reference counterpart: the reference
+ *   rsg_set_launch / rsg_set_time_slice / rsg_set_schedule / rsg_set_team,
+ *   rsg_hold_admission     engine tuning (no reference counterpart: the reference
  *                          solves one member at a time)
  *   rsg_system_solve       AMGSolver::solve / PCGSolver::solve on one large system
  *                          (core/include/LinearSolver.hpp:602-653, :147-186)
@@ -331,6 +331,15 @@ int rsg_set_graphs(rsg_context* ctx, int32_t enable);
 #define RSG_SCHEDULE_ROUND_ROBIN 0
 #define RSG_SCHEDULE_LARGEST_FIRST 1
 int rsg_set_schedule(rsg_context* ctx, int32_t policy);
+/* Batch submission: hold = 1 stops the engine from admitting pending tickets
+ * (members already running go on); hold = 0 releases one hold, and once none
+ * is left every pending ticket is admitted together at the next launch
+ * boundary.  Holds nest (one release per hold; releasing with none held is
+ * RSG_ERR_STATE).  Without it the engine starts a launch with the first
+ * ticket submitted, and tickets submitted while that launch runs wait for the
+ * next one (up to one time slice).  A caller that submits many solves at
+ * once holds, submits them all, then releases; results do not depend on it. */
+int rsg_hold_admission(rsg_context* ctx, int32_t hold);
 /* Engine counters: out[0] launches, out[1] members admitted, out[2] members retired,
  * out[3] kernel milliseconds (HIP events), out[4] max concurrent members. */
 int rsg_stats(rsg_context* ctx, double* out);

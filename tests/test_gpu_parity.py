@@ -11,6 +11,7 @@ the oracle and the reference's golden fields.
 """
 import ctypes as C
 import os
+import time
 
 import numpy as np
 import pytest
@@ -300,6 +301,46 @@ def test_status_forced_from_host(rsg_lib, gpu_ctx, meshes):
     r, _ = P.solve()
     P.close()
     assert r.diverged == 1 and r.final_iter == 0
+
+
+def test_admission_hold(rsg_lib, gpu_ctx, meshes, rs):
+    """rsg_hold_admission: tickets submitted under a hold stay pending while
+    any hold is left (holds nest, one release each); released, they run with
+    the results of solves submitted without a hold.  Releasing with none held
+    and an out-of-range argument are errors."""
+    lib, ctx = rsg_lib, gpu_ctx
+    s = settings_for("channel_bounded", max_iterations=60, convergence_tol=1e-12)
+    P = Problem(lib, ctx, meshes["channel_bounded"], "channel_bounded", s)
+    try:
+        r0, a0 = P.solve()
+        held = 0
+        try:
+            for _ in range(2):
+                chk(lib, ctx, lib.rsg_hold_admission(ctx, 1))
+                held += 1
+            t = (C.c_int64 * 2)(P.submit(), P.submit())
+            st = (C.c_int32 * 2)()
+            for _ in range(2):
+                time.sleep(0.3)
+                chk(lib, ctx, lib.rsg_poll(ctx, t, 2, st))
+                assert list(st) == [0, 0], "admitted while held"
+                if held == 2:
+                    chk(lib, ctx, lib.rsg_hold_admission(ctx, 0))
+                    held -= 1
+        finally:
+            for _ in range(held):
+                lib.rsg_hold_admission(ctx, 0)
+        for x in t:
+            r, a = P.fetch(x)
+            assert (r.final_iter, r.converged) == (r0.final_iter, r0.converged)
+            for k in ("U", "p", "k", "omega", "nuT"):
+                assert np.array_equal(a[k], a0[k]), k
+        assert lib.rsg_hold_admission(ctx, 0) == -4
+        assert lib.rsg_hold_admission(ctx, 2) == -1
+    finally:
+        P.close()
+    with pytest.raises(RuntimeError):
+        rs.engine_hold_admission(False)
 
 
 def test_forward_model_end_to_end(rs, oracle):
