@@ -305,9 +305,10 @@ def test_status_forced_from_host(rsg_lib, gpu_ctx, meshes):
 
 def test_admission_hold(rsg_lib, gpu_ctx, meshes, rs):
     """rsg_hold_admission: tickets submitted under a hold stay pending while
-    any hold is left (holds nest, one release each); released, they run with
-    the results of solves submitted without a hold.  Releasing with none held
-    and an out-of-range argument are errors."""
+    any hold is left (holds nest, one release each); released, they start
+    together in one launch (60 iterations each, 200 per launch) and give the
+    results of solves submitted without a hold.  Releasing with none held and
+    an out-of-range argument are errors."""
     lib, ctx = rsg_lib, gpu_ctx
     s = settings_for("channel_bounded", max_iterations=60, convergence_tol=1e-12)
     P = Problem(lib, ctx, meshes["channel_bounded"], "channel_bounded", s)
@@ -327,9 +328,15 @@ def test_admission_hold(rsg_lib, gpu_ctx, meshes, rs):
                 if held == 2:
                     chk(lib, ctx, lib.rsg_hold_admission(ctx, 0))
                     held -= 1
+            st0 = (C.c_double * 5)()
+            chk(lib, ctx, lib.rsg_stats(ctx, st0))
         finally:
             for _ in range(held):
                 lib.rsg_hold_admission(ctx, 0)
+        chk(lib, ctx, lib.rsg_wait(ctx, t, 2, C.c_double(600.0)))
+        st1 = (C.c_double * 5)()
+        chk(lib, ctx, lib.rsg_stats(ctx, st1))
+        assert st1[0] - st0[0] == 1 and st1[1] - st0[1] == 2, (list(st0), list(st1))
         for x in t:
             r, a = P.fetch(x)
             assert (r.final_iter, r.converged) == (r0.final_iter, r0.converged)
