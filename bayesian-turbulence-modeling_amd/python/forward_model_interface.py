@@ -40,12 +40,18 @@ class ForwardModelBase(ABC):
         return None
 
 
+_META = (("simple_iters", "simple_iters"), ("log_lik", "log_lik"), ("latency_s", "latency"))
+
+
 def _wrap(result) -> EvaluationResult:
+    """Reference python/forward_model_interface.py:116-124: predictions,
+    converged = non-empty and all finite, status as text.  Only .predictions
+    and .status are required of the C++ result; the engine's iteration count,
+    log-likelihood and latency go to metadata where the result has them."""
     preds = list(result.predictions) if result.predictions else []
     converged = bool(preds) and all(np.isfinite(p) for p in preds)
-    return EvaluationResult(predictions=preds, converged=converged, status=str(result.status),
-                            metadata={"simple_iters": result.simple_iters, "log_lik": result.log_lik,
-                                      "latency_s": result.latency})
+    meta = {key: getattr(result, attr) for key, attr in _META if hasattr(result, attr)}
+    return EvaluationResult(predictions=preds, converged=converged, status=str(result.status), metadata=meta)
 
 
 def _param_names(param_set) -> List[str]:

@@ -55,3 +55,39 @@ def test_parameter_names_fallbacks():
     assert FMI._param_names({"a1": 0.31}) == ["theta_0", "theta_1"]
     assert FMI._param_names(_Set()) == ["theta_0", "theta_1", "theta_2"]
     assert FMI._param_names(SimpleNamespace(active_names=lambda: ["a1"])) == ["a1"]
+
+
+class _BareResult:
+    """A C++ result with only the two attributes the protocol requires."""
+    def __init__(self, preds, status):
+        self.predictions, self.status = preds, status
+
+
+class _BareModel:
+    def __init__(self, fail=False):
+        self.fail = fail
+
+    def evaluate(self, theta):
+        return _BareResult([], "diverged") if self.fail else _BareResult([0.01 * t for t in theta], "converged")
+
+
+@pytest.mark.parametrize("cls", [FMI.InternalIncompressibleForwardModel, FMI.InternalCompressibleForwardModel])
+def test_wrapper_needs_only_predictions_and_status(cls):
+    ok = cls(_BareModel(), {"names": ["a1", "betaStar"]}).evaluate([0.31, 0.09])
+    assert isinstance(ok, FMI.EvaluationResult)
+    assert ok.predictions == pytest.approx([0.0031, 0.0009]) and ok.converged and ok.status == "converged"
+    assert ok.metadata == {}
+    bad = cls(_BareModel(fail=True), {}).evaluate([0.31, 0.09])
+    assert bad.predictions == [] and not bad.converged and bad.status == "diverged"
+
+
+def test_evaluation_result_defaults_and_truth():
+    r = FMI.EvaluationResult([1.0, 2.0])
+    assert r.status == "ok" and r.converged is True and r.metadata == {} and bool(r)
+    assert not FMI.EvaluationResult([], converged=True)
+    assert not FMI.EvaluationResult([0.003], converged=False)
+
+
+def test_base_is_abstract():
+    with pytest.raises(TypeError):
+        FMI.ForwardModelBase()
