@@ -170,6 +170,7 @@ struct Ticket {
     std::vector<double> warmBnd;     // U(3nB) p k om
     int warmTurb = 0;
     int precision = 0;
+    bool noFields = false;           // RSG_INIT_NO_FIELDS: fields stay on the device
     int state = RSG_TICKET_PENDING;
     // results (host copies)
     RsgScalars sc{};
@@ -389,18 +390,39 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
 }
 
 // Copies a finished member's state to host (ctx->mu not held).
+// k device arrays of n doubles each to dst[q * n ..]: one copy of their span
+// when they lie at a constant stride (the member workspace carves them back
+// to back, 256-byte aligned), else one copy per array.  A retirement costs a
+// few synchronous copies instead of seventeen.
+hipError_t gather_to_host(double* const* src, int k, int n, double* dst) {
+    const ptrdiff_t st = k > 1 ? src[1] - src[0] : n;
+    bool strided = st >= n;
+    for (int q = 1; q < k && strided; ++q) strided = src[q] == src[0] + q * st;
+    if (strided) {
+        std::vector<double> span((size_t)(k - 1) * st + n);
+        const hipError_t e = hipMemcpy(span.data(), src[0], 8 * span.size(), hipMemcpyDeviceToHost);
+        for (int q = 0; q < k && e == hipSuccess; ++q)
+            memcpy(dst + (size_t)q * n, span.data() + (size_t)q * st, 8 * (size_t)n);
+        return e;
+    }
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < k && e == hipSuccess; ++q)
+        e = hipMemcpy(dst + (size_t)q * n, src[q], 8 * (size_t)n, hipMemcpyDeviceToHost);
+    return e;
+}
+
 hipError_t collect(Ticket& T, const rsg::PrepMesh& pm, const rsg_settings& st, int nObs) {
     const int n = pm.nC, nB = pm.nB;
     RsgMemberDev& M = T.M;
-    T.cells.assign(11 * (size_t)n, 0.0);
-    T.bnd.assign(6 * (size_t)(nB > 0 ? nB : 1), 0.0);
-    double* csrc[] = {M.Ux, M.Uy, M.Uz, M.p, M.k, M.om, M.nuT, M.F1, M.F2, M.Pk, M.CDkw};
     hipError_t e = hipSuccess;
-    for (int q = 0; q < 11 && e == hipSuccess; ++q)
-        e = hipMemcpy(T.cells.data() + (size_t)q * n, csrc[q], 8 * (size_t)n, hipMemcpyDeviceToHost);
-    double* bsrc[] = {M.Ubx, M.Uby, M.Ubz, M.pb, M.kb, M.omb};
-    for (int q = 0; q < 6 && e == hipSuccess && nB > 0; ++q)
-        e = hipMemcpy(T.bnd.data() + (size_t)q * nB, bsrc[q], 8 * (size_t)nB, hipMemcpyDeviceToHost);
+    if (!T.noFields) {
+        T.cells.assign(11 * (size_t)n, 0.0);
+        T.bnd.assign(6 * (size_t)(nB > 0 ? nB : 1), 0.0);
+        double* csrc[] = {M.Ux, M.Uy, M.Uz, M.p, M.k, M.om, M.nuT, M.F1, M.F2, M.Pk, M.CDkw};
+        e = gather_to_host(csrc, 11, n, T.cells.data());
+        double* bsrc[] = {M.Ubx, M.Uby, M.Ubz, M.pb, M.kb, M.omb};
+        if (e == hipSuccess && nB > 0) e = gather_to_host(bsrc, 6, nB, T.bnd.data());
+    }
     if (e == hipSuccess && st.record_history && M.hist) {
         const int len = std::min(T.sc.histLen, M.histCap);
         T.hist.resize(6 * (size_t)std::max(len, 1));
@@ -973,7 +995,9 @@ int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const
         T->init[0] = init->U[0]; T->init[1] = init->U[1]; T->init[2] = init->U[2];
         T->init[3] = init->p; T->init[4] = init->k; T->init[5] = init->omega;
         T->precision = precision;
-        if (init->kind == 1) {
+        T->noFields = (init->kind & RSG_INIT_NO_FIELDS) != 0;
+        const int kind = init->kind & ~RSG_INIT_NO_FIELDS;
+        if (kind == 1) {
             const int n = pm.nC, nB = pm.nB;
             if (!init->U_cells || !init->p_cells || !init->k_cells || !init->om_cells || !init->nuT ||
                 !init->F1 || !init->F2 || !init->Pk || !init->CDkw ||
@@ -993,8 +1017,9 @@ int rsg_submit(rsg_context* ctx, int32_t problem_id, const double* coeffs, const
                 memcpy(T->warmBnd.data() + 4 * (size_t)nB, init->k_bnd, 8 * (size_t)nB);
                 memcpy(T->warmBnd.data() + 5 * (size_t)nB, init->om_bnd, 8 * (size_t)nB);
             }
-        } else if (init->kind != 0) {
-            return set_err(ctx, RSG_ERR_ARG, "init kind must be 0 (uniform) or 1 (fields)");
+        } else if (kind != 0) {
+            return set_err(ctx, RSG_ERR_ARG, "init kind must be 0 (uniform) or 1 (fields), optionally with "
+                                             "RSG_INIT_NO_FIELDS");
         }
         T->id = ctx->nextTicket++;
         T->tSubmit = now_s(ctx->t0);
@@ -1166,6 +1191,10 @@ int rsg_fetch(rsg_context* ctx, int64_t ticket, rsg_result* out) {
     out->diverged = T.sc.status == RSGM_DIVERGED;
     out->turb_established = T.sc.turbEstablished;
     for (int i = 0; i < 6; ++i) out->inner_iters[i] = T.sc.innerIters[i];
+    if (T.noFields && (out->U || out->p || out->k || out->omega || out->nuT || out->F1 || out->F2 || out->Pk ||
+                       out->CDkw || out->U_bnd || out->p_bnd || out->k_bnd || out->om_bnd))
+        return set_err(ctx, RSG_ERR_ARG, "fields of a solve submitted with RSG_INIT_NO_FIELDS were not collected: "
+                                         "pass NULL field pointers");
     const double* c = T.cells.data();
     if (out->U) for (int i = 0; i < n; ++i) { out->U[3 * i] = c[i]; out->U[3 * i + 1] = c[n + i]; out->U[3 * i + 2] = c[2 * n + i]; }
     double* dst[] = {out->p, out->k, out->omega, out->nuT, out->F1, out->F2, out->Pk, out->CDkw};

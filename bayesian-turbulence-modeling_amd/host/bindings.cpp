@@ -320,6 +320,10 @@ PYBIND11_MODULE(rans_sst_py, m) {
             else if (p == "mixed") fm.setPrecision(RSG_PREC_MIXED);
             else throw std::invalid_argument("precision must be 'fp64' or 'mixed'");
         })
+        .def("set_collect_fields", &ForwardModel::setCollectFields, py::arg("on"),
+             "False: later solves keep their fields on the device and return predictions, log-likelihood and "
+             "status only (no last_fields, no warm-start cache entry).")
+        .def("collect_fields", &ForwardModel::collectFields)
         .def("cache_size", [](ForwardModel& fm) { return fm.cache().size(); })
         .def("clear_cache", [](ForwardModel& fm) { fm.cache().clear(); });
 

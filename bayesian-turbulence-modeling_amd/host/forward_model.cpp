@@ -207,9 +207,11 @@ int64_t ForwardModel::submit(const std::vector<double>& theta, bool useCache) {
         init.U_bnd = f.Ub.data(); init.p_bnd = f.pb.data(); init.k_bnd = f.kb.data(); init.om_bnd = f.omb.data();
         init.turb_established = f.turbEstablished ? 1 : 0;
     }
+    if (!collectFields_) init.kind |= RSG_INIT_NO_FIELDS;
     int64_t t = -1;
     int rc = rsg_submit(c, problemId_, coeffs.data(), &init, precision_, &t);   // copies the warm state
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+    if (!collectFields_) noFieldTickets_.insert(t);
     return t;
 }
 
@@ -217,14 +219,21 @@ EvaluationResult ForwardModel::complete(int64_t ticket, const std::vector<double
     rsg_context* c = Engine::get().ctx();
     int rc = rsg_wait(c, &ticket, 1, -1.0);
     if (rc != RSG_OK) throw std::runtime_error(engineError(c, rc));
+    bool fields;
+    {
+        std::lock_guard<std::mutex> l(mu_);
+        fields = noFieldTickets_.erase(ticket) == 0;
+    }
     FlowFields f;
-    f.resize(mesh_.nCells(), mesh_.nFaces() - mesh_.nInternalFaces());
     ConvergenceInfo conv;
     if (settings_.recordHistory) conv.history.assign(6 * (size_t)std::max(settings_.maxIterations, 1), 0.0);
     rsg_result r{};
-    r.U = f.U.data(); r.p = f.p.data(); r.k = f.k.data(); r.omega = f.omega.data(); r.nuT = f.nuT.data();
-    r.F1 = f.F1.data(); r.F2 = f.F2.data(); r.Pk = f.Pk.data(); r.CDkw = f.CDkw.data();
-    r.U_bnd = f.Ub.data(); r.p_bnd = f.pb.data(); r.k_bnd = f.kb.data(); r.om_bnd = f.omb.data();
+    if (fields) {
+        f.resize(mesh_.nCells(), mesh_.nFaces() - mesh_.nInternalFaces());
+        r.U = f.U.data(); r.p = f.p.data(); r.k = f.k.data(); r.omega = f.omega.data(); r.nuT = f.nuT.data();
+        r.F1 = f.F1.data(); r.F2 = f.F2.data(); r.Pk = f.Pk.data(); r.CDkw = f.CDkw.data();
+        r.U_bnd = f.Ub.data(); r.p_bnd = f.pb.data(); r.k_bnd = f.kb.data(); r.om_bnd = f.omb.data();
+    }
     r.history = conv.history.empty() ? nullptr : conv.history.data();
     r.history_cap = (int32_t)(conv.history.size() / 6);
     std::vector<double> pred(obsOp_.nObs());
@@ -253,12 +262,14 @@ EvaluationResult ForwardModel::complete(int64_t ticket, const std::vector<double
         return res;
     } else if (conv.converged) {
         res.status = EvaluationStatus::Converged;
-        if (storeCache) cache_.store(theta, f);
+        if (storeCache && fields) cache_.store(theta, f);
     } else {
         res.status = EvaluationStatus::Unconverged;
     }
-    lastFields_ = std::move(f);
-    hasLastFields_ = true;
+    // a solve whose fields stayed on the device leaves no last fields (an
+    // earlier solve's would not be this theta's)
+    if (fields) lastFields_ = std::move(f);
+    hasLastFields_ = fields;
     // ForwardModel.cpp:95-97, evaluated on the device at the end of the solve
     if (r.n_predictions != obsOp_.nObs()) throw std::runtime_error("device observables missing");
     res.predictions = std::move(pred);

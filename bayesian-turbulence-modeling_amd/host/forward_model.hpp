@@ -10,6 +10,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -127,6 +128,11 @@ public:
     const Mesh& mesh() const { return mesh_; }
     void setPrecision(int p) { precision_ = p; }
     int precision() const { return precision_; }
+    // false: solves submitted afterwards leave their fields on the device
+    // (RSG_INIT_NO_FIELDS); evaluate returns predictions, log-likelihood and
+    // status only, and neither last_fields nor the warm-start cache is updated
+    void setCollectFields(bool on) { std::lock_guard<std::mutex> l(mu_); collectFields_ = on; }
+    bool collectFields() const { return collectFields_; }
 
     // Submission split for servers: returns an engine ticket.
     int64_t submit(const std::vector<double>& theta, bool useCache);
@@ -147,6 +153,8 @@ private:
     bool hasLastFields_ = false;
     ConvergenceInfo lastConv_;
     int precision_ = RSG_PREC_FP64;
+    bool collectFields_ = true;
+    std::set<int64_t> noFieldTickets_;   // submitted with RSG_INIT_NO_FIELDS, not yet completed
     int meshId_ = -1, problemId_ = -1;
     uint64_t ctxGen_ = 0;         // engine generation the ids belong to
     std::mutex mu_;

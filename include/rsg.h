@@ -148,8 +148,16 @@ typedef struct rsg_settings {
 
 /* Initial state.  kind 0: uniform (SIMPLESolver::initUniform); kind 1: full
  * fields (warm start from a cached converged state, ForwardModel.cpp:56-60). */
+/* OR into rsg_init.kind (rsg_submit): the solve's fields stay on the device
+ * and are dropped at retirement; only status, iteration counts, history and
+ * the device-evaluated predictions and log-likelihood are kept.  rsg_fetch of
+ * such a ticket returns RSG_ERR_ARG unless every field pointer is NULL.  For
+ * callers that need theta -> log-likelihood only (MCMC chains): no field copy
+ * to the host at retirement, which the engine thread otherwise makes between
+ * launches. */
+#define RSG_INIT_NO_FIELDS 0x100
 typedef struct rsg_init {
-    int32_t kind;
+    int32_t kind;                 /* 0: uniform state, 1: fields (warm start); | RSG_INIT_NO_FIELDS */
     double U[3], p, k, omega;
     const double *U_cells, *p_cells, *k_cells, *om_cells;   /* [3n], [n] ... */
     const double *nuT, *F1, *F2, *Pk, *CDkw;                  /* [n] */

@@ -388,6 +388,60 @@ def test_forward_model_end_to_end(rs, oracle):
     assert 0.0 < rel(f2["U"].ravel(), f["U"].ravel()) < 0.05
 
 
+def test_fields_left_on_device(rs, rsg_lib, gpu_ctx, meshes):
+    """RSG_INIT_NO_FIELDS: a solve whose fields stay on the device returns the
+    status, iterations, history and device-evaluated predictions of the same
+    solve with fields; rsg_fetch with field pointers is refused.  Through
+    rans_sst_py (set_collect_fields(False)): the same predictions and
+    log-likelihood, no last_fields and no warm-start cache entry."""
+    lib, ctx = rsg_lib, gpu_ctx
+    s = settings_for("channel_bounded", max_iterations=60, convergence_tol=1e-12, record_history=1)
+    P = Problem(lib, ctx, meshes["channel_bounded"], "channel_bounded", s)
+    try:
+        r0, a0 = P.solve()
+        init = A.Init()
+        init.kind = 0x100; init.U[0] = 1.0; init.k = 1e-4; init.omega = 1.0
+        t = C.c_int64()
+        chk(lib, ctx, lib.rsg_submit(ctx, P.pid, D(A.DEFAULT_COEFFS), C.byref(init), 0, C.byref(t)))
+        chk(lib, ctx, lib.rsg_wait(ctx, (C.c_int64 * 1)(t.value), 1, C.c_double(600.0)))
+        r, a = A.new_result(P.msh.nC, P.msh.nF - P.msh.nIF, s.max_iterations)
+        assert lib.rsg_fetch(ctx, t, C.byref(r)) == -1
+        bare = A.Result()
+        hist = np.zeros(6 * s.max_iterations)
+        bare.history = A.dptr(hist); bare.history_cap = s.max_iterations
+        chk(lib, ctx, lib.rsg_fetch(ctx, t, C.byref(bare)))
+        lib.rsg_release(ctx, t)
+        assert (bare.status, bare.final_iter, bare.converged) == (2, r0.final_iter, r0.converged)
+        assert np.array_equal(hist[:6 * bare.history_len], a0["history"][:6 * r0.history_len])
+        bad = A.Init(); bad.kind = 2
+        assert lib.rsg_submit(ctx, P.pid, D(A.DEFAULT_COEFFS), C.byref(bad), 0, C.byref(t)) == -1
+    finally:
+        P.close()
+
+    y = rs.Mesh.geometric_channel_nodes(32, 1.06, 1.0)
+    mesh = rs.Mesh.make_channel_periodic_2d(np.linspace(0.0, 1.0, 5), np.array(y))
+    mesh.compute_wall_distance()
+    st = rs.SolverSettings()
+    st.max_iterations = 30000; st.convergence_tol = 1e-6; st.verbose = False
+    st.body_force = rs.Vec3(1.0, 0.0, 0.0)
+    obs = rs.ObservationOperator()
+    obs.add_skin_friction("bottom_wall", rs.Vec3(0.5, 0.0, 0.5), 0.005, 0.001, 1.0)
+    obs.add_velocity_profile(rs.Vec3(0.5, 1.0, 0.5), 0, 20.0, 1.0)
+    bcs = rs.FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+    ps = rs.InferenceParameterSet.a1_betaStar()
+    with_f = rs.ForwardModel(mesh, ps, obs, bcs, 1.0 / 180.0, st)
+    without = rs.ForwardModel(mesh, ps, obs, bcs, 1.0 / 180.0, st)
+    without.set_collect_fields(False)
+    assert with_f.collect_fields() and not without.collect_fields()
+    ra, rb = with_f.evaluate([0.31, 0.09]), without.evaluate([0.31, 0.09])
+    print("with fields", ra.status, ra.simple_iters, list(ra.predictions), flush=True)
+    assert (str(ra.status), ra.simple_iters) == (str(rb.status), rb.simple_iters)
+    assert list(ra.predictions) == list(rb.predictions) and ra.log_lik == rb.log_lik
+    assert str(ra.status).split(".")[-1] == "Converged"
+    assert with_f.has_last_fields() and not without.has_last_fields()
+    assert (with_f.cache_size(), without.cache_size()) == (1, 0)
+
+
 @pytest.mark.parametrize("path", ["member", "grid"])
 def test_gauss_seidel_trajectory_vs_oracle(rsg_lib, gpu_ctx, oracle, meshes, path):
     """Gauss-Seidel (LinearSolver.hpp:258-299) as the momentum and turbulence
