@@ -289,16 +289,17 @@ void member_init_scalars(RsgScalars& s, int turbEstablished) {
     s.turbEstablished = turbEstablished;
 }
 
-// The compressible fields of a member-kernel solve (RsgCompDev's arrays).
+// The compressible fields of a member-kernel solve (RsgCompDev's arrays);
+// T, rho and p_mech first and back to back, so that retirement copies them at once.
 void carve_comp(rsg::Carver& cv, int nC, int nB, RsgCompDev& Q) {
     const size_t n = nC, nb = std::max(nB, 1);
-    Q.T = cv.take<double>(n); Q.Tb = cv.take<double>(nb); Q.rho = cv.take<double>(n);
+    Q.T = cv.take<double>(n); Q.rho = cv.take<double>(n); Q.pMech = cv.take<double>(n);
+    Q.Tb = cv.take<double>(nb);
     Q.mu = cv.take<double>(n); Q.Told = cv.take<double>(n);
     Q.Ux0 = cv.take<double>(n); Q.Uy0 = cv.take<double>(n); Q.Uz0 = cv.take<double>(n);
     Q.mdu = cv.take<double>(n);
     for (int q = 0; q < 3; ++q) Q.gt[q] = cv.take<double>(n);
     Q.zeroB = cv.take<double>(nb);
-    Q.pMech = cv.take<double>(n);
 }
 
 // Admit one ticket (ctx->mu held).  Returns false when the arena is full.
@@ -443,9 +444,8 @@ hipError_t collect_comp(Ticket& T, const rsg::PrepMesh& pm) {
     const RsgCompDev& Q = T.compDev;
     T.compCells.assign(3 * n, 0.0);
     T.compTb.assign(std::max<size_t>(nB, 1), 0.0);
-    hipError_t e = hipMemcpy(T.compCells.data(), Q.T, 8 * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(T.compCells.data() + n, Q.rho, 8 * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(T.compCells.data() + 2 * n, Q.pMech, 8 * n, hipMemcpyDeviceToHost);
+    double* src[] = {Q.T, Q.rho, Q.pMech};
+    hipError_t e = gather_to_host(src, 3, (int)n, T.compCells.data());
     if (e == hipSuccess && nB > 0) e = hipMemcpy(T.compTb.data(), Q.Tb, 8 * nB, hipMemcpyDeviceToHost);
     RsgCompDev d;
     if (e == hipSuccess) e = hipMemcpy(&d, T.dComp, sizeof d, hipMemcpyDeviceToHost);
