@@ -289,6 +289,25 @@ void member_init_scalars(RsgScalars& s, int turbEstablished) {
     s.turbEstablished = turbEstablished;
 }
 
+// Host arrays src[q] (n doubles each) to device arrays dst[q]: one copy of
+// the device span when the destinations lie at a constant stride (the member
+// workspace carves them back to back, 256-byte aligned; the alignment gaps
+// inside the span belong to the same workspace), else one copy per array.
+hipError_t scatter_from_host(double* const* dst, int k, int n, const double* const* src) {
+    const ptrdiff_t st = k > 1 ? dst[1] - dst[0] : n;
+    bool strided = st >= n;
+    for (int q = 1; q < k && strided; ++q) strided = dst[q] == dst[0] + q * st;
+    if (strided) {
+        std::vector<double> span((size_t)(k - 1) * st + n, 0.0);
+        for (int q = 0; q < k; ++q) memcpy(span.data() + (size_t)q * st, src[q], 8 * (size_t)n);
+        return hipMemcpy(dst[0], span.data(), 8 * span.size(), hipMemcpyHostToDevice);
+    }
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < k && e == hipSuccess; ++q)
+        e = hipMemcpy(dst[q], src[q], 8 * (size_t)n, hipMemcpyHostToDevice);
+    return e;
+}
+
 // The compressible fields of a member-kernel solve (RsgCompDev's arrays);
 // T, rho and p_mech first and back to back, so that retirement copies them at once.
 void carve_comp(rsg::Carver& cv, int nC, int nB, RsgCompDev& Q) {
@@ -363,18 +382,18 @@ bool admit(rsg_context* ctx, Ticket& T, std::string& err) {
         // warm cells arrive as U interleaved (3n) then scalars; de-interleave on host
         std::vector<double> tmp(3 * (size_t)n);
         for (int c = 0; c < n; ++c) { tmp[c] = T.warmCells[3 * c]; tmp[n + c] = T.warmCells[3 * c + 1]; tmp[2 * n + c] = T.warmCells[3 * c + 2]; }
-        for (int q = 0; q < 3 && e == hipSuccess; ++q)
-            e = hipMemcpy(cdst[q], tmp.data() + (size_t)q * n, 8 * (size_t)n, hipMemcpyHostToDevice);
-        for (int q = 3; q < 11 && e == hipSuccess; ++q)
-            e = hipMemcpy(cdst[q], T.warmCells.data() + 3 * (size_t)n + (size_t)(q - 3) * n, 8 * (size_t)n, hipMemcpyHostToDevice);
-        if (nB > 0) {
+        const double* csrc[11];
+        for (int q = 0; q < 3; ++q) csrc[q] = tmp.data() + (size_t)q * n;
+        for (int q = 3; q < 11; ++q) csrc[q] = T.warmCells.data() + 3 * (size_t)n + (size_t)(q - 3) * n;
+        e = scatter_from_host(cdst, 11, n, csrc);
+        if (e == hipSuccess && nB > 0) {
             std::vector<double> tb(3 * (size_t)nB);
             for (int b = 0; b < nB; ++b) { tb[b] = T.warmBnd[3 * b]; tb[nB + b] = T.warmBnd[3 * b + 1]; tb[2 * nB + b] = T.warmBnd[3 * b + 2]; }
             double* bdst[] = {M.Ubx, M.Uby, M.Ubz, M.pb, M.kb, M.omb};
-            for (int q = 0; q < 3 && e == hipSuccess; ++q)
-                e = hipMemcpy(bdst[q], tb.data() + (size_t)q * nB, 8 * (size_t)nB, hipMemcpyHostToDevice);
-            for (int q = 3; q < 6 && e == hipSuccess; ++q)
-                e = hipMemcpy(bdst[q], T.warmBnd.data() + 3 * (size_t)nB + (size_t)(q - 3) * nB, 8 * (size_t)nB, hipMemcpyHostToDevice);
+            const double* bsrc[6];
+            for (int q = 0; q < 3; ++q) bsrc[q] = tb.data() + (size_t)q * nB;
+            for (int q = 3; q < 6; ++q) bsrc[q] = T.warmBnd.data() + 3 * (size_t)nB + (size_t)(q - 3) * nB;
+            e = scatter_from_host(bdst, 6, nB, bsrc);
         }
         T.warmCells.clear(); T.warmCells.shrink_to_fit();
         T.warmBnd.clear(); T.warmBnd.shrink_to_fit();
